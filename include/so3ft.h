@@ -1,0 +1,126 @@
+/*
+ * so3ft.h -- C ABI of the MI355X (gfx950) SO(3) Fourier transform library
+ * (libso3ft.so, built from pysofft_amd/csrc).
+ *
+ * This is the drop-in boundary for PySOFFT's f2py module `pysofft._soft.py`
+ * (pysofft/py_wrap.f90): each entry point below names the py_wrap procedure it
+ * replaces.  Differences of kind, not of meaning:
+ *   - every call returns an int status (SO3FT_OK or a negative code) and never
+ *     aborts; so3ft_last_error() returns a message for the calling thread;
+ *   - arrays are DEVICE pointers (hipMalloc'd or torch CUDA tensors) owned by the
+ *     caller; the library owns only its plan data and scratch;
+ *   - every transform takes the HIP stream to run on (NULL = default stream) and
+ *     returns after enqueueing (stream-ordered, like any HIP call).
+ *
+ * Layouts are the reference's:
+ *   coefficients : complex128 (or complex64 for the *_c64 entry points), length
+ *                  NC = (4bw^3-bw)/3, "mnl" order (pysofft/utils.f90:87-201)
+ *   SO(3) function: C-order (gamma, alpha, beta), shape (2bw,2bw,2bw)
+ *                  (docs/defs/so3_func.md:5-13), complex128 or float64
+ *   *_many       : batch-major, i.e. (n, NC) and (n, 2bw, 2bw, 2bw)
+ *   SH coefficients for correlation: complex128, SHTNS LMc order l(l+1)+m, bw^2
+ *
+ * A handle may be used from one stream at a time (scratch is per handle).
+ */
+#ifndef SO3FT_H
+#define SO3FT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int64_t so3ft_handle;
+
+enum {
+    SO3FT_OK = 0,
+    SO3FT_EINVAL = -1,    /* bad argument (bandwidth, lmax, size, null pointer) */
+    SO3FT_EHANDLE = -2,   /* unknown or destroyed handle                        */
+    SO3FT_EHIP = -3,      /* HIP runtime error (message in so3ft_last_error)    */
+    SO3FT_ENOMEM = -4,    /* device allocation failed                           */
+    SO3FT_EUNSUPPORTED = -5
+};
+
+/* recurrence types, pysofft/soft.f08:14-15 (py_wrap.f90:48-63) */
+int so3ft_get_kostelec_recurrence_val(void);
+int so3ft_get_risbo_recurrence_val(void);
+
+const char *so3ft_last_error(void);
+
+/* ---- lifecycle: py_wrap.f90:79-91 py_init_soft, :172-180 py_destroy,
+ *      :152-171 py_reset, :181-188 py_init_fft ---------------------------- */
+int so3ft_init(int64_t bw, int64_t lmax, int precompute_wigners, int init_ffts,
+               int64_t recurrence_type, int32_t fftw_flags, so3ft_handle *out);
+int so3ft_destroy(so3ft_handle h);
+int so3ft_reset(so3ft_handle h, int64_t bw, int64_t lmax, int precompute_wigners,
+                int init_ffts, int64_t recurrence_type, int32_t fftw_flags);
+int so3ft_init_fft(so3ft_handle h, int use_real_fft);
+
+/* ---- getters / setters: py_wrap.f90:100-151 ------------------------------- */
+int so3ft_get_bw(so3ft_handle h, int64_t *bw);
+int so3ft_get_lmax(so3ft_handle h, int64_t *lmax);
+int so3ft_set_lmax(so3ft_handle h, int64_t lmax);
+int so3ft_get_recurrence_type(so3ft_handle h, int64_t *rt);
+int so3ft_wigners_are_precomputed(so3ft_handle h, int *yes);
+int so3ft_get_fftw_flags(so3ft_handle h, int32_t *flags);
+
+/* ---- complex transforms: py_wrap.f90:236-257, 280-301 --------------------- */
+int so3ft_isoft(so3ft_handle h, const void *coeff, void *so3func, void *stream);   /* py_isoft */
+int so3ft_soft(so3ft_handle h, const void *so3func, void *coeff, void *stream);    /* py_soft  */
+int so3ft_isoft_many(so3ft_handle h, const void *coeffs, void *so3funcs, int64_t n, void *stream);
+int so3ft_soft_many(so3ft_handle h, const void *so3funcs, void *coeffs, int64_t n, void *stream);
+
+/* ---- real transforms: py_wrap.f90:258-279, 302-323 ------------------------ */
+int so3ft_irsoft(so3ft_handle h, const void *coeff, void *so3func_real, void *stream);  /* py_irsoft */
+int so3ft_rsoft(so3ft_handle h, const void *so3func_real, void *coeff, void *stream);   /* py_rsoft  */
+int so3ft_irsoft_many(so3ft_handle h, const void *coeffs, void *so3funcs_real, int64_t n, void *stream);
+int so3ft_rsoft_many(so3ft_handle h, const void *so3funcs_real, void *coeffs, int64_t n, void *stream);
+
+/* ---- stage-level entry points: py_wrap.f90:190-233 -------------------------
+ * fft_array has the SO(3)-function shape; entry (i_n, i_m, beta) holds
+ * F(beta, m, n) with i = m>=0 ? m : 2bw+m (pysofft/utils.f90:45-57).
+ * The inverse DWT writes every (m,n) column it owns and zeros the others. */
+int so3ft_inverse_wigner_trf_cmplx(so3ft_handle h, const void *coeff, void *fft_array, void *stream);
+int so3ft_forward_wigner_trf_cmplx(so3ft_handle h, const void *fft_array, void *coeff, void *stream);
+
+/* ---- py_wrap.f90:417-436 py_fft / py_ifft: 2-D DFT over (alpha,gamma) of every
+ * beta slice (FFTW_FORWARD / FFTW_BACKWARD sign), scaled by 1/(2bw) ---------- */
+int so3ft_fft(so3ft_handle h, const void *f1, void *f2, void *stream);
+int so3ft_ifft(so3ft_handle h, const void *f1, void *f2, void *stream);
+
+/* ---- py_wrap.f90:325-344 integrate_over_so3 (synchronises the stream) ----- */
+int so3ft_integrate_over_so3_cmplx(so3ft_handle h, const void *f, double *re, double *im, void *stream);
+int so3ft_integrate_over_so3_real(so3ft_handle h, const void *f, double *out, void *stream);
+
+/* ---- rotational cross-correlation: py_wrap.f90:368-378, 379-391 ----------- */
+int so3ft_cross_correlation_ylm_cmplx(so3ft_handle h, const void *f_lm, const void *g_lm, void *cc,
+                                      void *stream);
+/* radial_limits: 0-based inclusive [lo, hi] shell range (py_wrap passes 1-based);
+ * radial_sampling_points is a HOST array of length n_radial. */
+int so3ft_cross_correlation_ylm_cmplx_3d(so3ft_handle h, const void *f_lms, const void *g_lms, void *cc,
+                                         const double *radial_sampling_points, int64_t n_radial,
+                                         int64_t r_lo, int64_t r_hi, void *stream);
+
+/* ---- Wigner-D rotation of SH coefficient sets (LMc layout, bw^2 each) by ZYZ Euler
+ * angles, pysofft/soft.py:713-764 (numpy there; no f2py counterpart):
+ *   y'^l_n = e^{-i n gamma} sum_m y^l_m e^{-i m alpha} d^l_{n,m}(beta).
+ * euler is a DEVICE (nrot, 3) float64 array.  paired = 0: out (nrot, nset, bw^2), every
+ * set by every rotation; paired = 1: nset == nrot, set i by rotation i, out (nrot, bw^2). */
+int so3ft_rotate_ylm_cmplx(int64_t bw, const void *ylms, int64_t nset, const void *euler, int64_t nrot,
+                           int paired, void *out, void *stream);
+
+/* ---- measurement hook (no counterpart in the reference): launches one kernel of
+ * the pipeline `reps` times on `stream` between two HIP events and returns the mean
+ * duration of one launch.  which: 0 inverse DWT (a = coeffs, b = T x (2bw)^3 scratch),
+ * 1 forward DWT (a = scratch, b = coeffs), 2 FFT pass along alpha (a -> b),
+ * 3 FFT pass along gamma (a -> b).  T transforms per launch. */
+int so3ft_bench_kernel(so3ft_handle h, int which, int reps, void *a, void *b, int64_t T, void *stream,
+                       float *ms_per_launch);
+
+/* ---- utils: utils.f90:471-505 enforce_real_sym_mnl(_many), in place on device */
+int so3ft_enforce_real_sym_mnl(int64_t bw, void *coeffs, int64_t n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SO3FT_H */

@@ -61,9 +61,10 @@ contains
   subroutine ref_dlml_all(bw, out) bind(C, name="ref_dlml_all")
     integer(c_int64_t), value :: bw
     real(c_double), intent(out) :: out(2*bw, (bw*(bw + 1))/2)
-    real(c_double) :: trig(2*bw, 3)
+    real(c_double), allocatable :: trig(:, :)
     integer(c_int64_t) :: b
     b = bw
+    allocate(trig(2*bw, 3))
     trig = create_trig_samples(b)
     out = compute_all_dlml_l_contiguous(b, trig(:, 2), trig(:, 3), .true.)
   end subroutine
@@ -72,10 +73,10 @@ contains
   subroutine ref_genwig(m1, m2, bw, out) bind(C, name="ref_genwig")
     integer(c_int64_t), value :: m1, m2, bw
     real(c_double), intent(out) :: out(2*bw, bw - m2)
-    real(c_double) :: trig(2*bw, 3), dlml(2*bw, (bw*(bw + 1))/2)
-    real(c_double) :: w(bw - m2, 2*bw)
+    real(c_double), allocatable :: trig(:, :), dlml(:, :), w(:, :)
     integer(c_int64_t) :: b, a1, a2
     b = bw; a1 = m1; a2 = m2
+    allocate(trig(2*bw, 3), dlml(2*bw, (bw*(bw + 1))/2), w(bw - m2, 2*bw))
     trig = create_trig_samples(b)
     dlml = compute_all_dlml_l_contiguous(b, trig(:, 2), trig(:, 3), .true.)
     w = genwig_l2_trsp(a1, a2, b, trig(:, 1), dlml(:, triangular_to_flat_index(a1, a2, b)), .true.)

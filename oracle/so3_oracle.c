@@ -321,22 +321,25 @@ void or_plan_seeds(const oplan *P, double *s) { memcpy(s, P->seeds, sizeof(doubl
 /* discrete Wigner transforms, complex                                        */
 /* ------------------------------------------------------------------------- */
 
-/* one variant (m,n) of the inverse DWT: F(beta',m,n) = s0 * sum_l c_l [(-1)^l] d~^l_{pq}(beta) */
+/* one variant (m,n) of the inverse DWT: F(beta',m,n) = s0 * sum_l c_l [(-1)^l] d~^l_{pq}(beta)
+ * (l outer, beta inner: the row-wise form of the reference's matmul(coeff_part, wig_mat)) */
 static void inv_variant(const oplan *P, const zc *c, zc *F, i64 m, i64 n, const double *wig, i64 L,
                         double s0, int lsign, int rev)
 {
     i64 N = P->N, bw = P->bw;
     const zc *cs = c + P->slice[mnidx(m, n, bw)];
     i64 im = wrap(m, N), in = wrap(n, N);
-    for (i64 k = 0; k < N; ++k) {
-        zc acc = 0;
-        for (i64 i = 0; i < L; ++i) {
-            i64 l = bw - L + i;
-            double sl = (lsign && (l & 1)) ? -1.0 : 1.0;
-            acc += cs[i] * (sl * s0) * wig[i * N + k];
-        }
-        F[FIDX(in, im, rev ? N - 1 - k : k)] = acc;
+    double *ar = calloc(2 * N, sizeof(double)), *ai = ar + N;
+    for (i64 i = 0; i < L; ++i) {
+        i64 l = bw - L + i;
+        double sl = (lsign && (l & 1)) ? -1.0 : 1.0;
+        double cr = creal(cs[i]) * (sl * s0), ci = cimag(cs[i]) * (sl * s0);
+        const double *wr = wig + i * N;
+        for (i64 k = 0; k < N; ++k) { ar[k] += cr * wr[k]; ai[k] += ci * wr[k]; }
     }
+    zc *Fc = F + FIDX(in, im, 0);
+    for (i64 k = 0; k < N; ++k) Fc[rev ? N - 1 - k : k] = ar[k] + I * ai[k];
+    free(ar);
 }
 
 /* soft.f08:534-631 inverse_wigner_loop_body_cmplx_kostelec */
@@ -385,13 +388,21 @@ static void fwd_variant(const oplan *P, const zc *F, zc *c, i64 m, i64 n, const 
     i64 N = P->N, bw = P->bw;
     zc *cs = c + P->slice[mnidx(m, n, bw)];
     i64 im = wrap(m, N), in = wrap(n, N);
+    double *wr = malloc(sizeof(double) * 2 * N), *wi = wr + N;
+    const zc *Fc = F + FIDX(in, im, 0);
+    for (i64 k = 0; k < N; ++k) {
+        zc v = Fc[rev ? N - 1 - k : k];
+        wr[k] = P->w[k] * creal(v); wi[k] = P->w[k] * cimag(v);
+    }
     for (i64 i = 0; i < L; ++i) {
         i64 l = bw - L + i;
-        zc acc = 0;
-        for (i64 k = 0; k < N; ++k) acc += (P->w[k] * F[FIDX(in, im, rev ? N - 1 - k : k)]) * wig[i * N + k];
+        const double *d = wig + i * N;
+        double ar = 0, ai = 0;
+        for (i64 k = 0; k < N; ++k) { ar += wr[k] * d[k]; ai += wi[k] * d[k]; }
         double sl = (lsign && (l & 1)) ? -1.0 : 1.0;
-        cs[i] = acc * (s0 * sl);
+        cs[i] = (ar + I * ai) * (s0 * sl);
     }
+    free(wr);
 }
 
 /* soft.f08:937-1023 forward_wigner_loop_body_cmplx_kostelec */

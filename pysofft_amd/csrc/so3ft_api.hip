@@ -1,0 +1,632 @@
+// C ABI of libso3ft (include/so3ft.h): plan objects and the transform pipelines.
+//
+// A plan (the counterpart of the reference's so3ft instance, pysofft/soft.f08:16-126)
+// holds, on the device: cos(beta_k), the Driscoll-Healy weights, the seeds
+// d~^q_{p,q}(beta_k) of the Kostelec recurrence, the mnl slice table, the
+// order-pair work list and the FFT twiddles; plus one lazily allocated complex
+// scratch array for the DWT <-> FFT intermediate (like fft_c2c_in, soft.f08:30).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/so3ft.h"
+#include "so3ft_internal.h"
+
+namespace {
+
+using namespace so3;
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(SO3FT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+struct Plan {
+    int64_t bw = 0, lmax = 0, N = 0, nc = 0, npair = 0;
+    int precompute = 0, init_ffts = 0;
+    int64_t recurrence = 0;
+    int32_t fftw_flags = 0;
+    int device = 0;
+    DeviceTables tb;
+    FftDesc fft;
+    double2 *scratch = nullptr;
+    int64_t scratch_T = 0;
+    double *red = nullptr;  // integration partials + result
+};
+
+std::mutex g_mu;
+std::map<int64_t, Plan *> g_plans;
+int64_t g_next = 1;
+
+Plan *get(so3ft_handle h)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_plans.find(h);
+    return it == g_plans.end() ? nullptr : it->second;
+}
+
+void free_plan_device(Plan *P)
+{
+    hipFree(P->tb.cosb); hipFree(P->tb.w); hipFree(P->tb.seeds); hipFree(P->tb.slice);
+    hipFree(P->tb.pairs); hipFree(P->tb.tw); hipFree(P->scratch); hipFree(P->red);
+    P->tb = DeviceTables();
+    P->scratch = nullptr; P->scratch_T = 0; P->red = nullptr;
+}
+
+template <class T>
+int upload(T **dst, const std::vector<T> &src)
+{
+    HIPCHK(hipMalloc((void **)dst, sizeof(T) * (src.empty() ? 1 : src.size())));
+    if (!src.empty()) HIPCHK(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+    return SO3FT_OK;
+}
+
+inline int64_t mnidx(int64_t m, int64_t n, int64_t bw) { return (m + bw - 1) * (2 * bw - 1) + (n + bw - 1); }
+
+// mnl layout slice starts by walking the storage order of pysofft/utils.f90:140-201
+std::vector<int64_t> slice_table(int64_t bw)
+{
+    std::vector<int64_t> t((2 * bw - 1) * (2 * bw - 1), 0);
+    int64_t pos = 0;
+    for (int64_t m1 = 0; m1 < bw; ++m1)
+        for (int64_t m2 = m1; m2 < bw; ++m2) {
+            const int64_t L = bw - m2;
+            t[mnidx(m1, m2, bw)] = pos; pos += L;
+            if (m1 == 0 && m2 == 0) continue;
+            t[mnidx(-m2, -m1, bw)] = pos; pos += L;
+            if (m1 != m2) {
+                t[mnidx(m2, m1, bw)] = pos; pos += L;
+                t[mnidx(-m1, -m2, bw)] = pos; pos += L;
+            }
+            if (m1 == 0 || m2 == 0) continue;
+            t[mnidx(m1, -m2, bw)] = pos; pos += L;
+            t[mnidx(-m1, m2, bw)] = pos; pos += L;
+            if (m1 == m2) continue;
+            t[mnidx(m2, -m1, bw)] = pos; pos += L;
+            t[mnidx(-m2, m1, bw)] = pos; pos += L;
+        }
+    return t;
+}
+
+int build_plan(Plan *P, int64_t bw, int64_t lmax, int precompute, int init_ffts, int64_t rt, int32_t flags)
+{
+    if (bw < 1 || bw > 256) return fail(SO3FT_EINVAL, "bw must satisfy 1 <= bw <= 256, got " + std::to_string(bw));
+    if (lmax < 0 || lmax >= bw) return fail(SO3FT_EINVAL, "lmax must satisfy 0 <= lmax < bw");
+    if (rt != 0 && rt != 1) return fail(SO3FT_EINVAL, "unknown recurrence type " + std::to_string(rt));
+    if (rt == 1 && !precompute)
+        return fail(SO3FT_EUNSUPPORTED,
+                    "recurrence_type=risbo without precomputed Wigners uses the lmn coefficient layout, "
+                    "which this library does not provide yet");
+    const int64_t N = 2 * bw;
+    P->bw = bw; P->lmax = lmax; P->N = N; P->nc = (4 * bw * bw * bw - bw) / 3;
+    P->precompute = precompute; P->init_ffts = init_ffts; P->recurrence = rt; P->fftw_flags = flags;
+    HIPCHK(hipGetDevice(&P->device));
+
+    const double pi = 3.14159265358979323846;
+    std::vector<double> beta(N), cosb(N), sincos(N), cos2(N), w(N);
+    for (int64_t k = 0; k < N; ++k) {  // wigner.f90:34-58
+        beta[k] = (double)(2 * k + 1) * (pi / (double)(2 * N));
+        cosb[k] = std::cos(beta[k]);
+        sincos[k] = std::cos(beta[k] / 2) * std::sin(beta[k] / 2);
+        cos2[k] = std::pow(std::cos(beta[k] / 2), 2);
+    }
+    {  // utils.f90:389-404
+        const double xi = pi / (double)(4 * bw);
+        for (int64_t k = 0; k < N; ++k) {
+            const double ko = (double)(2 * k + 1);
+            double s = 0.0;
+            for (int64_t j = 0; j < bw; ++j) {
+                const double jo = (double)(2 * j + 1);
+                s += (1.0 / jo) * std::sin(ko * jo * xi);
+            }
+            w[k] = s * ((2.0 / (double)bw) * std::sin(ko * xi));
+        }
+    }
+    // seeds d~^q_{p,q}(beta_k): stair recursion of wigner.f90:130-187, stored [tri(p,q)][k]
+    const int64_t ntri = bw * (bw + 1) / 2;
+    std::vector<double> seeds(ntri * N), tmp(bw * N);
+    for (int64_t k = 0; k < N; ++k) tmp[k] = std::sqrt(0.5);
+    for (int64_t m = 0; m < bw; ++m) {
+        const int64_t sw = m + 1, sw2 = (m == 0) ? m : sw;
+        const int64_t lim = std::min(bw - m, sw2);
+        for (int64_t i = 1; i <= lim; ++i) {
+            const int64_t l = m + i - 2;
+            const double f = std::sqrt((double)((2 * l + 2) * (2 * l + 3)) / (double)((l + m) * (l + m + 1)));
+            for (int64_t k = 0; k < N; ++k) tmp[(i - 1) * N + k] = tmp[(i - 1) * N + k] * f * cos2[k];
+        }
+        for (int64_t i = sw; i <= bw - m - 1; ++i) {
+            const int64_t l = m + i - 1;
+            const double f = std::sqrt((double)((2 * l + 2) * (2 * l + 3)) / (double)((l + m + 1) * (l - m + 1)));
+            for (int64_t k = 0; k < N; ++k) tmp[i * N + k] = tmp[(i - 1) * N + k] * f * sincos[k];
+        }
+        const int64_t base = (m * (2 * bw - m + 1)) / 2;
+        std::memcpy(&seeds[base * N], tmp.data(), sizeof(double) * (bw - m) * N);
+    }
+    // pairs with q <= lmax, most work (largest bw-q) first
+    std::vector<int32_t> pairs;
+    for (int64_t q = 0; q <= lmax; ++q)
+        for (int64_t p = 0; p <= q; ++p) pairs.push_back((int32_t)(p | (q << 16)));
+    P->npair = (int64_t)pairs.size();
+    // twiddles exp(-2 pi i k/N)
+    std::vector<double2> tw(N);
+    for (int64_t k = 0; k < N; ++k) {
+        const long double th = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)N;
+        tw[k] = make_double2((double)cosl(th), -(double)sinl(th));
+    }
+    FftDesc fd;
+    fd.N = (int)N;
+    int64_t n = N;
+    while (n % 4 == 0) { fd.rad[fd.nrad++] = 4; n /= 4; }
+    while (n % 2 == 0) { fd.rad[fd.nrad++] = 2; n /= 2; }
+    for (int64_t pr = 3; n > 1; pr += 2)
+        while (n % pr == 0) { fd.rad[fd.nrad++] = (int)pr; n /= pr; }
+    P->fft = fd;
+
+    int rc;
+    if ((rc = upload(&P->tb.cosb, cosb))) return rc;
+    if ((rc = upload(&P->tb.w, w))) return rc;
+    if ((rc = upload(&P->tb.seeds, seeds))) return rc;
+    if ((rc = upload(&P->tb.slice, slice_table(bw)))) return rc;
+    if ((rc = upload(&P->tb.pairs, pairs))) return rc;
+    if ((rc = upload(&P->tb.tw, tw))) return rc;
+    HIPCHK(hipMalloc((void **)&P->red, sizeof(double) * (2 * 1024 + 2)));
+    HIPCHK(dwt_set_attributes((int)bw, (int)N));
+    return SO3FT_OK;
+}
+
+// scratch for T transforms (complex N^3 each)
+int ensure_scratch(Plan *P, int64_t T)
+{
+    if (P->scratch_T >= T) return SO3FT_OK;
+    hipFree(P->scratch);
+    P->scratch = nullptr; P->scratch_T = 0;
+    const size_t bytes = sizeof(double2) * (size_t)P->N * P->N * P->N * (size_t)T;
+    if (hipMalloc((void **)&P->scratch, bytes) != hipSuccess) {
+        hipGetLastError();
+        return fail(SO3FT_ENOMEM, "scratch allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    P->scratch_T = T;
+    return SO3FT_OK;
+}
+
+// transforms per launch for the *_many paths: bounded scratch (<= 4 GiB)
+int64_t chunk_of(const Plan *P, int64_t n)
+{
+    const double per = 16.0 * (double)P->N * P->N * P->N;
+    int64_t c = (int64_t)(4.0 * 1024 * 1024 * 1024 / per);
+    if (c < 1) c = 1;
+    return std::min(c, n);
+}
+
+inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+int check_ptrs(const void *a, const void *b)
+{
+    if (!a || !b) return fail(SO3FT_EINVAL, "null array pointer");
+    return SO3FT_OK;
+}
+
+// F (i_n,i_m,beta) -> SO(3) function: FFT along m (in place), then along n with 1/(2 pi)
+int inverse_fft_passes(Plan *P, double2 *F, void *out, int out_real, int64_t T, hipStream_t s)
+{
+    const int64_t N = P->N, N2 = N * N, N3 = N2 * N;
+    FftAxis a{};
+    a.in_t = N3; a.in_o = N2; a.in_r = N; a.out_t = N3; a.out_o = N2; a.out_r = N;
+    a.mask_rows = 1; a.mask_outer = 1; a.sign = -1; a.scale = 1.0;
+    HIPCHK(launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, F, F, a, (int)T, s));
+    FftAxis b{};
+    b.in_t = N3; b.in_o = N; b.in_r = N2; b.out_t = N3; b.out_o = N; b.out_r = N2;
+    b.mask_rows = 1; b.mask_outer = 0; b.out_real = out_real; b.sign = -1;
+    b.scale = 1.0 / (2.0 * 3.14159265358979323846);
+    HIPCHK(launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, F, out, b, (int)T, s));
+    return SO3FT_OK;
+}
+
+// SO(3) function -> F: FFT along n (gamma) into F, then along m in place with 2 pi/(2bw)^2
+int forward_fft_passes(Plan *P, const void *in, int in_real, double2 *F, int64_t T, hipStream_t s)
+{
+    const int64_t N = P->N, N2 = N * N, N3 = N2 * N;
+    FftAxis b{};
+    b.in_t = N3; b.in_o = N; b.in_r = N2; b.out_t = N3; b.out_o = N; b.out_r = N2;
+    b.in_real = in_real; b.sign = +1; b.scale = 1.0;
+    HIPCHK(launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, in, F, b, (int)T, s));
+    FftAxis a{};
+    a.in_t = N3; a.in_o = N2; a.in_r = N; a.out_t = N3; a.out_o = N2; a.out_r = N;
+    a.mask_outer = 1; a.sign = +1;
+    a.scale = 2.0 * 3.14159265358979323846 / ((double)N * (double)N);
+    HIPCHK(launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, F, F, a, (int)T, s));
+    return SO3FT_OK;
+}
+
+int run_inverse(Plan *P, int mode, const void *coeff, int64_t c_ts, const void *aux, void *out, int out_real,
+                int64_t n, hipStream_t s)
+{
+    const int64_t N = P->N, N3 = N * N * N;
+    const int64_t ch = chunk_of(P, n);
+    int rc = ensure_scratch(P, ch);
+    if (rc) return rc;
+    const size_t osz = out_real ? sizeof(double) : sizeof(double2);
+    for (int64_t i0 = 0; i0 < n; i0 += ch) {
+        const int64_t T = std::min(ch, n - i0);
+        const double2 *c = reinterpret_cast<const double2 *>(coeff) + i0 * c_ts;
+        const double2 *ax = aux ? reinterpret_cast<const double2 *>(aux) + i0 * c_ts : nullptr;
+        HIPCHK(launch_dwt_inverse(P->tb, (int)P->bw, (int)N, (int)P->npair, mode, c, c_ts, ax, P->scratch, N3,
+                                  (int)T, s));
+        void *o = reinterpret_cast<char *>(out) + (size_t)i0 * N3 * osz;
+        if ((rc = inverse_fft_passes(P, P->scratch, o, out_real, T, s))) return rc;
+    }
+    return SO3FT_OK;
+}
+
+int run_forward(Plan *P, int mode, const void *in, int in_real, void *coeff, int64_t n, hipStream_t s)
+{
+    const int64_t N = P->N, N3 = N * N * N;
+    const int64_t ch = chunk_of(P, n);
+    int rc = ensure_scratch(P, ch);
+    if (rc) return rc;
+    const size_t isz = in_real ? sizeof(double) : sizeof(double2);
+    for (int64_t i0 = 0; i0 < n; i0 += ch) {
+        const int64_t T = std::min(ch, n - i0);
+        const void *f = reinterpret_cast<const char *>(in) + (size_t)i0 * N3 * isz;
+        if ((rc = forward_fft_passes(P, f, in_real, P->scratch, T, s))) return rc;
+        double2 *c = reinterpret_cast<double2 *>(coeff) + i0 * P->nc;
+        HIPCHK(launch_dwt_forward(P->tb, (int)P->bw, (int)N, (int)P->npair, mode, P->scratch, N3, c, P->nc, (int)T,
+                                  s));
+    }
+    return SO3FT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int so3ft_get_kostelec_recurrence_val(void) { return 0; }
+int so3ft_get_risbo_recurrence_val(void) { return 1; }
+const char *so3ft_last_error(void) { return g_err.c_str(); }
+
+int so3ft_init(int64_t bw, int64_t lmax, int precompute_wigners, int init_ffts, int64_t recurrence_type,
+               int32_t fftw_flags, so3ft_handle *out)
+{
+    if (!out) return fail(SO3FT_EINVAL, "null handle pointer");
+    Plan *P = new Plan();
+    int rc = build_plan(P, bw, lmax, precompute_wigners, init_ffts, recurrence_type, fftw_flags);
+    if (rc) {
+        free_plan_device(P);
+        delete P;
+        return rc;
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    *out = g_next++;
+    g_plans[*out] = P;
+    return SO3FT_OK;
+}
+
+int so3ft_destroy(so3ft_handle h)
+{
+    Plan *P = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_plans.find(h);
+        if (it == g_plans.end()) return fail(SO3FT_EHANDLE, "unknown handle");
+        P = it->second;
+        g_plans.erase(it);
+    }
+    hipDeviceSynchronize();
+    free_plan_device(P);
+    delete P;
+    return SO3FT_OK;
+}
+
+int so3ft_reset(so3ft_handle h, int64_t bw, int64_t lmax, int precompute_wigners, int init_ffts,
+                int64_t recurrence_type, int32_t fftw_flags)
+{
+    Plan *P = get(h);
+    if (!P) return fail(SO3FT_EHANDLE, "unknown handle");
+    Plan fresh;
+    int rc = build_plan(&fresh, bw, lmax, precompute_wigners, init_ffts, recurrence_type, fftw_flags);
+    if (rc) {
+        free_plan_device(&fresh);
+        return rc;
+    }
+    hipDeviceSynchronize();
+    free_plan_device(P);
+    *P = fresh;
+    return SO3FT_OK;
+}
+
+int so3ft_init_fft(so3ft_handle h, int use_real_fft)
+{
+    (void)use_real_fft;
+    return get(h) ? SO3FT_OK : fail(SO3FT_EHANDLE, "unknown handle");
+}
+
+int so3ft_get_bw(so3ft_handle h, int64_t *v)
+{
+    Plan *P = get(h);
+    if (!P || !v) return fail(SO3FT_EHANDLE, "unknown handle");
+    *v = P->bw;
+    return SO3FT_OK;
+}
+int so3ft_get_lmax(so3ft_handle h, int64_t *v)
+{
+    Plan *P = get(h);
+    if (!P || !v) return fail(SO3FT_EHANDLE, "unknown handle");
+    *v = P->lmax;
+    return SO3FT_OK;
+}
+int so3ft_set_lmax(so3ft_handle h, int64_t lmax)
+{
+    Plan *P = get(h);
+    if (!P) return fail(SO3FT_EHANDLE, "unknown handle");
+    if (lmax < 0 || lmax >= P->bw) return fail(SO3FT_EINVAL, "lmax must satisfy 0 <= lmax < bw");
+    return so3ft_reset(h, P->bw, lmax, P->precompute, P->init_ffts, P->recurrence, P->fftw_flags);
+}
+int so3ft_get_recurrence_type(so3ft_handle h, int64_t *v)
+{
+    Plan *P = get(h);
+    if (!P || !v) return fail(SO3FT_EHANDLE, "unknown handle");
+    *v = P->recurrence;
+    return SO3FT_OK;
+}
+int so3ft_wigners_are_precomputed(so3ft_handle h, int *v)
+{
+    Plan *P = get(h);
+    if (!P || !v) return fail(SO3FT_EHANDLE, "unknown handle");
+    *v = P->precompute;
+    return SO3FT_OK;
+}
+int so3ft_get_fftw_flags(so3ft_handle h, int32_t *v)
+{
+    Plan *P = get(h);
+    if (!P || !v) return fail(SO3FT_EHANDLE, "unknown handle");
+    *v = P->fftw_flags;
+    return SO3FT_OK;
+}
+
+#define GETP(h)                                                  \
+    Plan *P = get(h);                                            \
+    if (!P) return fail(SO3FT_EHANDLE, "unknown handle");
+
+int so3ft_isoft_many(so3ft_handle h, const void *coeffs, void *so3funcs, int64_t n, void *stream)
+{
+    GETP(h);
+    if (int rc = check_ptrs(coeffs, so3funcs)) return rc;
+    if (n < 0) return fail(SO3FT_EINVAL, "negative count");
+    return run_inverse(P, DWT_COMPLEX, coeffs, P->nc, nullptr, so3funcs, 0, n, S(stream));
+}
+int so3ft_isoft(so3ft_handle h, const void *coeff, void *so3func, void *stream)
+{
+    return so3ft_isoft_many(h, coeff, so3func, 1, stream);
+}
+int so3ft_soft_many(so3ft_handle h, const void *so3funcs, void *coeffs, int64_t n, void *stream)
+{
+    GETP(h);
+    if (int rc = check_ptrs(coeffs, so3funcs)) return rc;
+    if (n < 0) return fail(SO3FT_EINVAL, "negative count");
+    return run_forward(P, DWT_COMPLEX, so3funcs, 0, coeffs, n, S(stream));
+}
+int so3ft_soft(so3ft_handle h, const void *so3func, void *coeff, void *stream)
+{
+    return so3ft_soft_many(h, so3func, coeff, 1, stream);
+}
+int so3ft_irsoft_many(so3ft_handle h, const void *coeffs, void *so3funcs, int64_t n, void *stream)
+{
+    GETP(h);
+    if (int rc = check_ptrs(coeffs, so3funcs)) return rc;
+    if (n < 0) return fail(SO3FT_EINVAL, "negative count");
+    return run_inverse(P, DWT_REAL, coeffs, P->nc, nullptr, so3funcs, 1, n, S(stream));
+}
+int so3ft_irsoft(so3ft_handle h, const void *coeff, void *so3func, void *stream)
+{
+    return so3ft_irsoft_many(h, coeff, so3func, 1, stream);
+}
+int so3ft_rsoft_many(so3ft_handle h, const void *so3funcs, void *coeffs, int64_t n, void *stream)
+{
+    GETP(h);
+    if (int rc = check_ptrs(coeffs, so3funcs)) return rc;
+    if (n < 0) return fail(SO3FT_EINVAL, "negative count");
+    return run_forward(P, DWT_REAL, so3funcs, 1, coeffs, n, S(stream));
+}
+int so3ft_rsoft(so3ft_handle h, const void *so3func, void *coeff, void *stream)
+{
+    return so3ft_rsoft_many(h, so3func, coeff, 1, stream);
+}
+
+int so3ft_inverse_wigner_trf_cmplx(so3ft_handle h, const void *coeff, void *fft_array, void *stream)
+{
+    GETP(h);
+    if (int rc = check_ptrs(coeff, fft_array)) return rc;
+    const int64_t N3 = P->N * P->N * P->N;
+    double2 *F = reinterpret_cast<double2 *>(fft_array);
+    HIPCHK(launch_dwt_inverse(P->tb, (int)P->bw, (int)P->N, (int)P->npair, DWT_COMPLEX,
+                              reinterpret_cast<const double2 *>(coeff), P->nc, nullptr, F, N3, 1, S(stream)));
+    HIPCHK(launch_zero_invalid_columns(F, N3, (int)P->bw, (int)P->N, (int)P->lmax, 1, S(stream)));
+    return SO3FT_OK;
+}
+
+int so3ft_forward_wigner_trf_cmplx(so3ft_handle h, const void *fft_array, void *coeff, void *stream)
+{
+    GETP(h);
+    if (int rc = check_ptrs(coeff, fft_array)) return rc;
+    const int64_t N3 = P->N * P->N * P->N;
+    HIPCHK(launch_dwt_forward(P->tb, (int)P->bw, (int)P->N, (int)P->npair, DWT_COMPLEX,
+                              reinterpret_cast<const double2 *>(fft_array), N3, reinterpret_cast<double2 *>(coeff),
+                              P->nc, 1, S(stream)));
+    return SO3FT_OK;
+}
+
+static int fft2_scaled(Plan *P, const void *f1, void *f2, int sign, hipStream_t s)
+{
+    const int64_t N = P->N, N2 = N * N, N3 = N2 * N;
+    FftAxis b{};
+    b.in_t = N3; b.in_o = N; b.in_r = N2; b.out_t = N3; b.out_o = N; b.out_r = N2;
+    b.sign = sign; b.scale = 1.0;
+    HIPCHK(launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, f1, f2, b, 1, s));
+    FftAxis a{};
+    a.in_t = N3; a.in_o = N2; a.in_r = N; a.out_t = N3; a.out_o = N2; a.out_r = N;
+    a.sign = sign; a.scale = 1.0 / (double)N;
+    HIPCHK(launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, f2, f2, a, 1, s));
+    return SO3FT_OK;
+}
+
+int so3ft_fft(so3ft_handle h, const void *f1, void *f2, void *stream)
+{
+    GETP(h);
+    if (int rc = check_ptrs(f1, f2)) return rc;
+    return fft2_scaled(P, f1, f2, -1, S(stream));
+}
+int so3ft_ifft(so3ft_handle h, const void *f1, void *f2, void *stream)
+{
+    GETP(h);
+    if (int rc = check_ptrs(f1, f2)) return rc;
+    return fft2_scaled(P, f1, f2, +1, S(stream));
+}
+
+static int integrate(Plan *P, const void *f, int is_real, double *re, double *im, hipStream_t s)
+{
+    const int nb = 1024;
+    HIPCHK(launch_integrate(f, is_real, P->tb.w, (int)P->N, P->red, nb, P->red + 2 * nb, s));
+    double res[2];
+    HIPCHK(hipMemcpyAsync(res, P->red + 2 * nb, sizeof(res), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const double sc = std::pow(3.14159265358979323846 / (double)P->bw, 2);
+    *re = res[0] * sc;
+    if (im) *im = res[1] * sc;
+    return SO3FT_OK;
+}
+
+int so3ft_integrate_over_so3_cmplx(so3ft_handle h, const void *f, double *re, double *im, void *stream)
+{
+    GETP(h);
+    if (!f || !re || !im) return fail(SO3FT_EINVAL, "null pointer");
+    return integrate(P, f, 0, re, im, S(stream));
+}
+int so3ft_integrate_over_so3_real(so3ft_handle h, const void *f, double *out, void *stream)
+{
+    GETP(h);
+    if (!f || !out) return fail(SO3FT_EINVAL, "null pointer");
+    return integrate(P, f, 1, out, nullptr, S(stream));
+}
+
+int so3ft_cross_correlation_ylm_cmplx(so3ft_handle h, const void *f_lm, const void *g_lm, void *cc, void *stream)
+{
+    GETP(h);
+    if (!f_lm || !g_lm || !cc) return fail(SO3FT_EINVAL, "null pointer");
+    return run_inverse(P, DWT_CORR, f_lm, P->bw * P->bw, g_lm, cc, 0, 1, S(stream));
+}
+
+int so3ft_cross_correlation_ylm_cmplx_3d(so3ft_handle h, const void *f_lms, const void *g_lms, void *cc,
+                                         const double *rad, int64_t n_radial, int64_t r_lo, int64_t r_hi,
+                                         void *stream)
+{
+    GETP(h);
+    if (!f_lms || !g_lms || !cc || !rad) return fail(SO3FT_EINVAL, "null pointer");
+    if (n_radial < 2 || r_lo < 0 || r_hi >= n_radial || r_lo > r_hi)
+        return fail(SO3FT_EINVAL, "radial_limits out of bounds for radial_sampling_points");
+    const int64_t N3 = P->N * P->N * P->N, nb = P->bw * P->bw;
+    // soft.f08:2696-2737: sum_r r^2 C_r / (r_hi - r_lo + dr); one shell at a time through a
+    // second scratch slot
+    int rc = ensure_scratch(P, 2);
+    if (rc) return rc;
+    double2 *tmp = P->scratch + N3;
+    const double step = rad[1] - rad[0];
+    const double inv = 1.0 / (rad[r_hi] - rad[r_lo] + step);
+    for (int64_t r = r_lo; r <= r_hi; ++r) {
+        const double2 *f = reinterpret_cast<const double2 *>(f_lms) + r * nb;
+        const double2 *g = reinterpret_cast<const double2 *>(g_lms) + r * nb;
+        HIPCHK(launch_dwt_inverse(P->tb, (int)P->bw, (int)P->N, (int)P->npair, DWT_CORR, f, nb, g, P->scratch, N3,
+                                  1, S(stream)));
+        if ((rc = inverse_fft_passes(P, P->scratch, tmp, 0, 1, S(stream)))) return rc;
+        HIPCHK(launch_radial_accumulate(reinterpret_cast<double2 *>(cc), tmp, N3, rad[r] * rad[r] * inv,
+                                        r == r_lo, S(stream)));
+    }
+    return SO3FT_OK;
+}
+
+int so3ft_bench_kernel(so3ft_handle h, int which, int reps, void *a, void *b, int64_t T, void *stream,
+                       float *ms_per_launch)
+{
+    GETP(h);
+    if (!a || !b || !ms_per_launch || reps < 1 || T < 1) return fail(SO3FT_EINVAL, "bad argument");
+    hipStream_t s = S(stream);
+    const int64_t N = P->N, N2 = N * N, N3 = N2 * N;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    auto launch = [&]() -> hipError_t {
+        switch (which) {
+        case 0:
+            return launch_dwt_inverse(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX,
+                                      reinterpret_cast<const double2 *>(a), P->nc, nullptr,
+                                      reinterpret_cast<double2 *>(b), N3, (int)T, s);
+        case 1:
+            return launch_dwt_forward(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX,
+                                      reinterpret_cast<const double2 *>(a), N3, reinterpret_cast<double2 *>(b),
+                                      P->nc, (int)T, s);
+        case 2: {
+            FftAxis ax{};
+            ax.in_t = N3; ax.in_o = N2; ax.in_r = N; ax.out_t = N3; ax.out_o = N2; ax.out_r = N;
+            ax.sign = -1; ax.scale = 1.0;
+            return launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, a, b, ax, (int)T, s);
+        }
+        default: {
+            FftAxis ax{};
+            ax.in_t = N3; ax.in_o = N; ax.in_r = N2; ax.out_t = N3; ax.out_o = N; ax.out_r = N2;
+            ax.sign = -1; ax.scale = 1.0;
+            return launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, a, b, ax, (int)T, s);
+        }
+        }
+    };
+    HIPCHK(launch());  // warm-up
+    HIPCHK(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; ++i) HIPCHK(launch());
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_launch = ms / (float)reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return SO3FT_OK;
+}
+
+int so3ft_enforce_real_sym_mnl(int64_t bw, void *coeffs, int64_t n, void *stream)
+{
+    if (bw < 1 || bw > 256 || !coeffs || n < 0) return fail(SO3FT_EINVAL, "bad argument");
+    // slice table for this bw (cached per bw)
+    static std::mutex mu;
+    static std::map<std::pair<int, int64_t>, int64_t *> cache;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    int64_t *d = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto key = std::make_pair(dev, bw);
+        auto it = cache.find(key);
+        if (it == cache.end()) {
+            int rc = upload(&d, slice_table(bw));
+            if (rc) return rc;
+            cache[key] = d;
+        } else {
+            d = it->second;
+        }
+    }
+    const int64_t nc = (4 * bw * bw * bw - bw) / 3;
+    HIPCHK(launch_enforce_real_sym(reinterpret_cast<double2 *>(coeffs), nc, d, (int)bw, (int)n, S(stream)));
+    return SO3FT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+// Internal declarations shared by the libso3ft translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace so3 {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+// Variant table for a pair 0 <= p <= q < bw (pysofft/soft.f08:534-631): the
+// (m, n) orders that share d~^l_{p,q}(beta) through the symmetries of
+// docs/defs/wigner.md:30-40.  Index v:
+//   0 (p,q)   1 (-q,-p)   2 (q,p)  (-1)^(p+q)   3 (-p,-q) (-1)^(p+q)
+//   4 (p,-q)  (-1)^(p+l) reversed beta     5 (-p,q)  (-1)^(q+l) reversed
+//   6 (q,-p)  (-1)^(p+l) reversed          7 (-q,p)  (-1)^(q+l) reversed
+struct Variant {
+    int m, n;
+    double sign;   // constant sign
+    bool lsign;    // extra (-1)^l
+    bool rev;      // beta -> pi - beta (index k -> N-1-k)
+    bool valid;
+};
+
+__host__ __device__ inline Variant variant(int v, int p, int q)
+{
+    Variant r;
+    const double spq = ((p + q) & 1) ? -1.0 : 1.0;
+    const double sp = (p & 1) ? -1.0 : 1.0;
+    const double sq = (q & 1) ? -1.0 : 1.0;
+    switch (v) {
+    case 0: r = {p, q, 1.0, false, false, true}; break;
+    case 1: r = {-q, -p, 1.0, false, false, !(p == 0 && q == 0)}; break;
+    case 2: r = {q, p, spq, false, false, p != q}; break;
+    case 3: r = {-p, -q, spq, false, false, p != q}; break;
+    case 4: r = {p, -q, sp, true, true, p != 0}; break;
+    case 5: r = {-p, q, sq, true, true, p != 0}; break;
+    case 6: r = {q, -p, sp, true, true, p != 0 && p != q}; break;
+    default: r = {-q, p, sq, true, true, p != 0 && p != q}; break;
+    }
+    return r;
+}
+
+// DWT coefficient/row assembly modes
+enum DwtMode : int {
+    DWT_COMPLEX = 0,   // isoft/soft (soft.f08:534-631, 937-1023)
+    DWT_REAL = 1,      // irsoft/rsoft half-complex semantics (soft.f08:1259-1318, 1620-1707)
+    DWT_CORR = 2,      // cross-correlation coefficients from SH pairs (soft.f08:2242-2341)
+};
+
+struct DeviceTables {
+    double *cosb = nullptr;    // [N]    cos(beta_k)
+    double *w = nullptr;       // [N]    Driscoll-Healy weights
+    double *seeds = nullptr;   // [ntri][N]  d~^q_{p,q}(beta_k)
+    int64_t *slice = nullptr;  // [(2bw-1)^2] mnl start of f^l_{m,n}
+    int32_t *pairs = nullptr;  // [npair] p | q << 16, ordered by decreasing bw-q
+    double2 *tw = nullptr;     // [N]    exp(-2 pi i k / N)
+};
+
+struct FftDesc {
+    int N = 0;
+    int nrad = 0;
+    int rad[32] = {0};
+};
+
+// launches (stream-ordered; return hipError_t of the launch)
+hipError_t launch_dwt_inverse(const DeviceTables &t, int bw, int N, int npair, int mode,
+                              const double2 *coeff, int64_t c_ts, const double2 *aux,
+                              double2 *F, int64_t f_ts, int T, hipStream_t s);
+hipError_t launch_dwt_forward(const DeviceTables &t, int bw, int N, int npair, int mode,
+                              const double2 *F, int64_t f_ts, double2 *coeff, int64_t c_ts, int T,
+                              hipStream_t s);
+
+struct FftAxis {
+    int64_t in_t, in_o, in_r;      // element strides: transform, outer line, fft row
+    int64_t out_t, out_o, out_r;
+    int mask_rows;                 // 1: treat rows whose order is Nyquist or |order| > lmax as zero
+    int mask_outer;                // 1: whole line zero when the outer order is invalid
+    int in_real, out_real;         // input float64 (imag 0) / output real part into float64
+    int sign;                      // -1 FFTW_FORWARD, +1 FFTW_BACKWARD
+    double scale;
+};
+hipError_t launch_fft_axis(const FftDesc &fd, const double2 *tw, int bw, int lmax, const void *in,
+                           void *out, const FftAxis &ax, int T, hipStream_t s);
+
+hipError_t launch_zero_invalid_columns(double2 *F, int64_t f_ts, int bw, int N, int lmax, int T,
+                                       hipStream_t s);
+hipError_t launch_scale(double2 *X, int64_t n, double scale, hipStream_t s);
+hipError_t launch_integrate(const void *f, int is_real, const double *w, int N, double *partials,
+                            int nblocks, double *result2, hipStream_t s);
+hipError_t launch_enforce_real_sym(double2 *c, int64_t c_ts, const int64_t *slice, int bw, int T,
+                                   hipStream_t s);
+hipError_t launch_radial_accumulate(double2 *acc, const double2 *x, int64_t n, double w, int first,
+                                    hipStream_t s);
+
+int dwt_lds_bytes_inverse(int bw, int N);
+int dwt_lds_bytes_forward(int bw, int N);
+hipError_t dwt_set_attributes(int bw, int N);
+
+}  // namespace so3

@@ -1,0 +1,205 @@
+"""Pins the CPU oracle (oracle/) against the reference's own code and mathematics.
+
+Each oracle function is checked against at least one of: vectors produced by the
+reference's Fortran (tests/golden/ref_vectors.npz, and the live build in
+oracle/_ref when present), the reference's analytic known-answer tests
+(tests/test_fortran_backend.py:8-114, 329-461), or closed forms / invariants.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import RefLib, brute
+from conftest import rel_l2
+
+
+@pytest.mark.parametrize("bw", [1, 2, 3, 5, 8, 13])
+def test_layout_vs_reference_vectors(golden, bw):
+    assert np.array_equal(O.coeff_degrees(bw), golden[f"degrees_{bw}"].astype(np.int64))
+
+
+@pytest.mark.parametrize("bw", [1, 2, 3, 4, 7, 16, 32, 64, 128, 256])
+def test_weights_vs_reference_vectors(golden, bw):
+    w = O.legendre_weights(bw)
+    np.testing.assert_allclose(w, golden[f"weights_{bw}"], rtol=0, atol=4e-16 * bw)
+    assert abs(w.sum() - 2.0) < 1e-13          # sum of weights = int sin = 2
+    # exactness on even Legendre polynomials P_m(cos b), m <= bw (utils.f90:378)
+    b = O.beta_samples(2 * bw)
+    for m in range(0, min(bw, 12)):
+        pm = np.polynomial.legendre.legval(np.cos(b), [0] * m + [1])
+        assert abs(np.dot(w, pm) - (2.0 if m == 0 else 0.0)) < 1e-12
+
+
+def test_seeds_vs_reference_vectors(golden):
+    s = O.seeds(9)
+    np.testing.assert_array_equal(s, golden["seeds_9"])
+    # closed form d~^l_{m,l} (tests/test_fortran_backend.py:8-12)
+    b = O.beta_samples(18)
+    for m in range(9):
+        for l in range(m, 9):
+            tri = (m * (2 * 9 - m + 1)) // 2 + l - m
+            np.testing.assert_allclose(s[tri], brute.dlml_closed_form(l, m, b), rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("p,q,bw", [(0, 0, 16), (3, 7, 16), (5, 5, 16), (0, 15, 16), (32, 56, 129), (0, 0, 128),
+                                    (17, 90, 128), (127, 127, 128)])
+def test_genwig_vs_reference_vectors(golden, p, q, bw):
+    g = O.genwig(p, q, bw)
+    ref = golden[f"genwig_{p}_{q}_{bw}"]
+    assert np.abs(g - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def test_genwig_vs_jacobi():
+    bw = 12
+    b = O.beta_samples(2 * bw)
+    for (p, q) in [(0, 0), (0, 5), (2, 3), (4, 4), (1, 11)]:
+        g = O.genwig(p, q, bw)
+        for l in range(q, bw):
+            np.testing.assert_allclose(g[l - q], brute.wigner_d_norm(l, p, q, b), rtol=1e-11, atol=1e-13)
+
+
+def test_legendre_and_assoc_legendre():
+    # d~^l_{00} = sqrt((2l+1)/2) P_l(cos b) (tests/test_fortran_backend.py:205-231)
+    bw = 129
+    b = O.beta_samples(2 * bw)
+    g = O.genwig(0, 0, bw)
+    P = np.array([np.polynomial.legendre.legval(np.cos(b), [0] * l + [1]) for l in range(bw)])
+    np.testing.assert_allclose(g, P * np.sqrt((2 * np.arange(bw)[:, None] + 1) / 2), rtol=1e-9, atol=1e-11)
+
+
+def test_risbo_step_vs_reference_vectors(golden):
+    cb, sb = golden["risbo_cs"]
+    sq = np.sqrt(np.arange(25))
+    d = None
+    for l in range(12):
+        d = O.risbo_step(d, l, cb, sb, sq)
+    np.testing.assert_array_equal(d, golden["risbo_l11"])
+
+
+@pytest.mark.parametrize("bw", [1, 4, 6])
+def test_enforce_real_sym_vs_reference_vectors(golden, bw):
+    c = golden[f"sym_in_{bw}"].copy()
+    O.enforce_real_sym(c, bw)
+    np.testing.assert_array_equal(c, golden[f"sym_out_{bw}"])
+
+
+@pytest.mark.parametrize("bw", [2, 3, 4, 8])
+def test_isoft_vs_reference_dmatrices(golden, bw):
+    P = O.OraclePlan(bw)
+    f = P.isoft(golden[f"isoft_c_{bw}"])
+    assert rel_l2(f, golden[f"isoft_f_{bw}"]) < 1e-13
+
+
+@pytest.mark.parametrize("bw", [2, 3, 5])
+def test_isoft_vs_jacobi_direct_sum(bw):
+    rng = np.random.default_rng(bw)
+    P = O.OraclePlan(bw)
+    c = rng.standard_normal(P.nc) + 1j * rng.standard_normal(P.nc)
+    assert rel_l2(P.isoft(c), brute.isoft_direct(c, O.coeff_degrees(bw), bw)) < 1e-13
+
+
+def test_isoft_soft_kats():
+    """tests/test_fortran_backend.py:329-461 at bw=32: unit coefficient -> D^l_mn on the grid."""
+    bw = 32
+    P = O.OraclePlan(bw)
+    tab = O.slice_table(bw)
+    for (l, m, n) in [(0, 0, 0), (1, 1, -1), (1, 0, 1), (2, -2, 1), (2, 1, -2), (2, 0, 0), (5, 3, 5), (9, -7, 9)]:
+        c = np.zeros(P.nc, np.complex128)
+        c[tab[m + bw - 1, n + bw - 1] + l - max(abs(m), abs(n))] = 1
+        f = P.isoft(c)
+        D = brute.basis(l, m, n, bw)
+        assert np.abs(f - D).max() < 1e-13
+        c2 = P.soft(D)
+        assert np.abs(c2 - c).max() < 1e-12
+    b = O.beta_samples(2 * bw)
+    for eta in [1, 5, 15]:
+        c = np.zeros(P.nc, np.complex128)
+        for l in range(bw):
+            c[tab[bw - 1, bw - 1] + l] = brute.cos_2eta_coeff(l, eta)
+        f = P.isoft(c)
+        np.testing.assert_allclose(f[0, 0, :], brute.cos_2eta_func(b, eta), rtol=1e-10, atol=1e-10)
+        F = np.broadcast_to(brute.cos_2eta_func(b, eta)[None, None, :], f.shape).astype(np.complex128)
+        np.testing.assert_allclose(P.soft(F), c, rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize("bw", [1, 2, 3, 7, 10])
+def test_invertibility_and_real(bw):
+    rng = np.random.default_rng(100 + bw)
+    P = O.OraclePlan(bw)
+    c = rng.standard_normal(P.nc) + 1j * rng.standard_normal(P.nc)
+    f = P.isoft(c)
+    assert rel_l2(P.soft(f), c) < 1e-13
+    cs = c.copy()
+    O.enforce_real_sym(cs, bw)
+    fr = P.irsoft(cs)
+    assert np.abs(P.isoft(cs).real - fr).max() < 1e-13
+    assert rel_l2(P.rsoft(fr), cs) < 1e-13
+    g = rng.standard_normal(P.grid_shape)
+    assert rel_l2(P.rsoft(g), P.soft(g + 0j)) < 1e-13
+
+
+def test_many_and_integrate():
+    bw = 6
+    rng = np.random.default_rng(3)
+    P = O.OraclePlan(bw)
+    cs = rng.standard_normal((3, P.nc)) + 1j * rng.standard_normal((3, P.nc))
+    fs = P.isoft_many(cs)
+    for i in range(3):
+        np.testing.assert_array_equal(fs[i], P.isoft(cs[i]))
+    np.testing.assert_allclose(P.soft_many(fs), cs, atol=1e-12)
+    v = P.integrate(np.ones(P.grid_shape, np.complex128))
+    assert abs(v - 8 * np.pi ** 2) < 1e-10
+
+
+def test_fft2_vs_numpy():
+    bw = 6
+    rng = np.random.default_rng(4)
+    P = O.OraclePlan(bw)
+    X = rng.standard_normal(P.grid_shape) + 1j * rng.standard_normal(P.grid_shape)
+    np.testing.assert_allclose(P.fft2(X, -1), np.fft.fft2(X, axes=(0, 1)), atol=1e-12)
+    np.testing.assert_allclose(P.fft2(X, +1), np.fft.ifft2(X, axes=(0, 1)) * 144, atol=1e-12)
+
+
+def test_rotate_vs_jacobi():
+    bw = 7
+    rng = np.random.default_rng(5)
+    y = rng.standard_normal(bw * bw) + 1j * rng.standard_normal(bw * bw)
+    for e in ([0.3, 1.1, 2.0], [5.0, 2.9, -1.0], [0.0, 0.0, 0.0]):
+        r = O.rotate_ylm_cmplx(y, e)[0, 0]
+        np.testing.assert_allclose(r, brute.rotate_ylm_direct(y, e, bw), atol=1e-12)
+    # group law: rotating by (a,b,g) then its inverse (-g,-b,-a) is the identity
+    e = np.array([0.7, 1.2, 2.5])
+    r = O.rotate_ylm_cmplx(O.rotate_ylm_cmplx(y, e)[0, 0], -e[::-1])[0, 0]
+    np.testing.assert_allclose(r, y, atol=1e-12)
+
+
+def test_correlation_argmax_recovers_rotation():
+    """tests/test_fortran_backend.py:695-722: argmax of |cc| is the grid rotation."""
+    bw = 8
+    P = O.OraclePlan(bw)
+    rng = np.random.default_rng(6)
+    f = rng.random(bw * bw) + 1j * rng.random(bw * bw)
+    a = np.arange(2 * bw) * np.pi / bw
+    b = O.beta_samples(2 * bw)
+    for _ in range(5):
+        idx = rng.integers(0, 2 * bw, 3)
+        g = O.rotate_ylm_cmplx(f, [a[idx[0]], b[idx[1]], a[idx[2]]])[0, 0]
+        cc = P.cross_correlation_ylm_cmplx(f, g)
+        am = np.array(np.unravel_index(np.argmax(np.abs(cc.T)), cc.shape))
+        assert list(am[[1, 0, 2]]) == list(idx)
+    # correlation equals the isoft of its closed-form coefficients (soft.f08:2277, :2367)
+    g = rng.random(bw * bw) + 1j * rng.random(bw * bw)
+    assert rel_l2(P.cross_correlation_ylm_cmplx(f, g), P.isoft(O.corr_coeffs(bw, f, g))) < 1e-14
+    # plant a mistake in the oracle's input: conj(g) instead of g must not pass
+    assert rel_l2(P.cross_correlation_ylm_cmplx(f, np.conj(g)), P.isoft(O.corr_coeffs(bw, f, g))) > 1e-3
+
+
+@pytest.mark.skipif(not RefLib.available(), reason="reference build oracle/_ref not present")
+def test_live_reference_build_agrees():
+    for bw in [3, 9, 17]:
+        assert np.array_equal(O.coeff_degrees(bw), RefLib.coeff_degrees(bw))
+        np.testing.assert_array_equal(O.seeds(bw), RefLib.seeds(bw))
+        for m, n in [(0, 0), (-2, 1), (bw - 1, -(bw - 1)), (1, -bw + 1)]:
+            for l in range(max(abs(m), abs(n)), bw):
+                assert O.slice_table(bw)[m + bw - 1, n + bw - 1] + l - max(abs(m), abs(n)) == \
+                    RefLib.coeff_location_mnl(m, n, l, bw)
