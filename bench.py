@@ -36,6 +36,9 @@ def main():
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=0, help="transforms in the CPU sample (0 = #threads)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 1 GiB device copy (counter calibration under rocprofv3 --pmc)")
+    ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing section")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; --gpus must be 1 (multi-GPU sharding "
@@ -79,18 +82,29 @@ def main():
     ms_per_step = ms / args.steps
     value = B * args.steps / (ms / 1e3)
 
+    if args.calib_copy:
+        src = torch.ones(2 ** 27, dtype=torch.float64, device=dev)  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     # per-kernel durations (HIP events around repeated launches on this stream)
     L = _lib.lib()
     sp = C.c_void_p(stream.cuda_stream)
     scratch = torch.empty((B, N, N, N), dtype=torch.complex128, device=dev)
     out = C.c_float()
     kern = {}
-    for name, which, a, b in [("dwt_inverse_f64", 0, c, scratch), ("dwt_forward_f64", 1, scratch, c2),
+    for name, which, a, b in [] if args.no_kernels else [("dwt_inverse_f64", 0, c, scratch), ("dwt_forward_f64", 1, scratch, c2),
                               ("fft_axis_alpha_f64", 2, f, scratch), ("fft_axis_gamma_f64", 3, f, scratch)]:
         _lib.check(L.so3ft_bench_kernel(s._handle, which, 5, C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()),
                                         B, sp, C.byref(out)))
         kern[name] = out.value
     del scratch
+    if args.no_kernels:
+        print(json.dumps({"value": B * args.steps / (ms / 1e3), "ms_per_step": ms / args.steps}), flush=True)
+        return
     dwt_flops = 4.0 * N * nc(bw) * B  # complex x real MACs over the 2bw betas, per direction
     fft_pass_bytes = 2.0 * 16 * N ** 3 * B  # one read + one write of the grid
     per_step = {"dwt_inverse_f64": kern["dwt_inverse_f64"], "dwt_forward_f64": kern["dwt_forward_f64"],
