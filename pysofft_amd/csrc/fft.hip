@@ -9,6 +9,8 @@
 // The pass along m (stride N) and the pass along n (stride N^2) are the same
 // kernel with different strides.  Orders that carry no data (the Nyquist order
 // bw and |order| > lmax) are skipped on load.
+#include <cstdlib>
+
 #include "so3ft_internal.h"
 
 namespace so3 {
@@ -142,6 +144,8 @@ static int fft_cols(int N)
 hipError_t launch_fft_axis(const FftDesc &fd, const double2 *tw, int bw, int lmax, const void *in, void *out,
                            const FftAxis &ax, int T, hipStream_t s)
 {
+    if (fft_reg_supported(fd.N) && !getenv("SO3FT_FFT_STOCKHAM"))
+        return launch_fft_reg(fd.N, tw, bw, lmax, in, out, ax, T, 0, 0, 0, fd.N, s);
     FftK k;
     k.N = fd.N;
     k.C = fft_cols(fd.N);

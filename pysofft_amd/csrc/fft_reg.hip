@@ -1,0 +1,221 @@
+// Register-resident two-step FFT pass for power-of-two line lengths N = R1 * R2.
+//
+// For one line family (rows at element stride in_r / out_r, beta columns contiguous)
+// a workgroup owns C beta columns.  Step 1: thread (n2, c) loads the R1 elements
+// x[R2*n1 + n2], n1 < R1, straight into registers, runs a length-R1 DFT, multiplies
+// by W_N^{n2 k1} and writes y[k1][n2] to LDS.  Step 2 (one barrier later): thread
+// (k1, c) reads y[k1][n2], n2 < R2, runs a length-R2 DFT and stores
+// X[k1 + R1*k2] directly from registers.  All arithmetic is FP64; the storage type
+// T is double2 (complex128) or float2 (complex64) so the same kernel serves the FP64
+// path and the mixed-precision path (FP64 butterflies on complex64 data).
+#include "so3ft_internal.h"
+
+namespace so3 {
+
+struct FftRegK {
+    int N, C, bw, lmax;
+    int64_t in_t, in_o, in_r, out_t, out_o, out_r;
+    int mask_rows, mask_outer, in_real, out_real, sign;
+    int col0, ncols;   // beta column window [col0, col0+ncols) (slab processing)
+    double scale;
+};
+
+__device__ __forceinline__ bool order_ok2(int i, int N, int bw, int lmax)
+{
+    if (i == bw) return false;
+    const int m = (i < bw) ? i : i - N;
+    return (m < 0 ? -m : m) <= lmax;
+}
+
+__device__ __forceinline__ double2 cmul2(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// In-register DFT of length R (power of two) by recursive radix-2 decimation in
+// time; W_R^j = tw[j * (N/R)] with tw the sign-adjusted length-N table in LDS.
+template <int R>
+struct Dft {
+    __device__ __forceinline__ static void run(double2 *v, const double2 *tw, int NR, int sign)
+    {
+        double2 e[R / 2], o[R / 2];
+#pragma unroll
+        for (int i = 0; i < R / 2; ++i) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
+        Dft<R / 2>::run(e, tw, NR * 2, sign);
+        Dft<R / 2>::run(o, tw, NR * 2, sign);
+#pragma unroll
+        for (int k = 0; k < R / 2; ++k) {
+            double2 t;
+            if (k == 0) t = o[0];
+            else if (4 * k == R) t = (sign < 0) ? make_double2(o[k].y, -o[k].x) : make_double2(-o[k].y, o[k].x);
+            else t = cmul2(o[k], tw[k * NR]);
+            v[k] = make_double2(e[k].x + t.x, e[k].y + t.y);
+            v[k + R / 2] = make_double2(e[k].x - t.x, e[k].y - t.y);
+        }
+    }
+};
+template <>
+struct Dft<1> {
+    __device__ __forceinline__ static void run(double2 *, const double2 *, int, int) {}
+};
+
+template <class T> struct Scalar;
+template <> struct Scalar<double2> { typedef double type; };
+template <> struct Scalar<float2> { typedef float type; };
+
+template <class T> __device__ __forceinline__ double2 ld(const T *p, int64_t off);
+template <> __device__ __forceinline__ double2 ld<double2>(const double2 *p, int64_t off) { return p[off]; }
+template <> __device__ __forceinline__ double2 ld<float2>(const float2 *p, int64_t off)
+{
+    const float2 v = p[off];
+    return make_double2((double)v.x, (double)v.y);
+}
+template <class T> __device__ __forceinline__ void st(T *p, int64_t off, double2 v);
+template <> __device__ __forceinline__ void st<double2>(double2 *p, int64_t off, double2 v) { p[off] = v; }
+template <> __device__ __forceinline__ void st<float2>(float2 *p, int64_t off, double2 v)
+{
+    p[off] = make_float2((float)v.x, (float)v.y);
+}
+
+// TI: input element type (double2/float2; in_real reads the .x-sized scalar instead)
+// TO: output element type
+template <int R1, int R2, class TI, class TO>
+__global__ __launch_bounds__(256) void fft_reg_kernel(const void *__restrict__ in, void *__restrict__ out,
+                                                      const double2 *__restrict__ twg, FftRegK k)
+{
+    constexpr int N = R1 * R2;
+    extern __shared__ double2 sm[];
+    double2 *tw = sm;           // N
+    double2 *Y = sm + N;        // [k1][n2][c]  N * C
+    const int C = k.C, tid = threadIdx.x;
+    const int c = tid % C, grp = tid / C;
+    const int64_t o = blockIdx.y, t = blockIdx.z;
+    const int cb = k.col0 + blockIdx.x * C;
+    if (k.mask_outer && !order_ok2((int)o, N, k.bw, k.lmax)) return;
+
+    for (int i = tid; i < N; i += blockDim.x) {
+        double2 w = twg[i];
+        if (k.sign > 0) w.y = -w.y;
+        tw[i] = w;
+    }
+    const bool colok = (cb + c) < (k.col0 + k.ncols);
+    const int64_t ib = t * k.in_t + o * k.in_o + cb + c;
+    double2 v[R1 > R2 ? R1 : R2];
+    if (grp < R2) {
+        const int n2 = grp;
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) {
+            const int r = R2 * n1 + n2;
+            double2 x = make_double2(0.0, 0.0);
+            if (colok && (!k.mask_rows || order_ok2(r, N, k.bw, k.lmax))) {
+                const int64_t off = ib + (int64_t)r * k.in_r;
+                if (k.in_real) x.x = (double)reinterpret_cast<const typename Scalar<TI>::type *>(in)[off];
+                else x = ld<TI>(reinterpret_cast<const TI *>(in), off);
+            }
+            v[n1] = x;
+        }
+    }
+    __syncthreads();  // twiddles visible
+    if (grp < R2) {
+        const int n2 = grp;
+        Dft<R1>::run(v, tw, N / R1, k.sign);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) {
+            const double2 y = (k1 == 0 || n2 == 0) ? v[k1] : cmul2(v[k1], tw[n2 * k1]);
+            Y[(k1 * R2 + n2) * C + c] = y;
+        }
+    }
+    __syncthreads();
+    if (grp < R1) {
+        const int k1 = grp;
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) v[n2] = Y[(k1 * R2 + n2) * C + c];
+        Dft<R2>::run(v, tw, N / R2, k.sign);
+        if (colok) {
+            const int64_t ob = t * k.out_t + o * k.out_o + cb + c;
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) {
+                const int64_t off = ob + (int64_t)(k1 + R1 * k2) * k.out_r;
+                const double2 x = make_double2(v[k2].x * k.scale, v[k2].y * k.scale);
+                if (k.out_real) reinterpret_cast<typename Scalar<TO>::type *>(out)[off] = (typename Scalar<TO>::type)x.x;
+                else st<TO>(reinterpret_cast<TO *>(out), off, x);
+            }
+        }
+    }
+}
+
+static bool split_pow2(int N, int &R1, int &R2)
+{
+    switch (N) {
+    case 4: R1 = 2; R2 = 2; return true;
+    case 8: R1 = 4; R2 = 2; return true;
+    case 16: R1 = 4; R2 = 4; return true;
+    case 32: R1 = 8; R2 = 4; return true;
+    case 64: R1 = 8; R2 = 8; return true;
+    case 128: R1 = 16; R2 = 8; return true;
+    case 256: R1 = 16; R2 = 16; return true;
+    case 512: R1 = 32; R2 = 16; return true;
+    default: return false;
+    }
+}
+
+bool fft_reg_supported(int N)
+{
+    int a, b;
+    return split_pow2(N, a, b);
+}
+
+template <class TI, class TO>
+static hipError_t launch_reg_typed(int N, const double2 *tw, int bw, int lmax, const void *in, void *out,
+                                   const FftAxis &ax, int T, int col0, int ncols, hipStream_t s)
+{
+    int R1 = 0, R2 = 0;
+    if (!split_pow2(N, R1, R2)) return hipErrorInvalidValue;
+    FftRegK k;
+    k.N = N;
+    k.C = (R1 >= 32) ? 8 : 16;
+    if (k.C > N) k.C = N;
+    k.bw = bw; k.lmax = lmax;
+    k.in_t = ax.in_t; k.in_o = ax.in_o; k.in_r = ax.in_r;
+    k.out_t = ax.out_t; k.out_o = ax.out_o; k.out_r = ax.out_r;
+    k.mask_rows = ax.mask_rows; k.mask_outer = ax.mask_outer;
+    k.in_real = ax.in_real; k.out_real = ax.out_real; k.sign = ax.sign; k.scale = ax.scale;
+    k.col0 = col0; k.ncols = ncols;
+    const int threads = (R1 > R2 ? R1 : R2) * k.C;
+    const int bytes = (int)sizeof(double2) * (N + N * k.C);
+    dim3 grid((ncols + k.C - 1) / k.C, N, T), block(threads);
+#define SO3_REG(A, B)                                                                                   \
+    do {                                                                                                \
+        static bool attr = false;                                                                       \
+        if (!attr) {                                                                                    \
+            hipError_t e = hipFuncSetAttribute((const void *)fft_reg_kernel<A, B, TI, TO>,              \
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);  \
+            if (e != hipSuccess) return e;                                                              \
+            attr = true;                                                                                \
+        }                                                                                               \
+        hipLaunchKernelGGL((fft_reg_kernel<A, B, TI, TO>), grid, block, bytes, s, in, out, tw, k);     \
+    } while (0)
+    switch (N) {
+    case 4: SO3_REG(2, 2); break;
+    case 8: SO3_REG(4, 2); break;
+    case 16: SO3_REG(4, 4); break;
+    case 32: SO3_REG(8, 4); break;
+    case 64: SO3_REG(8, 8); break;
+    case 128: SO3_REG(16, 8); break;
+    case 256: SO3_REG(16, 16); break;
+    default: SO3_REG(32, 16); break;
+    }
+#undef SO3_REG
+    return hipGetLastError();
+}
+
+hipError_t launch_fft_reg(int N, const double2 *tw, int bw, int lmax, const void *in, void *out,
+                          const FftAxis &ax, int T, int in_c64, int out_c64, int col0, int ncols, hipStream_t s)
+{
+    if (!in_c64 && !out_c64) return launch_reg_typed<double2, double2>(N, tw, bw, lmax, in, out, ax, T, col0, ncols, s);
+    if (in_c64 && out_c64) return launch_reg_typed<float2, float2>(N, tw, bw, lmax, in, out, ax, T, col0, ncols, s);
+    if (in_c64) return launch_reg_typed<float2, double2>(N, tw, bw, lmax, in, out, ax, T, col0, ncols, s);
+    return launch_reg_typed<double2, float2>(N, tw, bw, lmax, in, out, ax, T, col0, ncols, s);
+}
+
+}  // namespace so3

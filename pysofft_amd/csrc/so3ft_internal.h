@@ -84,6 +84,12 @@ struct FftAxis {
 hipError_t launch_fft_axis(const FftDesc &fd, const double2 *tw, int bw, int lmax, const void *in,
                            void *out, const FftAxis &ax, int T, hipStream_t s);
 
+bool fft_reg_supported(int N);
+// register two-step pass for power-of-two N; in_c64/out_c64 select complex64 storage;
+// only beta columns [col0, col0+ncols) are processed
+hipError_t launch_fft_reg(int N, const double2 *tw, int bw, int lmax, const void *in, void *out,
+                          const FftAxis &ax, int T, int in_c64, int out_c64, int col0, int ncols, hipStream_t s);
+
 hipError_t launch_zero_invalid_columns(double2 *F, int64_t f_ts, int bw, int N, int lmax, int T,
                                        hipStream_t s);
 hipError_t launch_scale(double2 *X, int64_t n, double scale, hipStream_t s);
