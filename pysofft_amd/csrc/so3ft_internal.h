@@ -72,6 +72,14 @@ hipError_t launch_dwt_forward(const DeviceTables &t, int bw, int N, int npair, i
                               const double2 *F, int64_t f_ts, double2 *coeff, int64_t c_ts, int T,
                               hipStream_t s);
 
+// mixed precision: complex64 coefficients and DWT arrays, FP64 recurrence, FP32 MFMA
+// with d split into hi + lo, FP64 accumulation across 16-term blocks
+hipError_t launch_dwt_inverse_mixed(const DeviceTables &t, int bw, int N, int npair, int mode,
+                                    const float2 *coeff, int64_t c_ts, const float2 *aux, float2 *F, int64_t f_ts,
+                                    int T, hipStream_t s);
+hipError_t launch_dwt_forward_mixed(const DeviceTables &t, int bw, int N, int npair, int mode, const float2 *F,
+                                    int64_t f_ts, float2 *coeff, int64_t c_ts, int T, hipStream_t s);
+
 struct FftAxis {
     int64_t in_t, in_o, in_r;      // element strides: transform, outer line, fft row
     int64_t out_t, out_o, out_r;
