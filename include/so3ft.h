@@ -64,6 +64,14 @@ int so3ft_get_recurrence_type(so3ft_handle h, int64_t *rt);
 int so3ft_wigners_are_precomputed(so3ft_handle h, int *yes);
 int so3ft_get_fftw_flags(so3ft_handle h, int32_t *flags);
 
+/* ---- FP64 DWT engine selection (no counterpart in the reference, which has one
+ * CPU GEMV path): SO3FT_DWT_VALU (default) runs the contraction on the FP64 vector
+ * ALUs, SO3FT_DWT_MFMA on v_mfma_f64_16x16x4f64.  Both are FP64 throughout and give
+ * the same results to rounding.  Kept across so3ft_reset. */
+enum { SO3FT_DWT_VALU = 0, SO3FT_DWT_MFMA = 1 };
+int so3ft_set_dwt_engine(so3ft_handle h, int engine);
+int so3ft_get_dwt_engine(so3ft_handle h, int *engine);
+
 /* ---- complex transforms: py_wrap.f90:236-257, 280-301 --------------------- */
 int so3ft_isoft(so3ft_handle h, const void *coeff, void *so3func, void *stream);   /* py_isoft */
 int so3ft_soft(so3ft_handle h, const void *so3func, void *coeff, void *stream);    /* py_soft  */
@@ -113,7 +121,8 @@ int so3ft_rotate_ylm_cmplx(int64_t bw, const void *ylms, int64_t nset, const voi
  * the pipeline `reps` times on `stream` between two HIP events and returns the mean
  * duration of one launch.  which: 0 inverse DWT (a = coeffs, b = T x (2bw)^3 scratch),
  * 1 forward DWT (a = scratch, b = coeffs), 2 FFT pass along alpha (a -> b),
- * 3 FFT pass along gamma (a -> b).  T transforms per launch. */
+ * 3 FFT pass along gamma (a -> b), 4 / 5 as 0 / 1 on the VALU engine.  T transforms
+ * per launch. */
 int so3ft_bench_kernel(so3ft_handle h, int which, int reps, void *a, void *b, int64_t T, void *stream,
                        float *ms_per_launch);
 

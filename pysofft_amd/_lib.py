@@ -29,6 +29,8 @@ SIGNATURES = {
     "so3ft_get_recurrence_type": (_int, [_i64, _P(_i64)]),
     "so3ft_wigners_are_precomputed": (_int, [_i64, _P(_int)]),
     "so3ft_get_fftw_flags": (_int, [_i64, _P(_i32)]),
+    "so3ft_set_dwt_engine": (_int, [_i64, _int]),
+    "so3ft_get_dwt_engine": (_int, [_i64, _P(_int)]),
     "so3ft_isoft": (_int, [_i64, _vp, _vp, _vp]),
     "so3ft_soft": (_int, [_i64, _vp, _vp, _vp]),
     "so3ft_isoft_many": (_int, [_i64, _vp, _vp, _i64, _vp]),

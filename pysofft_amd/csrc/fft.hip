@@ -284,8 +284,14 @@ hipError_t launch_enforce_real_sym(double2 *c, int64_t c_ts, const int64_t *slic
 {
     const int W = 2 * bw - 1;
     int nb = (W * W + 255) / 256;
-    hipLaunchKernelGGL(enforce_real_sym_kernel, dim3(nb, T), dim3(256), 0, s, c, c_ts, slice, bw);
-    return hipGetLastError();
+    for (int t0 = 0; t0 < T; t0 += 32768) {  // grid y extent < 65536
+        const int nt = T - t0 < 32768 ? T - t0 : 32768;
+        hipLaunchKernelGGL(enforce_real_sym_kernel, dim3(nb, nt), dim3(256), 0, s, c + (int64_t)t0 * c_ts, c_ts, slice,
+                           bw);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 __global__ void radial_acc_kernel(double2 *acc, const double2 *x, int64_t n, double w, int first)

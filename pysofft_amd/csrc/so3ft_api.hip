@@ -40,10 +40,13 @@ struct Plan {
     int64_t recurrence = 0;
     int32_t fftw_flags = 0;
     int device = 0;
+    int dwt_engine = SO3FT_DWT_VALU;  // FP64 DWT engine (SO3FT_DWT_*)
     DeviceTables tb;
     FftDesc fft;
     double2 *scratch = nullptr;
     int64_t scratch_T = 0;
+    double2 *cstage = nullptr;  // staged mnl coefficients (correlation)
+    int64_t cstage_T = 0;
     double *red = nullptr;  // integration partials + result
 };
 
@@ -61,9 +64,10 @@ Plan *get(so3ft_handle h)
 void free_plan_device(Plan *P)
 {
     hipFree(P->tb.cosb); hipFree(P->tb.w); hipFree(P->tb.seeds); hipFree(P->tb.slice);
-    hipFree(P->tb.pairs); hipFree(P->tb.tw); hipFree(P->scratch); hipFree(P->red);
+    hipFree(P->tb.pairs); hipFree(P->tb.tw); hipFree(P->scratch); hipFree(P->red); hipFree(P->cstage);
     P->tb = DeviceTables();
     P->scratch = nullptr; P->scratch_T = 0; P->red = nullptr;
+    P->cstage = nullptr; P->cstage_T = 0;
 }
 
 template <class T>
@@ -202,12 +206,13 @@ int ensure_scratch(Plan *P, int64_t T)
     return SO3FT_OK;
 }
 
-// transforms per launch for the *_many paths: bounded scratch (<= 4 GiB)
+// transforms per launch for the *_many paths: bounded scratch (<= 4 GiB) and at
+// most 32768 transforms, which keeps every grid's z extent under 65536
 int64_t chunk_of(const Plan *P, int64_t n)
 {
     const double per = 16.0 * (double)P->N * P->N * P->N;
     int64_t c = (int64_t)(4.0 * 1024 * 1024 * 1024 / per);
-    if (c < 1) c = 1;
+    c = std::max<int64_t>(1, std::min<int64_t>(c, 32768));
     return std::min(c, n);
 }
 
@@ -251,6 +256,49 @@ int forward_fft_passes(Plan *P, const void *in, int in_real, double2 *F, int64_t
     return SO3FT_OK;
 }
 
+int ensure_cstage(Plan *P, int64_t T)
+{
+    if (P->cstage_T >= T) return SO3FT_OK;
+    hipFree(P->cstage);
+    P->cstage = nullptr; P->cstage_T = 0;
+    const size_t bytes = sizeof(double2) * (size_t)P->nc * (size_t)T;
+    if (hipMalloc((void **)&P->cstage, bytes) != hipSuccess) {
+        hipGetLastError();
+        return fail(SO3FT_ENOMEM, "coefficient staging allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    P->cstage_T = T;
+    return SO3FT_OK;
+}
+
+// inverse DWT of T transforms into F on the plan's engine; DWT_CORR takes SH pairs
+// (c, aux) with stride c_ts and stages their mnl correlation coefficients first
+int dwt_inverse(Plan *P, int mode, const double2 *c, int64_t c_ts, const double2 *aux, double2 *F, int64_t f_ts,
+                int64_t T, hipStream_t s)
+{
+    const int bw = (int)P->bw, N = (int)P->N, np = (int)P->npair;
+    if (P->dwt_engine == SO3FT_DWT_MFMA) {
+        HIPCHK(launch_dwt_inverse(P->tb, bw, N, np, mode, c, c_ts, aux, F, f_ts, (int)T, s));
+        return SO3FT_OK;
+    }
+    if (mode == DWT_CORR) {
+        if (int rc = ensure_cstage(P, T)) return rc;
+        HIPCHK(launch_corr_coeffs(P->tb, bw, c, aux, c_ts, P->cstage, P->nc, (int)T, s));
+        c = P->cstage; c_ts = P->nc; mode = DWT_COMPLEX;
+    }
+    HIPCHK(launch_dwt_inverse_valu(P->tb, bw, N, np, mode, c, c_ts, F, f_ts, (int)T, s));
+    return SO3FT_OK;
+}
+
+int dwt_forward(Plan *P, int mode, const double2 *F, int64_t f_ts, double2 *c, int64_t c_ts, int64_t T, hipStream_t s)
+{
+    const int bw = (int)P->bw, N = (int)P->N, np = (int)P->npair;
+    if (P->dwt_engine == SO3FT_DWT_MFMA)
+        HIPCHK(launch_dwt_forward(P->tb, bw, N, np, mode, F, f_ts, c, c_ts, (int)T, s));
+    else
+        HIPCHK(launch_dwt_forward_valu(P->tb, bw, N, np, mode, F, f_ts, c, c_ts, (int)T, s));
+    return SO3FT_OK;
+}
+
 int run_inverse(Plan *P, int mode, const void *coeff, int64_t c_ts, const void *aux, void *out, int out_real,
                 int64_t n, hipStream_t s)
 {
@@ -263,8 +311,7 @@ int run_inverse(Plan *P, int mode, const void *coeff, int64_t c_ts, const void *
         const int64_t T = std::min(ch, n - i0);
         const double2 *c = reinterpret_cast<const double2 *>(coeff) + i0 * c_ts;
         const double2 *ax = aux ? reinterpret_cast<const double2 *>(aux) + i0 * c_ts : nullptr;
-        HIPCHK(launch_dwt_inverse(P->tb, (int)P->bw, (int)N, (int)P->npair, mode, c, c_ts, ax, P->scratch, N3,
-                                  (int)T, s));
+        if ((rc = dwt_inverse(P, mode, c, c_ts, ax, P->scratch, N3, T, s))) return rc;
         void *o = reinterpret_cast<char *>(out) + (size_t)i0 * N3 * osz;
         if ((rc = inverse_fft_passes(P, P->scratch, o, out_real, T, s))) return rc;
     }
@@ -283,8 +330,7 @@ int run_forward(Plan *P, int mode, const void *in, int in_real, void *coeff, int
         const void *f = reinterpret_cast<const char *>(in) + (size_t)i0 * N3 * isz;
         if ((rc = forward_fft_passes(P, f, in_real, P->scratch, T, s))) return rc;
         double2 *c = reinterpret_cast<double2 *>(coeff) + i0 * P->nc;
-        HIPCHK(launch_dwt_forward(P->tb, (int)P->bw, (int)N, (int)P->npair, mode, P->scratch, N3, c, P->nc, (int)T,
-                                  s));
+        if ((rc = dwt_forward(P, mode, P->scratch, N3, c, P->nc, T, s))) return rc;
     }
     return SO3FT_OK;
 }
@@ -343,6 +389,7 @@ int so3ft_reset(so3ft_handle h, int64_t bw, int64_t lmax, int precompute_wigners
     }
     hipDeviceSynchronize();
     free_plan_device(P);
+    fresh.dwt_engine = P->dwt_engine;
     *P = fresh;
     return SO3FT_OK;
 }
@@ -393,6 +440,22 @@ int so3ft_get_fftw_flags(so3ft_handle h, int32_t *v)
     Plan *P = get(h);
     if (!P || !v) return fail(SO3FT_EHANDLE, "unknown handle");
     *v = P->fftw_flags;
+    return SO3FT_OK;
+}
+
+int so3ft_set_dwt_engine(so3ft_handle h, int engine)
+{
+    Plan *P = get(h);
+    if (!P) return fail(SO3FT_EHANDLE, "unknown handle");
+    if (engine != SO3FT_DWT_VALU && engine != SO3FT_DWT_MFMA) return fail(SO3FT_EINVAL, "unknown DWT engine");
+    P->dwt_engine = engine;
+    return SO3FT_OK;
+}
+int so3ft_get_dwt_engine(so3ft_handle h, int *v)
+{
+    Plan *P = get(h);
+    if (!P || !v) return fail(SO3FT_EHANDLE, "unknown handle");
+    *v = P->dwt_engine;
     return SO3FT_OK;
 }
 
@@ -451,8 +514,8 @@ int so3ft_inverse_wigner_trf_cmplx(so3ft_handle h, const void *coeff, void *fft_
     if (int rc = check_ptrs(coeff, fft_array)) return rc;
     const int64_t N3 = P->N * P->N * P->N;
     double2 *F = reinterpret_cast<double2 *>(fft_array);
-    HIPCHK(launch_dwt_inverse(P->tb, (int)P->bw, (int)P->N, (int)P->npair, DWT_COMPLEX,
-                              reinterpret_cast<const double2 *>(coeff), P->nc, nullptr, F, N3, 1, S(stream)));
+    if (int rc = dwt_inverse(P, DWT_COMPLEX, reinterpret_cast<const double2 *>(coeff), P->nc, nullptr, F, N3, 1, S(stream)))
+        return rc;
     HIPCHK(launch_zero_invalid_columns(F, N3, (int)P->bw, (int)P->N, (int)P->lmax, 1, S(stream)));
     return SO3FT_OK;
 }
@@ -462,9 +525,9 @@ int so3ft_forward_wigner_trf_cmplx(so3ft_handle h, const void *fft_array, void *
     GETP(h);
     if (int rc = check_ptrs(coeff, fft_array)) return rc;
     const int64_t N3 = P->N * P->N * P->N;
-    HIPCHK(launch_dwt_forward(P->tb, (int)P->bw, (int)P->N, (int)P->npair, DWT_COMPLEX,
-                              reinterpret_cast<const double2 *>(fft_array), N3, reinterpret_cast<double2 *>(coeff),
-                              P->nc, 1, S(stream)));
+    if (int rc = dwt_forward(P, DWT_COMPLEX, reinterpret_cast<const double2 *>(fft_array), N3,
+                             reinterpret_cast<double2 *>(coeff), P->nc, 1, S(stream)))
+        return rc;
     return SO3FT_OK;
 }
 
@@ -547,8 +610,7 @@ int so3ft_cross_correlation_ylm_cmplx_3d(so3ft_handle h, const void *f_lms, cons
     for (int64_t r = r_lo; r <= r_hi; ++r) {
         const double2 *f = reinterpret_cast<const double2 *>(f_lms) + r * nb;
         const double2 *g = reinterpret_cast<const double2 *>(g_lms) + r * nb;
-        HIPCHK(launch_dwt_inverse(P->tb, (int)P->bw, (int)P->N, (int)P->npair, DWT_CORR, f, nb, g, P->scratch, N3,
-                                  1, S(stream)));
+        if ((rc = dwt_inverse(P, DWT_CORR, f, nb, g, P->scratch, N3, 1, S(stream)))) return rc;
         if ((rc = inverse_fft_passes(P, P->scratch, tmp, 0, 1, S(stream)))) return rc;
         HIPCHK(launch_radial_accumulate(reinterpret_cast<double2 *>(cc), tmp, N3, rad[r] * rad[r] * inv,
                                         r == r_lo, S(stream)));
@@ -576,20 +638,31 @@ int so3ft_bench_kernel(so3ft_handle h, int which, int reps, void *a, void *b, in
             return launch_dwt_forward(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX,
                                       reinterpret_cast<const double2 *>(a), N3, reinterpret_cast<double2 *>(b),
                                       P->nc, (int)T, s);
+        case 4:
+            return launch_dwt_inverse_valu(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX,
+                                           reinterpret_cast<const double2 *>(a), P->nc,
+                                           reinterpret_cast<double2 *>(b), N3, (int)T, s);
+        case 5:
+            return launch_dwt_forward_valu(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX,
+                                           reinterpret_cast<const double2 *>(a), N3, reinterpret_cast<double2 *>(b),
+                                           P->nc, (int)T, s);
         case 2: {
             FftAxis ax{};
             ax.in_t = N3; ax.in_o = N2; ax.in_r = N; ax.out_t = N3; ax.out_o = N2; ax.out_r = N;
             ax.sign = -1; ax.scale = 1.0;
             return launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, a, b, ax, (int)T, s);
         }
-        default: {
+        case 3: {
             FftAxis ax{};
             ax.in_t = N3; ax.in_o = N; ax.in_r = N2; ax.out_t = N3; ax.out_o = N; ax.out_r = N2;
             ax.sign = -1; ax.scale = 1.0;
             return launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, a, b, ax, (int)T, s);
         }
+        default:
+            return hipErrorInvalidValue;
         }
     };
+    if (which < 0 || which > 5) return fail(SO3FT_EINVAL, "unknown kernel id");
     HIPCHK(launch());  // warm-up
     HIPCHK(hipEventRecord(e0, s));
     for (int i = 0; i < reps; ++i) HIPCHK(launch());

@@ -72,6 +72,19 @@ hipError_t launch_dwt_forward(const DeviceTables &t, int bw, int N, int npair, i
                               const double2 *F, int64_t f_ts, double2 *coeff, int64_t c_ts, int T,
                               hipStream_t s);
 
+// FP64 on the vector ALUs (lane = beta for the inverse, lane = degree for the forward)
+// (modes DWT_COMPLEX and DWT_REAL; correlation coefficients are staged by launch_corr_coeffs)
+hipError_t launch_dwt_inverse_valu(const DeviceTables &t, int bw, int N, int npair, int mode,
+                                   const double2 *coeff, int64_t c_ts, double2 *F, int64_t f_ts, int T,
+                                   hipStream_t s);
+hipError_t launch_dwt_forward_valu(const DeviceTables &t, int bw, int N, int npair, int mode,
+                                   const double2 *F, int64_t f_ts, double2 *coeff, int64_t c_ts, int T,
+                                   hipStream_t s);
+
+// mnl correlation coefficients of T pairs of SH sets (LMc, stride sh_ts) into out (stride c_ts)
+hipError_t launch_corr_coeffs(const DeviceTables &t, int bw, const double2 *f, const double2 *g, int64_t sh_ts,
+                              double2 *out, int64_t c_ts, int T, hipStream_t s);
+
 // mixed precision: complex64 coefficients and DWT arrays, FP64 recurrence, FP32 MFMA
 // with d split into hi + lo, FP64 accumulation across 16-term blocks
 hipError_t launch_dwt_inverse_mixed(const DeviceTables &t, int bw, int N, int npair, int mode,

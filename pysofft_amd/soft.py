@@ -138,6 +138,18 @@ class Soft:
         return self._get(_lib.lib().so3ft_get_fftw_flags, C.c_int32)
 
     @property
+    def dwt_engine(self):
+        """FP64 DWT engine: "valu" (vector FP64 FMA, default) or "mfma" (v_mfma_f64_16x16x4f64)."""
+        return {0: "valu", 1: "mfma"}[self._get(_lib.lib().so3ft_get_dwt_engine, C.c_int)]
+
+    @dwt_engine.setter
+    def dwt_engine(self, value):
+        code = {"valu": 0, "mfma": 1}.get(value)
+        if code is None:
+            raise ValueError("dwt_engine must be 'valu' or 'mfma'")
+        _lib.check(_lib.lib().so3ft_set_dwt_engine(self._handle, code))
+
+    @property
     def coeff_indices(self):
         if self._lmns is None:
             self._lmns = utils.get_coeff_degrees(self.bw)

@@ -242,3 +242,25 @@ def test_invalid_inputs_raise_not_abort():
     from pysofft_amd._lib import So3ftError
     with pytest.raises(So3ftError):
         _soft(300)
+
+
+@pytest.mark.parametrize("bw", [1, 2, 5, 16, 33])
+@pytest.mark.parametrize("n", [1, 3, 6, 17])
+def test_dwt_engines_agree_with_oracle(bw, n):
+    """Both FP64 DWT engines (vector FMA and MFMA) against the oracle for batch sizes that
+    exercise every transform grouping of the batched kernels (1, 3, 6, 17 transforms)."""
+    rng = np.random.default_rng(7 * bw + n)
+    s = _soft(bw)
+    P = O.OraclePlan(bw)
+    c = _rand_c(rng, (n, P.nc))
+    g = _rand_c(rng, (n,) + tuple(P.grid_shape))
+    fo, co = P.isoft_many(c), P.soft_many(g)
+    for eng in ("valu", "mfma"):
+        s.dwt_engine = eng
+        assert s.dwt_engine == eng
+        assert rel_l2(np.asarray(s.isoft_many(c)), fo) < 1e-12, eng
+        assert rel_l2(np.asarray(s.soft_many(g)), co) < 1e-12, eng
+        rr = np.asarray(s.rsoft_many(g.real))
+        assert rel_l2(rr, P.rsoft_many(g.real)) < 1e-12, eng
+        cr = np.stack([O.enforce_real_sym(ci.copy(), bw) for ci in c])
+        assert rel_l2(np.asarray(s.irsoft_many(cr)), P.irsoft_many(cr)) < 1e-12, eng
