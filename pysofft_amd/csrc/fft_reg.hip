@@ -1,4 +1,5 @@
-// Register-resident two-step FFT pass for power-of-two line lengths N = R1 * R2.
+// Register-resident two-step FFT pass for line lengths N = R1 * R2, each factor 2^a or
+// 3 * 2^a (every N = 2 bw with bw = 2^a or 3 * 2^a up to 256).
 //
 // For one line family (rows at element stride in_r / out_r, beta columns contiguous)
 // a workgroup owns C beta columns.  Step 1: thread (n2, c) loads the R1 elements
@@ -32,25 +33,37 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b)
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// In-register DFT of length R (power of two) by recursive radix-2 decimation in
-// time; W_R^j = tw[j * (N/R)] with tw the sign-adjusted length-N table in LDS.
+// In-register DFT of length R = 2^a or 3 * 2^a: recursive radix-2 decimation in time
+// down to a radix-3 (or trivial) base; W_R^j = tw[j * (N/R)] with tw the
+// sign-adjusted length-N table in LDS.
 template <int R>
 struct Dft {
     __device__ __forceinline__ static void run(double2 *v, const double2 *tw, int NR, int sign)
     {
-        double2 e[R / 2], o[R / 2];
+        if constexpr (R == 3) {
+            // X_k = sum_j x_j W_3^{jk};  W_3 = tw[NR], W_3^2 = tw[2 NR]
+            const double2 w1 = tw[NR], w2 = tw[2 * NR];
+            const double2 a = v[0], b = v[1], c = v[2];
+            const double2 b1 = cmul2(b, w1), c1 = cmul2(c, w2), b2 = cmul2(b, w2), c2 = cmul2(c, w1);
+            v[0] = make_double2(a.x + b.x + c.x, a.y + b.y + c.y);
+            v[1] = make_double2(a.x + b1.x + c1.x, a.y + b1.y + c1.y);
+            v[2] = make_double2(a.x + b2.x + c2.x, a.y + b2.y + c2.y);
+        } else {
+            static_assert(R % 2 == 0, "length must be 2^a or 3*2^a");
+            double2 e[R / 2], o[R / 2];
 #pragma unroll
-        for (int i = 0; i < R / 2; ++i) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
-        Dft<R / 2>::run(e, tw, NR * 2, sign);
-        Dft<R / 2>::run(o, tw, NR * 2, sign);
+            for (int i = 0; i < R / 2; ++i) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
+            Dft<R / 2>::run(e, tw, NR * 2, sign);
+            Dft<R / 2>::run(o, tw, NR * 2, sign);
 #pragma unroll
-        for (int k = 0; k < R / 2; ++k) {
-            double2 t;
-            if (k == 0) t = o[0];
-            else if (4 * k == R) t = (sign < 0) ? make_double2(o[k].y, -o[k].x) : make_double2(-o[k].y, o[k].x);
-            else t = cmul2(o[k], tw[k * NR]);
-            v[k] = make_double2(e[k].x + t.x, e[k].y + t.y);
-            v[k + R / 2] = make_double2(e[k].x - t.x, e[k].y - t.y);
+            for (int k = 0; k < R / 2; ++k) {
+                double2 t;
+                if (k == 0) t = o[0];
+                else if (4 * k == R) t = (sign < 0) ? make_double2(o[k].y, -o[k].x) : make_double2(-o[k].y, o[k].x);
+                else t = cmul2(o[k], tw[k * NR]);
+                v[k] = make_double2(e[k].x + t.x, e[k].y + t.y);
+                v[k + R / 2] = make_double2(e[k].x - t.x, e[k].y - t.y);
+            }
         }
     }
 };
@@ -148,12 +161,19 @@ static bool split_pow2(int N, int &R1, int &R2)
 {
     switch (N) {
     case 4: R1 = 2; R2 = 2; return true;
+    case 6: R1 = 3; R2 = 2; return true;
     case 8: R1 = 4; R2 = 2; return true;
+    case 12: R1 = 4; R2 = 3; return true;
     case 16: R1 = 4; R2 = 4; return true;
+    case 24: R1 = 6; R2 = 4; return true;
     case 32: R1 = 8; R2 = 4; return true;
+    case 48: R1 = 8; R2 = 6; return true;
     case 64: R1 = 8; R2 = 8; return true;
+    case 96: R1 = 12; R2 = 8; return true;
     case 128: R1 = 16; R2 = 8; return true;
+    case 192: R1 = 16; R2 = 12; return true;
     case 256: R1 = 16; R2 = 16; return true;
+    case 384: R1 = 24; R2 = 16; return true;
     case 512: R1 = 32; R2 = 16; return true;
     default: return false;
     }
@@ -173,7 +193,7 @@ static hipError_t launch_reg_typed(int N, const double2 *tw, int bw, int lmax, c
     if (!split_pow2(N, R1, R2)) return hipErrorInvalidValue;
     FftRegK k;
     k.N = N;
-    k.C = (R1 >= 32) ? 8 : 16;
+    k.C = (R1 >= 24) ? 8 : 16;
     if (k.C > N) k.C = N;
     k.bw = bw; k.lmax = lmax;
     k.in_t = ax.in_t; k.in_o = ax.in_o; k.in_r = ax.in_r;
@@ -197,12 +217,19 @@ static hipError_t launch_reg_typed(int N, const double2 *tw, int bw, int lmax, c
     } while (0)
     switch (N) {
     case 4: SO3_REG(2, 2); break;
+    case 6: SO3_REG(3, 2); break;
     case 8: SO3_REG(4, 2); break;
+    case 12: SO3_REG(4, 3); break;
     case 16: SO3_REG(4, 4); break;
+    case 24: SO3_REG(6, 4); break;
     case 32: SO3_REG(8, 4); break;
+    case 48: SO3_REG(8, 6); break;
     case 64: SO3_REG(8, 8); break;
+    case 96: SO3_REG(12, 8); break;
     case 128: SO3_REG(16, 8); break;
+    case 192: SO3_REG(16, 12); break;
     case 256: SO3_REG(16, 16); break;
+    case 384: SO3_REG(24, 16); break;
     default: SO3_REG(32, 16); break;
     }
 #undef SO3_REG

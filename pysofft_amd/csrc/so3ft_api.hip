@@ -335,6 +335,78 @@ int run_forward(Plan *P, int mode, const void *in, int in_real, void *coeff, int
     return SO3FT_OK;
 }
 
+// ---- mixed precision: complex64 storage, FP64 recurrence and FFT arithmetic,
+// FP32 MFMA contraction with d split into FP32 hi + lo parts
+int inverse_fft_passes_c64(Plan *P, float2 *F, float2 *out, int64_t T, hipStream_t s)
+{
+    const int64_t N = P->N, N2 = N * N, N3 = N2 * N;
+    FftAxis a{};
+    a.in_t = N3; a.in_o = N2; a.in_r = N; a.out_t = N3; a.out_o = N2; a.out_r = N;
+    a.mask_rows = 1; a.mask_outer = 1; a.sign = -1; a.scale = 1.0;
+    HIPCHK(launch_fft_reg((int)N, P->tb.tw, (int)P->bw, (int)P->lmax, F, F, a, (int)T, 1, 1, 0, (int)N, s));
+    FftAxis b{};
+    b.in_t = N3; b.in_o = N; b.in_r = N2; b.out_t = N3; b.out_o = N; b.out_r = N2;
+    b.mask_rows = 1; b.sign = -1; b.scale = 1.0 / (2.0 * 3.14159265358979323846);
+    HIPCHK(launch_fft_reg((int)N, P->tb.tw, (int)P->bw, (int)P->lmax, F, out, b, (int)T, 1, 1, 0, (int)N, s));
+    return SO3FT_OK;
+}
+
+int forward_fft_passes_c64(Plan *P, const float2 *in, float2 *F, int64_t T, hipStream_t s)
+{
+    const int64_t N = P->N, N2 = N * N, N3 = N2 * N;
+    FftAxis b{};
+    b.in_t = N3; b.in_o = N; b.in_r = N2; b.out_t = N3; b.out_o = N; b.out_r = N2;
+    b.sign = +1; b.scale = 1.0;
+    HIPCHK(launch_fft_reg((int)N, P->tb.tw, (int)P->bw, (int)P->lmax, in, F, b, (int)T, 1, 1, 0, (int)N, s));
+    FftAxis a{};
+    a.in_t = N3; a.in_o = N2; a.in_r = N; a.out_t = N3; a.out_o = N2; a.out_r = N;
+    a.mask_outer = 1; a.sign = +1;
+    a.scale = 2.0 * 3.14159265358979323846 / ((double)N * (double)N);
+    HIPCHK(launch_fft_reg((int)N, P->tb.tw, (int)P->bw, (int)P->lmax, F, F, a, (int)T, 1, 1, 0, (int)N, s));
+    return SO3FT_OK;
+}
+
+int check_c64(Plan *P)
+{
+    if (!fft_reg_supported((int)P->N))
+        return fail(SO3FT_EUNSUPPORTED, "complex64 transforms need bw = 2^a or 3*2^a (2 <= bw <= 256)");
+    return SO3FT_OK;
+}
+
+int run_inverse_c64(Plan *P, const float2 *coeff, float2 *out, int64_t n, hipStream_t s)
+{
+    if (int rc = check_c64(P)) return rc;
+    const int64_t N = P->N, N3 = N * N * N;
+    const int64_t ch = chunk_of(P, n);
+    int rc = ensure_scratch(P, (ch + 1) / 2);  // complex64 scratch: half the bytes
+    if (rc) return rc;
+    float2 *F = reinterpret_cast<float2 *>(P->scratch);
+    for (int64_t i0 = 0; i0 < n; i0 += ch) {
+        const int64_t T = std::min(ch, n - i0);
+        HIPCHK(launch_dwt_inverse_mixed(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX, coeff + i0 * P->nc,
+                                        P->nc, nullptr, F, N3, (int)T, s));
+        if ((rc = inverse_fft_passes_c64(P, F, out + i0 * N3, T, s))) return rc;
+    }
+    return SO3FT_OK;
+}
+
+int run_forward_c64(Plan *P, const float2 *in, float2 *coeff, int64_t n, hipStream_t s)
+{
+    if (int rc = check_c64(P)) return rc;
+    const int64_t N = P->N, N3 = N * N * N;
+    const int64_t ch = chunk_of(P, n);
+    int rc = ensure_scratch(P, (ch + 1) / 2);
+    if (rc) return rc;
+    float2 *F = reinterpret_cast<float2 *>(P->scratch);
+    for (int64_t i0 = 0; i0 < n; i0 += ch) {
+        const int64_t T = std::min(ch, n - i0);
+        if ((rc = forward_fft_passes_c64(P, in + i0 * N3, F, T, s))) return rc;
+        HIPCHK(launch_dwt_forward_mixed(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX, F, N3,
+                                        coeff + i0 * P->nc, P->nc, (int)T, s));
+    }
+    return SO3FT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -506,6 +578,24 @@ int so3ft_rsoft_many(so3ft_handle h, const void *so3funcs, void *coeffs, int64_t
 int so3ft_rsoft(so3ft_handle h, const void *so3func, void *coeff, void *stream)
 {
     return so3ft_rsoft_many(h, so3func, coeff, 1, stream);
+}
+
+int so3ft_isoft_many_c64(so3ft_handle h, const void *coeffs, void *so3funcs, int64_t n, void *stream)
+{
+    GETP(h);
+    if (int rc = check_ptrs(coeffs, so3funcs)) return rc;
+    if (n < 0) return fail(SO3FT_EINVAL, "n must be >= 0");
+    return run_inverse_c64(P, reinterpret_cast<const float2 *>(coeffs), reinterpret_cast<float2 *>(so3funcs), n,
+                           S(stream));
+}
+
+int so3ft_soft_many_c64(so3ft_handle h, const void *so3funcs, void *coeffs, int64_t n, void *stream)
+{
+    GETP(h);
+    if (int rc = check_ptrs(coeffs, so3funcs)) return rc;
+    if (n < 0) return fail(SO3FT_EINVAL, "n must be >= 0");
+    return run_forward_c64(P, reinterpret_cast<const float2 *>(so3funcs), reinterpret_cast<float2 *>(coeffs), n,
+                           S(stream));
 }
 
 int so3ft_inverse_wigner_trf_cmplx(so3ft_handle h, const void *coeff, void *fft_array, void *stream)

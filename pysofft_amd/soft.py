@@ -292,6 +292,18 @@ class Soft:
         return self._run(_lib.lib().so3ft_irsoft_many, coeffs, torch.complex128, (self.n_coeffs,), out,
                          torch.float64, self.grid_shape, many=True)
 
+    # -- mixed precision (no counterpart in the reference): complex64 storage, FP64
+    # recurrence and FFT butterflies, FP32 MFMA contraction with hi/lo-split d
+    def isoft_many_c64(self, coeffs, out=None):
+        torch = _torch()
+        return self._run(_lib.lib().so3ft_isoft_many_c64, coeffs, torch.complex64, (self.n_coeffs,), out,
+                         torch.complex64, self.grid_shape, many=True)
+
+    def soft_many_c64(self, so3funcs, out=None):
+        torch = _torch()
+        return self._run(_lib.lib().so3ft_soft_many_c64, so3funcs, torch.complex64, self.grid_shape, out,
+                         torch.complex64, (self.n_coeffs,), many=True)
+
     def _wrap(self, res, out):
         if out is None and isinstance(res, np.ndarray):
             return CoeffSO3(res, self.coeff_indices, coeff_order='mnl')

@@ -264,3 +264,33 @@ def test_dwt_engines_agree_with_oracle(bw, n):
         assert rel_l2(rr, P.rsoft_many(g.real)) < 1e-12, eng
         cr = np.stack([O.enforce_real_sym(ci.copy(), bw) for ci in c])
         assert rel_l2(np.asarray(s.irsoft_many(cr)), P.irsoft_many(cr)) < 1e-12, eng
+
+
+@pytest.mark.parametrize("bw", [2, 3, 8, 12, 16, 24, 32, 48, 64])
+def test_mixed_c64_vs_oracle(bw):
+    """complex64 transforms (FP32 MFMA contraction, FP64 recurrence and FFTs) against the
+    FP64 oracle on the same complex64-rounded inputs: per-transform relative L2 <= 1e-7
+    for isoft, soft and the round trip."""
+    rng = np.random.default_rng(500 + bw)
+    s = _soft(bw)
+    P = O.OraclePlan(bw)
+    n = 3
+    c = _rand_c(rng, (n, P.nc)).astype(np.complex64)
+    f = np.asarray(s.isoft_many_c64(c))
+    assert f.dtype == np.complex64
+    fo = P.isoft_many(c.astype(np.complex128))
+    g = _rand_c(rng, (n,) + tuple(P.grid_shape)).astype(np.complex64)
+    co = P.soft_many(g.astype(np.complex128))
+    cg = np.asarray(s.soft_many_c64(g))
+    c2 = np.asarray(s.soft_many_c64(f))
+    for i in range(n):
+        assert rel_l2(f[i], fo[i]) <= 1e-7, (bw, i, rel_l2(f[i], fo[i]))
+        assert rel_l2(cg[i], co[i]) <= 1e-7, (bw, i, rel_l2(cg[i], co[i]))
+        assert rel_l2(c2[i], c[i]) <= 2e-7, (bw, i, rel_l2(c2[i], c[i]))
+
+
+def test_mixed_c64_rejects_unsupported_bw():
+    from pysofft_amd import _lib
+    s = _soft(5)
+    with pytest.raises(_lib.So3ftError):
+        s.isoft_many_c64(np.zeros((1, s.n_coeffs), np.complex64))
