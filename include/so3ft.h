@@ -118,6 +118,17 @@ int so3ft_cross_correlation_ylm_cmplx_3d(so3ft_handle h, const void *f_lms, cons
                                          const double *radial_sampling_points, int64_t n_radial,
                                          int64_t r_lo, int64_t r_hi, void *stream);
 
+/* ---- batched correlation (no counterpart in the reference, whose callers loop over
+ * py_cross_correlation_ylm_cmplx): f_lms, g_lms (n, bw^2) LMc; cc (n, 2bw, 2bw, 2bw). */
+int so3ft_cross_correlation_ylm_cmplx_many(so3ft_handle h, const void *f_lms, const void *g_lms, void *cc,
+                                           int64_t n, void *stream);
+/* correlation and peak search without keeping the grids: for each pair, the first
+ * flat C-order (gamma, alpha, beta) index of max |C| into idx (DEVICE int64 (n,)) and
+ * max |C| into val (DEVICE float64 (n,), may be NULL).  The peak is the relative
+ * rotation as in tests/test_fortran_backend.py:695-722 (argmax of |corr|). */
+int so3ft_cross_correlation_argmax_many(so3ft_handle h, const void *f_lms, const void *g_lms, int64_t n,
+                                        void *idx, void *val, void *stream);
+
 /* ---- Wigner-D rotation of SH coefficient sets (LMc layout, bw^2 each) by ZYZ Euler
  * angles, pysofft/soft.py:713-764 (numpy there; no f2py counterpart):
  *   y'^l_n = e^{-i n gamma} sum_m y^l_m e^{-i m alpha} d^l_{n,m}(beta).

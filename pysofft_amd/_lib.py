@@ -48,6 +48,8 @@ SIGNATURES = {
     "so3ft_integrate_over_so3_cmplx": (_int, [_i64, _vp, _P(_dbl), _P(_dbl), _vp]),
     "so3ft_integrate_over_so3_real": (_int, [_i64, _vp, _P(_dbl), _vp]),
     "so3ft_cross_correlation_ylm_cmplx": (_int, [_i64, _vp, _vp, _vp, _vp]),
+    "so3ft_cross_correlation_ylm_cmplx_many": (_int, [_i64, _vp, _vp, _vp, _i64, _vp]),
+    "so3ft_cross_correlation_argmax_many": (_int, [_i64, _vp, _vp, _i64, _vp, _vp, _vp]),
     "so3ft_cross_correlation_ylm_cmplx_3d": (_int, [_i64, _vp, _vp, _vp, _P(_dbl), _i64, _i64, _i64, _vp]),
     "so3ft_enforce_real_sym_mnl": (_int, [_i64, _vp, _i64, _vp]),
     "so3ft_rotate_ylm_cmplx": (_int, [_i64, _vp, _i64, _vp, _i64, _int, _vp, _vp]),

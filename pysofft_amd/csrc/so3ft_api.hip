@@ -47,6 +47,9 @@ struct Plan {
     int64_t scratch_T = 0;
     double2 *cstage = nullptr;  // staged mnl coefficients (correlation)
     int64_t cstage_T = 0;
+    double *amax_v = nullptr;   // argmax partials
+    int64_t *amax_i = nullptr;
+    int64_t amax_cap = 0;
     double *red = nullptr;  // integration partials + result
 };
 
@@ -68,6 +71,8 @@ void free_plan_device(Plan *P)
     P->tb = DeviceTables();
     P->scratch = nullptr; P->scratch_T = 0; P->red = nullptr;
     P->cstage = nullptr; P->cstage_T = 0;
+    hipFree(P->amax_v); hipFree(P->amax_i);
+    P->amax_v = nullptr; P->amax_i = nullptr; P->amax_cap = 0;
 }
 
 template <class T>
@@ -679,6 +684,49 @@ int so3ft_cross_correlation_ylm_cmplx(so3ft_handle h, const void *f_lm, const vo
     GETP(h);
     if (!f_lm || !g_lm || !cc) return fail(SO3FT_EINVAL, "null pointer");
     return run_inverse(P, DWT_CORR, f_lm, P->bw * P->bw, g_lm, cc, 0, 1, S(stream));
+}
+
+int so3ft_cross_correlation_ylm_cmplx_many(so3ft_handle h, const void *f_lms, const void *g_lms, void *cc, int64_t n,
+                                           void *stream)
+{
+    GETP(h);
+    if (!f_lms || !g_lms || !cc) return fail(SO3FT_EINVAL, "null pointer");
+    if (n < 0) return fail(SO3FT_EINVAL, "n must be >= 0");
+    return run_inverse(P, DWT_CORR, f_lms, P->bw * P->bw, g_lms, cc, 0, n, S(stream));
+}
+
+int so3ft_cross_correlation_argmax_many(so3ft_handle h, const void *f_lms, const void *g_lms, int64_t n, void *idx,
+                                        void *val, void *stream)
+{
+    GETP(h);
+    if (!f_lms || !g_lms || !idx) return fail(SO3FT_EINVAL, "null pointer");
+    if (n < 0) return fail(SO3FT_EINVAL, "n must be >= 0");
+    const int64_t N3 = P->N * P->N * P->N, nb = P->bw * P->bw;
+    const int64_t ch = chunk_of(P, n);
+    int rc = ensure_scratch(P, ch);
+    if (rc) return rc;
+    const int64_t need = ch * argmax_partials(N3);
+    if (P->amax_cap < need) {
+        hipFree(P->amax_v); hipFree(P->amax_i);
+        P->amax_v = nullptr; P->amax_i = nullptr; P->amax_cap = 0;
+        if (hipMalloc((void **)&P->amax_v, sizeof(double) * need) != hipSuccess ||
+            hipMalloc((void **)&P->amax_i, sizeof(int64_t) * need) != hipSuccess) {
+            hipGetLastError();
+            return fail(SO3FT_ENOMEM, "argmax partials allocation failed");
+        }
+        P->amax_cap = need;
+    }
+    hipStream_t s = S(stream);
+    for (int64_t i0 = 0; i0 < n; i0 += ch) {
+        const int64_t T = std::min(ch, n - i0);
+        const double2 *f = reinterpret_cast<const double2 *>(f_lms) + i0 * nb;
+        const double2 *g = reinterpret_cast<const double2 *>(g_lms) + i0 * nb;
+        if ((rc = dwt_inverse(P, DWT_CORR, f, nb, g, P->scratch, N3, T, s))) return rc;
+        if ((rc = inverse_fft_passes(P, P->scratch, P->scratch, 0, T, s))) return rc;
+        HIPCHK(launch_argmax_abs(P->scratch, N3, (int)T, P->amax_v, P->amax_i, reinterpret_cast<int64_t *>(idx) + i0,
+                                 val ? reinterpret_cast<double *>(val) + i0 : nullptr, s));
+    }
+    return SO3FT_OK;
 }
 
 int so3ft_cross_correlation_ylm_cmplx_3d(so3ft_handle h, const void *f_lms, const void *g_lms, void *cc,

@@ -121,6 +121,12 @@ hipError_t launch_enforce_real_sym(double2 *c, int64_t c_ts, const int64_t *slic
 hipError_t launch_radial_accumulate(double2 *acc, const double2 *x, int64_t n, double w, int first,
                                     hipStream_t s);
 
+// per transform: first flat index (and value) of max |x| over n_per complex values;
+// pv / pi hold argmax_partials(n_per) * T partials
+int argmax_partials(int64_t n_per);
+hipError_t launch_argmax_abs(const double2 *x, int64_t n_per, int T, double *pv, int64_t *pi, int64_t *idx,
+                             double *val, hipStream_t s);
+
 int dwt_lds_bytes_inverse(int bw, int N);
 int dwt_lds_bytes_forward(int bw, int N);
 hipError_t dwt_set_attributes(int bw, int N);

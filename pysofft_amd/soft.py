@@ -356,6 +356,51 @@ class Soft:
                                                                    _stream_ptr(self.device)))
             return self._finish(o, f_lm, out)
 
+    # -- batched correlation (no counterpart in the reference, whose callers loop) ----
+    def cross_correlation_ylm_cmplx_many(self, f_lms, g_lms, out=None):
+        """(n, bw^2) LMc pairs -> (n, 2bw, 2bw, 2bw) correlation grids."""
+        torch = _torch()
+        nb = self.bw ** 2
+        with torch.cuda.device(self.device):
+            f = _Dev(f_lms, torch.complex128, self.device, None)
+            g = _Dev(g_lms, torch.complex128, self.device, None)
+            if f.t.ndim != 2 or f.t.shape[1] != nb or tuple(g.t.shape) != tuple(f.t.shape):
+                raise ValueError(f"expected two (n, {nb}) arrays")
+            n = f.t.shape[0]
+            o = torch.empty((n,) + self.grid_shape, dtype=torch.complex128, device=self.device)
+            _lib.check(_lib.lib().so3ft_cross_correlation_ylm_cmplx_many(
+                self._handle, f.ptr, g.ptr, C.c_void_p(o.data_ptr()), n, _stream_ptr(self.device)))
+            return self._finish(o, f_lms, out)
+
+    def correlation_peaks(self, f_lms, g_lms):
+        """Correlate (n, bw^2) pairs and locate each peak of |C| without keeping the grids.
+
+        Returns (index, value): index (n, 3) int64 grid indices (i_alpha, i_beta, i_gamma)
+        of the peak, i.e. the rotation (alpha, beta, gamma) = euler_angles at those
+        indices, and value (n,) = max |C|.  Arrays are torch CUDA tensors for torch
+        inputs, numpy otherwise."""
+        torch = _torch()
+        nb = self.bw ** 2
+        N = 2 * self.bw
+        with torch.cuda.device(self.device):
+            f = _Dev(f_lms, torch.complex128, self.device, None)
+            g = _Dev(g_lms, torch.complex128, self.device, None)
+            if f.t.ndim != 2 or f.t.shape[1] != nb or tuple(g.t.shape) != tuple(f.t.shape):
+                raise ValueError(f"expected two (n, {nb}) arrays")
+            n = f.t.shape[0]
+            idx = torch.empty(n, dtype=torch.int64, device=self.device)
+            val = torch.empty(n, dtype=torch.float64, device=self.device)
+            _lib.check(_lib.lib().so3ft_cross_correlation_argmax_many(
+                self._handle, f.ptr, g.ptr, n, C.c_void_p(idx.data_ptr()), C.c_void_p(val.data_ptr()),
+                _stream_ptr(self.device)))
+            # flat C-order (gamma, alpha, beta) -> (i_alpha, i_beta, i_gamma)
+            ig, rem = idx // (N * N), idx % (N * N)
+            ia, ib = rem // N, rem % N
+            out = torch.stack([ia, ib, ig], dim=1)
+            if _is_torch(f_lms):
+                return out, val
+            return out.cpu().numpy(), val.cpu().numpy()
+
     def cross_correlation_ylm_cmplx_3d(self, f_lms, g_lms, out=None, radial_sampling_points=None,
                                        radial_limits=None, use_mp=False):
         torch = _torch()

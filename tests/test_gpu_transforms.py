@@ -294,3 +294,75 @@ def test_mixed_c64_rejects_unsupported_bw():
     s = _soft(5)
     with pytest.raises(_lib.So3ftError):
         s.isoft_many_c64(np.zeros((1, s.n_coeffs), np.complex64))
+
+
+@pytest.mark.parametrize("engine", ["valu", "mfma"])
+def test_cross_correlation_many_vs_oracle(engine):
+    bw, n = 10, 5
+    rng = np.random.default_rng(31)
+    s = _soft(bw)
+    s.dwt_engine = engine
+    P = O.OraclePlan(bw)
+    f = _rand_c(rng, (n, bw * bw))
+    g = _rand_c(rng, (n, bw * bw))
+    got = np.asarray(s.cross_correlation_ylm_cmplx_many(f, g))
+    assert got.shape == (n,) + tuple(P.grid_shape)
+    for i in range(n):
+        assert rel_l2(got[i], P.cross_correlation_ylm_cmplx(f[i], g[i])) < 1e-12, i
+
+
+def test_correlation_peaks_find_grid_rotations():
+    """g = exact Wigner-D rotation of f by a grid rotation: the fused correlation + peak
+    search returns that rotation's grid indices (the reference's own check,
+    tests/test_fortran_backend.py:695-722), and max|C| equals the grid maximum."""
+    bw, n = 16, 12
+    rng = np.random.default_rng(16)
+    from pysofft_amd import Soft
+    s = _soft(bw)
+    a = np.arange(2 * bw) * np.pi / bw
+    b = O.beta_samples(2 * bw)
+    f = rng.random((n, bw * bw)) + 1j * rng.random((n, bw * bw))
+    idx = rng.integers(0, 2 * bw, (n, 3))
+    e = np.stack([a[idx[:, 0]], b[idx[:, 1]], a[idx[:, 2]]], 1)
+    g = Soft.rotate_ylm_cmplx(f, e)           # (n rotations, n sets, bw^2)
+    g = np.stack([g[i, i] for i in range(n)])
+    peak, val = s.correlation_peaks(f, g)
+    np.testing.assert_array_equal(peak, idx)
+    cc = np.asarray(s.cross_correlation_ylm_cmplx_many(f, g))
+    np.testing.assert_allclose(val, np.abs(cc).reshape(n, -1).max(1), rtol=1e-14)
+
+
+def test_workload_bw256_real_round_trip():
+    """Real FP64 transform at bw=256: Hermitian-symmetric coefficients (iid N(0,1) re/im
+    scaled by (2l+1)^-1/2), irsoft onto the 512^3 float64 grid, rsoft back: relative L2
+    <= 1e-11 for the round trip; grid samples of the l <= 3 part against direct sums with
+    the Jacobi-formula d (oracle/brute.py)."""
+    from pysofft_amd import Soft
+    from oracle import brute
+    bw = 256
+    s = Soft(bw)
+    rng = np.random.default_rng(256)
+    nc = s.n_coeffs
+    deg = O.coeff_degrees(bw)
+    ls = deg[:, 0]
+    c = (rng.standard_normal(nc) + 1j * rng.standard_normal(nc)) / np.sqrt(2 * ls + 1)
+    O.enforce_real_sym(c, bw)
+    f = s.irsoft(torch.from_numpy(c).cuda())
+    assert f.dtype == torch.float64 and tuple(f.shape) == (512, 512, 512)
+    c2 = s.rsoft(f)
+    assert rel_l2(c2.cpu().numpy(), c) <= 1e-11
+    del f, c2
+    sel = np.nonzero(ls <= 3)[0]
+    lo = np.zeros(nc, np.complex128)
+    lo[sel] = c[sel]
+    fl = s.irsoft(torch.from_numpy(lo).cuda())
+    a, b, _ = brute.grid_angles(bw)
+    for ig, ia, ib in [(0, 0, 0), (17, 200, 311), (511, 255, 3), (128, 384, 77)]:
+        v = 0j
+        for i in sel:
+            l, m, n = deg[i]
+            v += lo[i] * np.exp(-1j * m * a[ia]) * brute.wigner_d_norm(l, m, n, b[ib:ib + 1])[0] * \
+                np.exp(-1j * n * a[ig]) / (2 * np.pi)
+        assert abs(v.imag) < 1e-12
+        got = fl[ig, ia, ib].item()
+        assert abs(got - v.real) <= 1e-12 * max(1.0, abs(v)), (ig, ia, ib, got, v)
