@@ -1,15 +1,26 @@
-"""Benchmark: batched isoft+soft at bw=128 on one MI355X.
+"""Benchmark: batched isoft+soft at bw=128 on one MI355X, plus the alignment workload.
 
-A step is one isoft_many + one soft_many over a batch of synthetic complex
-transforms (coefficients iid N(0,1)) already resident in HBM.  `value` counts
-isoft+soft pairs per second.  The JSON line also carries the roofline of the
-dominant kernel (per-launch duration from HIP events on the launch stream) and
-the CPU baseline (the project's C/OpenMP restatement on this host, bounded sample).
+Headline step: one isoft_many + one soft_many over a batch of 16 complex transforms at
+bw=128 on the mixed-precision path (complex64 storage, FP32 MFMA contraction with
+hi/lo-split Wigner d, FP64 recurrence and FFT arithmetic), inputs resident in HBM.
+`value` counts isoft+soft pairs per second.  The same batch on the plain FP64 path is
+timed beside it, and the mixed results' relative L2 error against it is reported per
+transform (the FP64 path itself is pinned against the oracle by the test suite).
+
+The JSON line also carries
+  - roofline: the dominant kernel of the headline step, its per-launch time measured
+    with HIP events on the launch stream; HBM traffic per launch from the committed
+    rocprofv3 counter summary when one exists for that kernel;
+  - alignment: rotational cross-correlation of 1024 pairs at bw=64 (Haar-random
+    rotations, SNR 10) with peak search, pairs/s and angular error;
+  - cpu_baseline: the C/OpenMP oracle (FP64 isoft_many + soft_many at bw=128) on this
+    host's cores over a bounded sample.
 """
 import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -18,13 +29,32 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "isoft+soft transforms/s at bw=128 (MFMA vs FP64, relL2≤1e-7); alignments/s"
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
-BASELINE_PAIRS_PER_S = 0.574  # 1/(0.8466+0.8962 s), 1 core, bw=128
+PEAK_FP32_MFMA_TFLOPS = 157.3
+PUBLISHED_PAIRS_PER_S = 0.574  # 1/(0.8466 s + 0.8962 s): docs/speed.md:20-21, bw=128, one core
 
 
 def nc(bw):
     return (4 * bw ** 3 - bw) // 3
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch for `kernel` from profiles/*traffic*.md lines of the form
+    `| kernel | fetch_bytes | write_bytes | ... |` (counter values already scaled)."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for name in sorted(os.listdir(pdir)):
+        if "traffic" not in name or not name.endswith(".md"):
+            continue
+        with open(os.path.join(pdir, name)) as fh:
+            for line in fh:
+                m = re.match(r"\|\s*`?([A-Za-z0-9_]+)`?\s*\|\s*([0-9.eE+]+)\s*\|\s*([0-9.eE+]+)\s*\|", line)
+                if m and m.group(1) == kernel:
+                    return float(m.group(2)) + float(m.group(3))
+    return None
 
 
 def main():
@@ -35,14 +65,17 @@ def main():
     ap.add_argument("--bw", type=int, default=128)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=0, help="transforms in the CPU sample (0 = #threads)")
+    ap.add_argument("--cpu-sample", type=int, default=0, help="transforms in the CPU sample (0 = 2 x threads)")
+    ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing section")
+    ap.add_argument("--no-align", action="store_true", help="skip the alignment workload")
+    ap.add_argument("--align-pairs", type=int, default=1024)
+    ap.add_argument("--align-bw", type=int, default=64)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration under rocprofv3 --pmc)")
-    ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing section")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on exactly one GPU; --gpus must be 1 (multi-GPU sharding "
-                                   "is design-only for this build)"}))
+        print(json.dumps({"error": "this benchmark measures one GPU: --gpus must be 1; sharding across GPUs is "
+                                   "described in the design notes and not run"}))
         sys.exit(2)
 
     import torch
@@ -55,32 +88,65 @@ def main():
     s = Soft(bw)
     g = torch.Generator(device="cpu").manual_seed(1234)
     c = torch.complex(torch.randn(B, nc(bw), generator=g, dtype=torch.float64),
-                      torch.randn(B, nc(bw), generator=g, dtype=torch.float64)).to(dev)
-    f = torch.empty((B, N, N, N), dtype=torch.complex128, device=dev)
-    c2 = torch.empty_like(c)
-    print(f"# bw={bw} batch={B}: {B * N ** 3 * 16 / 2 ** 30:.2f} GiB grids", file=sys.stderr, flush=True)
+                      torch.randn(B, nc(bw), generator=g, dtype=torch.float64))
+    c32 = c.to(torch.complex64).to(dev)     # complex64 inputs of the mixed path
+    c64 = c32.to(torch.complex128)          # the same values for the FP64 path
+    f32 = torch.empty((B, N, N, N), dtype=torch.complex64, device=dev)
+    r32 = torch.empty_like(c32)
+    print(f"# bw={bw} batch={B}", file=sys.stderr, flush=True)
 
-    def step():
-        s.isoft_many(c, out=f)
-        s.soft_many(f, out=c2)
+    def mixed_step():
+        s.isoft_many_c64(c32, out=f32)
+        s.soft_many_c64(f32, out=r32)
+
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(step, k):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(k):
+            step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), time.perf_counter() - t0
 
     for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    err = (torch.linalg.vector_norm(c2 - c) / torch.linalg.vector_norm(c)).item()
-    stream = torch.cuda.current_stream(dev)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1)
-    ms_per_step = ms / args.steps
+        mixed_step()
+    ms, wall = timed(mixed_step, args.steps)
     value = B * args.steps / (ms / 1e3)
+    print(f"# mixed: {ms / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # FP64 path on the same (complex64-rounded) inputs: throughput and the reference
+    # results for the mixed path's error
+    f64 = torch.empty((B, N, N, N), dtype=torch.complex128, device=dev)
+    r64 = torch.empty_like(c64)
+
+    def fp64_step():
+        s.isoft_many(c64, out=f64)
+        s.soft_many(f64, out=r64)
+
+    for _ in range(max(1, args.warmup)):
+        fp64_step()
+    ms64, _ = timed(fp64_step, max(1, args.steps // 2))
+    fp64_value = B * max(1, args.steps // 2) / (ms64 / 1e3)
+    print(f"# fp64: {ms64 / max(1, args.steps // 2):.3f} ms/step", file=sys.stderr, flush=True)
+
+    def rel(a, b):
+        return [((a[i].to(torch.complex128) - b[i]).norm() / b[i].norm()).item() for i in range(B)]
+
+    s.isoft_many(c64, out=f64)
+    s.isoft_many_c64(c32, out=f32)
+    err_isoft = max(rel(f32, f64))
+    fin = f64.to(torch.complex64)
+    s.soft_many(fin.to(torch.complex128), out=r64)
+    s.soft_many_c64(fin, out=r32)
+    err_soft = max(rel(r32, r64))
+    s.isoft_many_c64(c32, out=f32)
+    s.soft_many_c64(f32, out=r32)
+    err_rt = max(rel(r32, c64))
+    del fin
 
     if args.calib_copy:
         src = torch.ones(2 ** 27, dtype=torch.float64, device=dev)  # 1 GiB
@@ -90,45 +156,79 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    # per-kernel durations (HIP events around repeated launches on this stream)
-    L = _lib.lib()
-    sp = C.c_void_p(stream.cuda_stream)
-    scratch = torch.empty((B, N, N, N), dtype=torch.complex128, device=dev)
-    out = C.c_float()
-    kern = {}
-    for name, which, a, b in [] if args.no_kernels else [("dwt_inverse_f64", 0, c, scratch), ("dwt_forward_f64", 1, scratch, c2),
-                              ("fft_axis_alpha_f64", 2, f, scratch), ("fft_axis_gamma_f64", 3, f, scratch)]:
-        _lib.check(L.so3ft_bench_kernel(s._handle, which, 5, C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()),
-                                        B, sp, C.byref(out)))
-        kern[name] = out.value
-    del scratch
-    if args.no_kernels:
-        print(json.dumps({"value": B * args.steps / (ms / 1e3), "ms_per_step": ms / args.steps}), flush=True)
-        return
-    dwt_flops = 4.0 * N * nc(bw) * B  # complex x real MACs over the 2bw betas, per direction
-    fft_pass_bytes = 2.0 * 16 * N ** 3 * B  # one read + one write of the grid
-    per_step = {"dwt_inverse_f64": kern["dwt_inverse_f64"], "dwt_forward_f64": kern["dwt_forward_f64"],
-                "fft_axis_alpha_f64": 2 * kern["fft_axis_alpha_f64"], "fft_axis_gamma_f64": 2 * kern["fft_axis_gamma_f64"]}
-    dom = max(per_step, key=per_step.get)
-    if dom.startswith("dwt"):
-        ach = dwt_flops / (kern[dom] * 1e-3) / 1e12
-        roof = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / PEAK_FP64_TFLOPS, "traffic": None}
-    else:
-        ach = fft_pass_bytes / (kern[dom] * 1e-3) / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": ach / PEAK_HBM_GBS, "traffic": None}
-    kernels = {k: {"ms_per_launch": v} for k, v in kern.items()}
-    for k in ("dwt_inverse_f64", "dwt_forward_f64"):
-        kernels[k]["tflops"] = dwt_flops / (kern[k] * 1e-3) / 1e12
-    for k in ("fft_axis_alpha_f64", "fft_axis_gamma_f64"):
-        kernels[k]["gbs"] = fft_pass_bytes / (kern[k] * 1e-3) / 1e9
+    kernels, roof = {}, None
+    if not args.no_kernels:
+        L = _lib.lib()
+        sp = C.c_void_p(stream.cuda_stream)
+        out = C.c_float()
+        dwt_flops = 4.0 * N * nc(bw) * B        # complex x real MACs over the 2bw betas, per direction
+        plan = [("dwt_inverse_mixed", 6, c32, f32, "dwt", 8), ("dwt_forward_mixed", 7, f32, r32, "dwt", 8),
+                ("fft_alpha_c64", 8, f32, f32, "fft", 8), ("fft_gamma_c64", 9, f32, f32, "fft", 8),
+                ("dwt_inverse_f64", 4, c64, f64, "dwt", 16), ("dwt_forward_f64", 5, f64, r64, "dwt", 16),
+                ("fft_alpha_f64", 2, f64, f64, "fft", 16), ("fft_gamma_f64", 3, f64, f64, "fft", 16)]
+        for name, which, a, b, kind, esz in plan:
+            _lib.check(L.so3ft_bench_kernel(s._handle, which, 5, C.c_void_p(a.data_ptr()),
+                                            C.c_void_p(b.data_ptr()), B, sp, C.byref(out)))
+            t = out.value
+            k = {"ms_per_launch": t}
+            if kind == "dwt":
+                k["tflops"] = dwt_flops / (t * 1e-3) / 1e12
+            else:
+                k["gbs"] = 2.0 * esz * N ** 3 * B / (t * 1e-3) / 1e9   # one read + one write of the grid
+            kernels[name] = k
+        # share of the headline step: each FFT pass runs twice per isoft+soft
+        per_step = {"dwt_inverse_mixed": kernels["dwt_inverse_mixed"]["ms_per_launch"],
+                    "dwt_forward_mixed": kernels["dwt_forward_mixed"]["ms_per_launch"],
+                    "fft_alpha_c64": 2 * kernels["fft_alpha_c64"]["ms_per_launch"],
+                    "fft_gamma_c64": 2 * kernels["fft_gamma_c64"]["ms_per_launch"]}
+        dom = max(per_step, key=per_step.get)
+        t = kernels[dom]["ms_per_launch"]
+        if dom.startswith("dwt"):
+            ach = dwt_flops / (t * 1e-3) / 1e12
+            roof = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": PEAK_FP32_MFMA_TFLOPS,
+                    "unit": "TFLOP/s", "frac": ach / PEAK_FP32_MFMA_TFLOPS}
+        else:
+            ach = 2.0 * 8 * N ** 3 * B / (t * 1e-3) / 1e9
+            roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": ach / PEAK_HBM_GBS}
+        roof["traffic"] = traffic_from_profiles(dom)
+        roof["share_of_step"] = per_step[dom] / sum(per_step.values())
+    del f64, r64
+
+    align = None
+    if not args.no_align:
+        from pysofft_amd import workloads as W
+        abw, npair = args.align_bw, args.align_pairs
+        rng = np.random.default_rng(4)
+        fl, gl, eul = W.rotated_pairs(npair, abw, rng, snr=10.0)
+        sa = Soft(abw)
+        ft, gt = torch.from_numpy(fl).to(dev), torch.from_numpy(gl).to(dev)
+        sa.correlation_peaks(ft[:8], gt[:8])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        idx, _ = sa.correlation_peaks(ft, gt)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        idx = idx.cpu().numpy()
+        a = np.arange(2 * abw) * np.pi / abw
+        b = (np.arange(2 * abw) + 0.5) * np.pi / (2 * abw)
+        found = np.stack([a[idx[:, 0]], b[idx[:, 1]], a[idx[:, 2]]], 1)
+        err = W.angular_distance(W.euler_to_matrix(found), W.euler_to_matrix(eul))
+        step = np.pi / abw
+        align = {"workload": f"{npair} pairs at bw={abw}, Haar-uniform rotations via exact Wigner-D, SNR 10, "
+                             f"FP64 correlation + peak search",
+                 "full_correlation_alignments_per_s": npair / dt,
+                 "angular_error_deg_median": float(np.degrees(np.median(err))),
+                 "angular_error_deg_max": float(np.degrees(err.max())),
+                 "grid_step_deg": float(np.degrees(step)),
+                 "fraction_within_one_step": float(np.mean(err <= step))}
+        del sa, ft, gt
 
     cpu = None
     if not args.no_cpu:
         import oracle as O
         nthr = O.num_threads()
-        ns = args.cpu_sample or nthr
+        ns = args.cpu_sample or 2 * nthr
         P = O.OraclePlan(bw)
         rng = np.random.default_rng(5)
         cc = rng.standard_normal((ns, P.nc)) + 1j * rng.standard_normal((ns, P.nc))
@@ -137,18 +237,20 @@ def main():
         P.soft_many(ff)
         dt = time.perf_counter() - t
         cpu = {"value": ns / dt, "unit": "isoft+soft pairs/s", "cores": nthr, "kind": "port",
-               "sample": f"{ns} complex FP64 isoft+soft pairs at bw={bw} (isoft_many+soft_many, OpenMP over "
-                         f"transforms, {nthr} threads), {dt:.1f} s"}
+               "sample": f"{ns} complex FP64 isoft+soft pairs at bw={bw} (C restatement, isoft_many + soft_many, "
+                         f"OpenMP over transforms, {nthr} threads), {dt:.1f} s"}
 
     line = {
-        "metric": "isoft+soft transforms/s at bw=128 (MFMA vs FP64, relL2<=1e-7); alignments/s",
-        "value": value, "unit": "isoft+soft pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": value / BASELINE_PAIRS_PER_S if bw == 128 else None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"batched complex isoft+soft, bw={bw}, homogeneous batch of {B}, FP64 (plain method)",
-                   "bw": bw, "batch": B, "precision": "fp64", "round_trip_relL2": err, "wall_s": wall},
-        "roofline": roof, "kernels": kernels, "cpu_baseline": cpu,
+        "metric": METRIC, "value": value, "unit": "isoft+soft pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": value / PUBLISHED_PAIRS_PER_S if bw == 128 else None,
+        "dtype": "f32 MFMA contraction, f64 recurrence/FFT, c64 storage", "data": "synthetic",
+        "config": {"workload": f"batched complex isoft+soft, bw={bw}, homogeneous batch of {B}, complex64 "
+                               f"coefficients iid N(0,1) (mixed precision)",
+                   "bw": bw, "batch": B,
+                   "relL2_vs_fp64_max": {"isoft": err_isoft, "soft": err_soft, "round_trip": err_rt},
+                   "fp64_pairs_per_s": fp64_value, "mixed_over_fp64": value / fp64_value, "wall_s": wall},
+        "roofline": roof, "kernels": kernels, "alignment": align, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

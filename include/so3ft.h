@@ -141,8 +141,9 @@ int so3ft_rotate_ylm_cmplx(int64_t bw, const void *ylms, int64_t nset, const voi
  * the pipeline `reps` times on `stream` between two HIP events and returns the mean
  * duration of one launch.  which: 0 inverse DWT (a = coeffs, b = T x (2bw)^3 scratch),
  * 1 forward DWT (a = scratch, b = coeffs), 2 FFT pass along alpha (a -> b),
- * 3 FFT pass along gamma (a -> b), 4 / 5 as 0 / 1 on the VALU engine.  T transforms
- * per launch. */
+ * 3 FFT pass along gamma (a -> b), 4 / 5 as 0 / 1 on the VALU engine, 6 / 7 the
+ * mixed-precision (complex64) inverse / forward DWT, 8 / 9 the complex64 FFT passes
+ * along alpha / gamma.  T transforms per launch. */
 int so3ft_bench_kernel(so3ft_handle h, int which, int reps, void *a, void *b, int64_t T, void *stream,
                        float *ms_per_launch);
 

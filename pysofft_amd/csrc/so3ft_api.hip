@@ -796,11 +796,28 @@ int so3ft_bench_kernel(so3ft_handle h, int which, int reps, void *a, void *b, in
             ax.sign = -1; ax.scale = 1.0;
             return launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, a, b, ax, (int)T, s);
         }
+        case 6:
+            return launch_dwt_inverse_mixed(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX,
+                                            reinterpret_cast<const float2 *>(a), P->nc, nullptr,
+                                            reinterpret_cast<float2 *>(b), N3, (int)T, s);
+        case 7:
+            return launch_dwt_forward_mixed(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX,
+                                            reinterpret_cast<const float2 *>(a), N3, reinterpret_cast<float2 *>(b),
+                                            P->nc, (int)T, s);
+        case 8:
+        case 9: {
+            FftAxis ax{};
+            ax.in_t = N3; ax.out_t = N3; ax.sign = -1; ax.scale = 1.0;
+            if (which == 8) { ax.in_o = N2; ax.in_r = N; ax.out_o = N2; ax.out_r = N; }
+            else { ax.in_o = N; ax.in_r = N2; ax.out_o = N; ax.out_r = N2; }
+            return launch_fft_reg((int)N, P->tb.tw, (int)P->bw, (int)P->lmax, a, b, ax, (int)T, 1, 1, 0, (int)N, s);
+        }
         default:
             return hipErrorInvalidValue;
         }
     };
-    if (which < 0 || which > 5) return fail(SO3FT_EINVAL, "unknown kernel id");
+    if (which < 0 || which > 9) return fail(SO3FT_EINVAL, "unknown kernel id");
+    if (which >= 8 && !fft_reg_supported((int)N)) return fail(SO3FT_EUNSUPPORTED, "no complex64 FFT for this bw");
     HIPCHK(launch());  // warm-up
     HIPCHK(hipEventRecord(e0, s));
     for (int i = 0; i < reps; ++i) HIPCHK(launch());
