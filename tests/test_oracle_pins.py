@@ -203,3 +203,21 @@ def test_live_reference_build_agrees():
             for l in range(max(abs(m), abs(n)), bw):
                 assert O.slice_table(bw)[m + bw - 1, n + bw - 1] + l - max(abs(m), abs(n)) == \
                     RefLib.coeff_location_mnl(m, n, l, bw)
+
+
+def test_correlation_scale_at_true_rotation():
+    """Scale and phase of the correlation, pinned by Parseval: for g = D(R0) f with R0 a
+    grid rotation, C(R0) = <D(R0) f, g> = sum_lm |f_lm|^2 (real, positive).  A wrong
+    normalisation 2 pi sqrt(2/(2l+1)) or sign (-1)^(m+n) in the coefficient product
+    (soft.f08:2367) breaks the equality."""
+    bw = 8
+    P = O.OraclePlan(bw)
+    rng = np.random.default_rng(61)
+    f = rng.standard_normal(bw * bw) + 1j * rng.standard_normal(bw * bw)
+    a = np.arange(2 * bw) * np.pi / bw
+    b = O.beta_samples(2 * bw)
+    for idx in ([3, 5, 11], [0, 0, 0], [15, 9, 2]):
+        g = O.rotate_ylm_cmplx(f, [a[idx[0]], b[idx[1]], a[idx[2]]])[0, 0]
+        cc = P.cross_correlation_ylm_cmplx(f, g)
+        v = cc[idx[2], idx[0], idx[1]]
+        assert abs(v - np.sum(np.abs(f) ** 2)) < 1e-11 * np.sum(np.abs(f) ** 2), (idx, v)
