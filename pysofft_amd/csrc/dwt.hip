@@ -502,8 +502,6 @@ hipError_t dwt_set_attributes(int, int)
     hipError_t e;
     if ((e = set_fwd_attr<0, 2, double2, double2>()) != hipSuccess) return e;
     if ((e = set_fwd_attr<0, 1, double2, double2>()) != hipSuccess) return e;
-    if ((e = set_fwd_attr<1, 4, float2, float2>()) != hipSuccess) return e;
-    if ((e = set_fwd_attr<1, 2, float2, float2>()) != hipSuccess) return e;
     done = true;
     return hipSuccess;
 }
@@ -557,20 +555,6 @@ hipError_t launch_dwt_forward(const DeviceTables &tb, int bw, int N, int npair, 
 {
     if (npair <= 0 || T <= 0) return hipSuccess;
     return fwd_typed<0, double2, double2>(tb, bw, N, npair, mode, F, f_ts, coeff, c_ts, T, s);
-}
-
-hipError_t launch_dwt_inverse_mixed(const DeviceTables &tb, int bw, int N, int npair, int mode, const float2 *coeff,
-                                    int64_t c_ts, const float2 *aux, float2 *F, int64_t f_ts, int T, hipStream_t s)
-{
-    if (npair <= 0 || T <= 0) return hipSuccess;
-    return inv_typed<1, float2, float2>(tb, bw, N, npair, mode, coeff, c_ts, aux, F, f_ts, T, s);
-}
-
-hipError_t launch_dwt_forward_mixed(const DeviceTables &tb, int bw, int N, int npair, int mode, const float2 *F,
-                                    int64_t f_ts, float2 *coeff, int64_t c_ts, int T, hipStream_t s)
-{
-    if (npair <= 0 || T <= 0) return hipSuccess;
-    return fwd_typed<1, float2, float2>(tb, bw, N, npair, mode, F, f_ts, coeff, c_ts, T, s);
 }
 
 }  // namespace so3

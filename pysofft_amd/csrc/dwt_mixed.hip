@@ -1,0 +1,412 @@
+// Mixed-precision discrete Wigner transforms: complex64 coefficients and fft arrays,
+// FP64 Kostelec recurrence (pysofft/wigner.f90:227-260), contraction on
+// v_mfma_f32_16x16x4f32 with d split into FP32 hi + lo parts (two MFMAs per product,
+// products of an exact FP32 operand with hi and lo), FP32 accumulation over 16 terms
+// and FP64 accumulation across those blocks.  Work unit: an order pair 0 <= p <= q with
+// its up to 8 symmetric variants (pysofft/soft.f08:534-631, 937-1023); the MFMA N
+// dimension is the 16 columns (8 variants x re/im) of one transform.
+//
+// MFMA operand layouts (v_mfma_f32_16x16x4f32, probed on gfx950): lane i + 16k holds
+// A[i][k] and B[k][i]; item r of the 4-float result of lane l is row 4(l>>4) + r, column
+// l & 15.
+//
+// inverse: one wave per (pair, 64-beta chunk, TB transforms).  Lane = beta runs the
+//   recurrence; every 4 degrees the wave writes d (hi, lo) for its 64 betas to a small
+//   LDS tile and reads it back as the A operands of four 16-beta tiles (M = beta,
+//   K = degree); B = the coefficients (K = degree, N = variant x re/im).
+// forward: NW waves per (pair, 64-degree chunk), wave w owning TB transforms.  For each
+//   64-beta tile the lanes (as betas, 64/NW per wave) run the recurrence and store w(beta) d (hi, lo) in LDS as
+//   [beta][degree]; A = that tile (M = degree, K = beta), B = the fft-array samples
+//   (K = beta, N = variant x re/im).
+#include "so3ft_internal.h"
+
+namespace so3 {
+
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int wrap_m(int m, int N) { return m >= 0 ? m : N + m; }
+
+__device__ inline void rec_consts_m(int l, int p, int q, double &a, double &x, double &b)
+{
+    const int64_t lp = l + 1;
+    const int64_t den = (lp * lp - (int64_t)p * p) * (lp * lp - (int64_t)q * q);
+    double cc = (double)lp * sqrt(1.0 / (double)den);
+    cc = cc * sqrt((double)(2 * l + 3));
+    const double cl = sqrt((double)(2 * l + 1));
+    a = cc * cl;
+    x = (l == 0) ? 0.0 : (double)((int64_t)p * q) / (double)((int64_t)l * (l + 1));
+    if (l == q) {
+        b = 0.0;
+    } else {
+        double c1 = sqrt((double)(((int64_t)l * l - (int64_t)p * p) * ((int64_t)l * l - (int64_t)q * q))) /
+                    (double)l;
+        c1 = c1 / sqrt((double)(2 * l - 1));
+        b = cc * c1;
+    }
+}
+
+__device__ __forceinline__ int64_t slice_m(const int64_t *slice, int m, int n, int bw)
+{
+    return slice[(int64_t)(m + bw - 1) * (2 * bw - 1) + (n + bw - 1)];
+}
+
+__device__ __forceinline__ f4_t mma32(float a, float b, f4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ void split(double d, float &hi, float &lo)
+{
+    hi = (float)d;
+    lo = (float)(d - (double)hi);
+}
+
+// ---------------------------------------------------------------------------
+// inverse
+// ---------------------------------------------------------------------------
+template <int TB>
+__global__ __launch_bounds__(64) void dwt_inverse_mix(const float2 *__restrict__ coeff, int64_t c_ts,
+                                                      float2 *__restrict__ F, int64_t f_ts, int T,
+                                                      const double *__restrict__ cosb, const double *__restrict__ seeds,
+                                                      const int64_t *__restrict__ slice,
+                                                      const int32_t *__restrict__ pairs, int bw, int N, int npair,
+                                                      int nch, int nunit)
+{
+    constexpr int DS = 80;                 // padded row of the d tile (float2 entries)
+    __shared__ double cst[3 * 258];
+    __shared__ float2 Dl[2][4][DS];        // [step parity][degree in step][beta] = (hi, lo)
+    const int id = blockIdx.x;
+    const int chunk = (id >> 3) % nch;
+    const int unit = ((id >> 3) / nch) * 8 + (id & 7);  // chunks of a unit share an XCD
+    if (unit >= nunit) return;
+    const int lane = threadIdx.x;
+    const int pq = pairs[unit % npair];
+    const int p = pq & 0xffff, q = pq >> 16;
+    const int L = bw - q;
+    const int b0 = chunk * 64;
+    const int beta = b0 + lane;
+    const int t0 = (unit / npair) * TB;
+    const int nt = min(TB, T - t0);
+
+    // B operand of this lane: column j = lane & 15 (variant j >> 1, part j & 1), degree
+    // offset k = lane >> 4 within each 4-degree step
+    const int j = lane & 15, kk = lane >> 4;
+    const int vj = j >> 1, partj = j & 1;
+    const Variant vr = variant(vj, p, q);
+    const int64_t offj = vr.valid ? slice_m(slice, vr.m, vr.n, bw) : slice_m(slice, p, q, bw);
+    const float sgn = vr.valid ? (float)vr.sign : 0.0f;
+    const bool lsg = vr.lsign;
+    const float *src[TB];
+#pragma unroll
+    for (int t = 0; t < TB; ++t)
+        src[t] = reinterpret_cast<const float *>(coeff + (int64_t)(t0 + min(t, nt - 1)) * c_ts + offj) + partj;
+
+    for (int i = lane; i < L - 1; i += 64) {
+        double a, x, b;
+        rec_consts_m(q + i, p, q, a, x, b);
+        cst[3 * i] = a; cst[3 * i + 1] = x; cst[3 * i + 2] = b;
+    }
+    const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
+    double dcur = (beta < N) ? seeds[tri * N + beta] : 0.0;
+    const double cb = (beta < N) ? cosb[beta] : 0.0;
+    double dprev = 0.0;
+
+    f4_t acc[TB][4];
+    double accd[TB][4][4];
+#pragma unroll
+    for (int t = 0; t < TB; ++t)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            acc[t][g] = f4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) accd[t][g][r] = 0.0;
+        }
+    __syncthreads();
+
+    // B operand of degree step l0: the signed coefficient of column j at degree l0 + kk;
+    // kept in a ring three steps deep so the loads are in flight while earlier steps run
+    auto loadb = [&](float (&dst)[TB], int l0) {
+        const int lk = l0 + kk;
+        const float s = (lsg && ((q + lk) & 1)) ? -sgn : sgn;
+        const int lc = min(lk, L - 1);   // clamped: past-the-end degrees enter with a zero sign
+        const float sz = (lk < L) ? s : 0.0f;
+#pragma unroll
+        for (int t = 0; t < TB; ++t) dst[t] = sz * src[t][2 * (int64_t)lc];
+    };
+    float bq0[TB], bq1[TB], bq2[TB];
+    loadb(bq0, 0);
+    loadb(bq1, 4);
+    loadb(bq2, 8);
+
+    // d for degrees l0..l0+3 of this lane's beta -> tile buf (then advanced past them)
+    auto dtile = [&](int buf, int l0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int li = l0 + k;
+            float hi, lo;
+            split(li < L ? dcur : 0.0, hi, lo);
+            Dl[buf][k][lane] = make_float2(hi, lo);
+            if (li < L - 1) {
+                const double ca = cst[3 * li], cx = cst[3 * li + 1], cbb = cst[3 * li + 2];
+                const double dn = (ca * (cb - cx)) * dcur - cbb * dprev;
+                dprev = dcur;
+                dcur = dn;
+            }
+        }
+    };
+    dtile(0, 0);
+    for (int l0 = 0, buf = 0; l0 < L; l0 += 4, buf ^= 1) {
+        // LDS operations of one wave execute in order; the empty asm keeps the compiler
+        // from moving tile reads above the writes that produced them
+        asm volatile("" ::: "memory");
+        float2 a[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) a[g] = Dl[buf][kk][16 * g + j];
+        float bv[TB];
+#pragma unroll
+        for (int t = 0; t < TB; ++t) { bv[t] = bq0[t]; bq0[t] = bq1[t]; bq1[t] = bq2[t]; }
+        loadb(bq2, l0 + 12);
+        // all hi products, then all lo products: no back-to-back dependent MFMAs
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int t = 0; t < TB; ++t) acc[t][g] = mma32(a[g].x, bv[t], acc[t][g]);
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int t = 0; t < TB; ++t) acc[t][g] = mma32(a[g].y, bv[t], acc[t][g]);
+        // the next step's recurrence runs on the VALU while the MFMAs execute
+        asm volatile("" ::: "memory");
+        if (l0 + 4 < L) dtile(buf ^ 1, l0 + 4);
+        if (((l0 >> 2) & 3) == 3 || l0 + 4 >= L) {  // flush every 16 degrees
+#pragma unroll
+            for (int t = 0; t < TB; ++t)
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) accd[t][g][r] += (double)acc[t][g][r];
+                    acc[t][g] = f4_t{0.f, 0.f, 0.f, 0.f};
+                }
+        }
+    }
+    // item r of lane l: beta row 4(l>>4) + r of tile g, column j = l & 15
+    if (!vr.valid) return;
+    const int64_t colbase = ((int64_t)wrap_m(vr.n, N) * N + wrap_m(vr.m, N)) * N;
+#pragma unroll
+    for (int t = 0; t < TB; ++t) {
+        if (t >= nt) continue;
+        float *Ft = reinterpret_cast<float *>(F + (int64_t)(t0 + t) * f_ts + colbase) + partj;
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int bt = b0 + 16 * g + 4 * kk + r;
+                if (bt < N) Ft[2 * (int64_t)(vr.rev ? N - 1 - bt : bt)] = (float)accd[t][g][r];
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// forward
+// ---------------------------------------------------------------------------
+template <int TB, int NW>
+__global__ __launch_bounds__(64 * NW) void dwt_forward_mix(const float2 *__restrict__ F, int64_t f_ts,
+                                                           float2 *__restrict__ coeff, int64_t c_ts, int T,
+                                                           const double *__restrict__ w,
+                                                           const double *__restrict__ cosb,
+                                                           const double *__restrict__ seeds,
+                                                           const int64_t *__restrict__ slice,
+                                                           const int32_t *__restrict__ pairs, int bw, int N, int npair,
+                                                           int nch, int nunit)
+{
+    constexpr int DS = 66;                 // padded row of the tile (float2 entries)
+    constexpr int BPW = 64 / NW;           // betas of a tile whose recurrence one wave runs
+    __shared__ double cst[3 * 258];
+    __shared__ float2 Dt[64][DS];          // [beta in tile][degree in chunk] = w d (hi, lo)
+    const int id = blockIdx.x;
+    const int chunk = (id >> 3) % nch;
+    const int unit = ((id >> 3) / nch) * 8 + (id & 7);
+    if (unit >= nunit) return;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index, uniform
+    const int pq = pairs[unit % npair];
+    const int p = pq & 0xffff, q = pq >> 16;
+    const int L = bw - q;
+    const int lc0 = chunk * 64;
+    if (lc0 >= L) return;
+    const int lend = min(L, lc0 + 64);
+    const int t0 = ((unit / npair) * NW + wv) * TB;
+    const int nt = min(TB, T - t0);        // <= 0 for spare waves of the last group
+
+    const int j = lane & 15, kk = lane >> 4;
+    const int vj = j >> 1, partj = j & 1;
+    const Variant vr = variant(vj, p, q);
+    const int64_t colj = ((int64_t)wrap_m(vr.n, N) * N + wrap_m(vr.m, N)) * N;
+    const float *src[TB];
+#pragma unroll
+    for (int t = 0; t < TB; ++t)
+        src[t] = reinterpret_cast<const float *>(F + (int64_t)max(0, min(t0 + t, T - 1)) * f_ts + colj) + partj;
+
+    for (int i = threadIdx.x; i < lend - 1; i += 64 * NW) {
+        double a, x, b;
+        rec_consts_m(q + i, p, q, a, x, b);
+        cst[3 * i] = a; cst[3 * i + 1] = x; cst[3 * i + 2] = b;
+    }
+    const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
+
+    f4_t acc[TB][4];
+    double accd[TB][4][4];
+#pragma unroll
+    for (int t = 0; t < TB; ++t)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            acc[t][g] = f4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) accd[t][g][r] = 0.0;
+        }
+    const int nlt = (lend - lc0 + 15) >> 4;   // 16-degree tiles with work
+    // B operand of beta step bs (betas 4 bs .. 4 bs + 3): the fft-array sample of column j
+    // at beta 4 bs + kk (reversed for v >= 4); ring three steps deep
+    const int nbs = (N + 3) >> 2;
+    auto loadb = [&](float (&dst)[TB], int bs) {
+        const int bt = min(4 * bs + kk, N - 1);   // clamped rows are multiplied by zero d rows
+#pragma unroll
+        for (int t = 0; t < TB; ++t) dst[t] = src[t][2 * (int64_t)(vr.rev ? N - 1 - bt : bt)];
+    };
+    float bq0[TB], bq1[TB], bq2[TB];
+    if (nt > 0) {
+        loadb(bq0, 0);
+        loadb(bq1, 1);
+        loadb(bq2, 2);
+    }
+
+    for (int b0 = 0; b0 < N; b0 += 64) {
+        __syncthreads();  // cst ready / previous tile consumed
+        if (lane < BPW) {
+            const int bl = wv * BPW + lane;
+            const int beta = b0 + bl;
+            const bool ok = beta < N;
+            double dc = ok ? seeds[tri * N + beta] : 0.0, dp = 0.0;
+            const double cb = ok ? cosb[beta] : 0.0;
+            const double wb = ok ? w[beta] : 0.0;
+            // unrolled so the constants' LDS reads run ahead of the chain through d
+            int li = 0;
+#pragma unroll 8
+            for (; li < lc0; ++li) {
+                const double ca = cst[3 * li], cx = cst[3 * li + 1], cbb = cst[3 * li + 2];
+                const double dn = (ca * (cb - cx)) * dc - cbb * dp;
+                dp = dc;
+                dc = dn;
+            }
+#pragma unroll 8
+            for (; li < lc0 + 64; ++li) {
+                float hi, lo;
+                split(li < lend ? wb * dc : 0.0, hi, lo);
+                Dt[bl][li - lc0] = make_float2(hi, lo);
+                if (li < lend - 1) {
+                    const double ca = cst[3 * li], cx = cst[3 * li + 1], cbb = cst[3 * li + 2];
+                    const double dn = (ca * (cb - cx)) * dc - cbb * dp;
+                    dp = dc;
+                    dc = dn;
+                }
+            }
+        }
+        __syncthreads();
+        if (nt <= 0) continue;
+        const int nb = min(64, N - b0);
+        for (int bb = 0; bb < nb; bb += 4) {
+            const int bs = (b0 + bb) >> 2;
+            float bv[TB];
+#pragma unroll
+            for (int t = 0; t < TB; ++t) { bv[t] = bq0[t]; bq0[t] = bq1[t]; bq1[t] = bq2[t]; }
+            loadb(bq2, min(bs + 3, nbs - 1));
+            float2 a[4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) a[g] = Dt[bb + kk][16 * g + j];
+            // all hi products, then all lo products: no back-to-back dependent MFMAs
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                if (g >= nlt) break;
+#pragma unroll
+                for (int t = 0; t < TB; ++t) acc[t][g] = mma32(a[g].x, bv[t], acc[t][g]);
+            }
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                if (g >= nlt) break;
+#pragma unroll
+                for (int t = 0; t < TB; ++t) acc[t][g] = mma32(a[g].y, bv[t], acc[t][g]);
+            }
+            if (((bb >> 2) & 3) == 3 || bb + 4 >= nb) {  // flush every 16 betas
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) accd[t][g][r] += (double)acc[t][g][r];
+                        acc[t][g] = f4_t{0.f, 0.f, 0.f, 0.f};
+                    }
+            }
+        }
+    }
+    // item r of lane l: degree row 4(l>>4) + r of tile g, column j = l & 15
+    if (!vr.valid || nt <= 0) return;
+    const int64_t o = slice_m(slice, vr.m, vr.n, bw);
+#pragma unroll
+    for (int t = 0; t < TB; ++t) {
+        if (t >= nt) continue;
+        float *ct = reinterpret_cast<float *>(coeff + (int64_t)(t0 + t) * c_ts + o) + partj;
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int li = lc0 + 16 * g + 4 * kk + r;
+                if (li < L) {
+                    const int l = q + li;
+                    const double s = vr.sign * ((vr.lsign && (l & 1)) ? -1.0 : 1.0);
+                    ct[2 * (int64_t)li] = (float)(s * accd[t][g][r]);
+                }
+            }
+    }
+}
+
+static int grid_m(int nunit, int nch) { return (nunit + 7) / 8 * 8 * nch; }
+
+hipError_t launch_dwt_inverse_mixed(const DeviceTables &tb, int bw, int N, int npair, int mode, const float2 *coeff,
+                                    int64_t c_ts, const float2 *aux, float2 *F, int64_t f_ts, int T, hipStream_t s)
+{
+    if (mode != DWT_COMPLEX || aux) return hipErrorInvalidValue;
+    if (npair <= 0 || T <= 0) return hipSuccess;
+    const int nch = (N + 63) / 64;
+#define SO3_MINV(TBV)                                                                                          \
+    do {                                                                                                       \
+        const int nunit = npair * ((T + TBV - 1) / TBV);                                                       \
+        hipLaunchKernelGGL((dwt_inverse_mix<TBV>), dim3(grid_m(nunit, nch)), dim3(64), 0, s, coeff, c_ts, F, f_ts, \
+                           T, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, npair, nch, nunit);                  \
+    } while (0)
+    if (T >= 2) SO3_MINV(2);  // TB = 4 needs 288 registers (one wave per SIMD)
+    else SO3_MINV(1);
+#undef SO3_MINV
+    return hipGetLastError();
+}
+
+hipError_t launch_dwt_forward_mixed(const DeviceTables &tb, int bw, int N, int npair, int mode, const float2 *F,
+                                    int64_t f_ts, float2 *coeff, int64_t c_ts, int T, hipStream_t s)
+{
+    if (mode != DWT_COMPLEX) return hipErrorInvalidValue;
+    if (npair <= 0 || T <= 0) return hipSuccess;
+    const int nch = (bw + 63) / 64;
+#define SO3_MFWD(TBV, NWV)                                                                                     \
+    do {                                                                                                       \
+        const int nunit = npair * ((T + TBV * NWV - 1) / (TBV * NWV));                                         \
+        hipLaunchKernelGGL((dwt_forward_mix<TBV, NWV>), dim3(grid_m(nunit, nch)), dim3(64 * NWV), 0, s, F, f_ts, \
+                           coeff, c_ts, T, tb.w, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, npair, nch, nunit); \
+    } while (0)
+    if (T >= 8) SO3_MFWD(2, 4);
+    else if (T >= 4) SO3_MFWD(1, 4);
+    else if (T >= 2) SO3_MFWD(1, 2);
+    else SO3_MFWD(1, 1);
+#undef SO3_MFWD
+    return hipGetLastError();
+}
+
+}  // namespace so3

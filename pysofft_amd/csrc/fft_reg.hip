@@ -90,70 +90,99 @@ template <> __device__ __forceinline__ void st<float2>(float2 *p, int64_t off, d
     p[off] = make_float2((float)v.x, (float)v.y);
 }
 
+template <class T> __device__ __forceinline__ T zero_of();
+template <> __device__ __forceinline__ double2 zero_of<double2>() { return make_double2(0.0, 0.0); }
+template <> __device__ __forceinline__ float2 zero_of<float2>() { return make_float2(0.f, 0.f); }
+__device__ __forceinline__ double2 widen(double2 v) { return v; }
+__device__ __forceinline__ double2 widen(float2 v) { return make_double2((double)v.x, (double)v.y); }
+
 // TI: input element type (double2/float2; in_real reads the .x-sized scalar instead)
 // TO: output element type
+//
+// Complex64 input (TI = float2): a block walks the column tiles
+// cb = col0 + (blockIdx.x + j * gridDim.x) * C and loads the rows of tile j+1, in their
+// 8-byte storage type, before transforming tile j.  The pass is latency-bound: the LDS
+// exchange caps a CU at two blocks, so without the prefetch a complex64 pass has half the
+// bytes in flight of a complex128 one.  Complex128 input keeps one tile per block (a
+// second FP64 row set would not fit the register budget).
 template <int R1, int R2, class TI, class TO>
 __global__ __launch_bounds__(256) void fft_reg_kernel(const void *__restrict__ in, void *__restrict__ out,
                                                       const double2 *__restrict__ twg, FftRegK k)
 {
     constexpr int N = R1 * R2;
+    constexpr int RM = R1 > R2 ? R1 : R2;
+    constexpr bool PF = sizeof(TI) == 8;
     extern __shared__ double2 sm[];
     double2 *tw = sm;           // N
     double2 *Y = sm + N;        // [k1][n2][c]  N * C
     const int C = k.C, tid = threadIdx.x;
     const int c = tid % C, grp = tid / C;
     const int64_t o = blockIdx.y, t = blockIdx.z;
-    const int cb = k.col0 + blockIdx.x * C;
     if (k.mask_outer && !order_ok2((int)o, N, k.bw, k.lmax)) return;
+    const int ntile = (k.ncols + C - 1) / C;
 
     for (int i = tid; i < N; i += blockDim.x) {
         double2 w = twg[i];
         if (k.sign > 0) w.y = -w.y;
         tw[i] = w;
     }
-    const bool colok = (cb + c) < (k.col0 + k.ncols);
-    const int64_t ib = t * k.in_t + o * k.in_o + cb + c;
-    double2 v[R1 > R2 ? R1 : R2];
-    if (grp < R2) {
-        const int n2 = grp;
+    // step-1 rows of tile jt for thread (n2 = grp, c): rows R2*n1 + n2, masked rows zero
+    auto load = [&](TI (&dst)[R1], int jt) {
+        const int cb = k.col0 + jt * C;
+        const bool colok = jt < ntile && (cb + c) < (k.col0 + k.ncols);
+        const int64_t ib = t * k.in_t + o * k.in_o + cb + c;
 #pragma unroll
         for (int n1 = 0; n1 < R1; ++n1) {
-            const int r = R2 * n1 + n2;
-            double2 x = make_double2(0.0, 0.0);
-            if (colok && (!k.mask_rows || order_ok2(r, N, k.bw, k.lmax))) {
+            const int r = R2 * n1 + grp;
+            TI x = zero_of<TI>();
+            if (grp < R2 && colok && (!k.mask_rows || order_ok2(r, N, k.bw, k.lmax))) {
                 const int64_t off = ib + (int64_t)r * k.in_r;
-                if (k.in_real) x.x = (double)reinterpret_cast<const typename Scalar<TI>::type *>(in)[off];
-                else x = ld<TI>(reinterpret_cast<const TI *>(in), off);
+                if (k.in_real) x.x = reinterpret_cast<const typename Scalar<TI>::type *>(in)[off];
+                else x = reinterpret_cast<const TI *>(in)[off];
             }
-            v[n1] = x;
+            dst[n1] = x;
         }
-    }
-    __syncthreads();  // twiddles visible
-    if (grp < R2) {
-        const int n2 = grp;
-        Dft<R1>::run(v, tw, N / R1, k.sign);
+    };
+    const int G = PF ? gridDim.x : ntile;
+    int jt = blockIdx.x;
+    TI raw[R1];
+    load(raw, jt);
+    for (; jt < ntile; jt += G) {
+        double2 v[RM];
 #pragma unroll
-        for (int k1 = 0; k1 < R1; ++k1) {
-            const double2 y = (k1 == 0 || n2 == 0) ? v[k1] : cmul2(v[k1], tw[n2 * k1]);
-            Y[(k1 * R2 + n2) * C + c] = y;
-        }
-    }
-    __syncthreads();
-    if (grp < R1) {
-        const int k1 = grp;
+        for (int n1 = 0; n1 < R1; ++n1) v[n1] = widen(raw[n1]);
+        if (PF) load(raw, jt + G);
+        const int cb = k.col0 + jt * C;
+        const bool colok = (cb + c) < (k.col0 + k.ncols);
+        __syncthreads();  // twiddles visible / previous tile's exchange consumed
+        if (grp < R2) {
+            const int n2 = grp;
+            Dft<R1>::run(v, tw, N / R1, k.sign);
 #pragma unroll
-        for (int n2 = 0; n2 < R2; ++n2) v[n2] = Y[(k1 * R2 + n2) * C + c];
-        Dft<R2>::run(v, tw, N / R2, k.sign);
-        if (colok) {
-            const int64_t ob = t * k.out_t + o * k.out_o + cb + c;
-#pragma unroll
-            for (int k2 = 0; k2 < R2; ++k2) {
-                const int64_t off = ob + (int64_t)(k1 + R1 * k2) * k.out_r;
-                const double2 x = make_double2(v[k2].x * k.scale, v[k2].y * k.scale);
-                if (k.out_real) reinterpret_cast<typename Scalar<TO>::type *>(out)[off] = (typename Scalar<TO>::type)x.x;
-                else st<TO>(reinterpret_cast<TO *>(out), off, x);
+            for (int k1 = 0; k1 < R1; ++k1) {
+                const double2 y = (k1 == 0 || n2 == 0) ? v[k1] : cmul2(v[k1], tw[n2 * k1]);
+                Y[(k1 * R2 + n2) * C + c] = y;
             }
         }
+        __syncthreads();
+        if (grp < R1) {
+            const int k1 = grp;
+#pragma unroll
+            for (int n2 = 0; n2 < R2; ++n2) v[n2] = Y[(k1 * R2 + n2) * C + c];
+            Dft<R2>::run(v, tw, N / R2, k.sign);
+            if (colok) {
+                const int64_t ob = t * k.out_t + o * k.out_o + cb + c;
+#pragma unroll
+                for (int k2 = 0; k2 < R2; ++k2) {
+                    const int64_t off = ob + (int64_t)(k1 + R1 * k2) * k.out_r;
+                    const double2 x = make_double2(v[k2].x * k.scale, v[k2].y * k.scale);
+                    if (k.out_real)
+                        reinterpret_cast<typename Scalar<TO>::type *>(out)[off] = (typename Scalar<TO>::type)x.x;
+                    else st<TO>(reinterpret_cast<TO *>(out), off, x);
+                }
+            }
+        }
+        if (!PF) break;
     }
 }
 
@@ -203,7 +232,11 @@ static hipError_t launch_reg_typed(int N, const double2 *tw, int bw, int lmax, c
     k.col0 = col0; k.ncols = ncols;
     const int threads = (R1 > R2 ? R1 : R2) * k.C;
     const int bytes = (int)sizeof(double2) * (N + N * k.C);
-    dim3 grid((ncols + k.C - 1) / k.C, N, T), block(threads);
+    // complex64 input: each block walks about four column tiles (prefetching the next);
+    // complex128 input: one tile per block
+    const int ntile = (ncols + k.C - 1) / k.C;
+    const int gx = (sizeof(TI) == 8) ? (ntile + 3) / 4 : ntile;
+    dim3 grid(gx, N, T), block(threads);
 #define SO3_REG(A, B)                                                                                   \
     do {                                                                                                \
         static bool attr = false;                                                                       \
