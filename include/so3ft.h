@@ -137,6 +137,41 @@ int so3ft_cross_correlation_argmax_many(so3ft_handle h, const void *f_lms, const
 int so3ft_rotate_ylm_cmplx(int64_t bw, const void *ylms, int64_t nset, const void *euler, int64_t nrot,
                            int paired, void *out, void *stream);
 
+/* ---- learned alignment proposal (no counterpart in the reference) ---------------
+ * A small MLP (BF16 operands on MFMA, FP32 accumulation, FP32 master weights and Adam)
+ * maps pair features to a 6-D rotation representation (Gram-Schmidt to a rotation).
+ * dims: nlayers+1 widths, dims[nlayers] == 6; hidden layers use GELU.  Parameters are
+ * host float32, layer by layer [W (out x in, row-major), b (out)].  x, R9, T9 are DEVICE
+ * float32 arrays (n, dims[0]) / (n, 9) (row-major rotation matrices).  The network
+ * must be given parameters (so3ft_mlp_set_params) before its first forward pass.
+ * Errors of these calls are reported by so3ft_mlp_last_error(). */
+const char *so3ft_mlp_last_error(void);
+int so3ft_mlp_create(const int32_t *dims, int nlayers, int64_t *handle);
+int so3ft_mlp_destroy(int64_t handle);
+int so3ft_mlp_num_params(int64_t handle, int64_t *n);
+int so3ft_mlp_set_params(int64_t handle, const float *params);
+int so3ft_mlp_get_params(int64_t handle, int which, float *params);  /* 0 weights, 1 gradients */
+int so3ft_mlp_forward(int64_t handle, const float *x, int64_t n, float *R9, float *out6, void *stream);
+/* forward, loss mean ||R - T||_F^2 (HOST float *loss, may be NULL), backward; one Adam step
+ * when lr > 0 (lr == 0 leaves the weights and only computes the gradients) */
+int so3ft_mlp_train_step(int64_t handle, const float *x, const float *T9, int64_t n, float lr, float *loss,
+                         void *stream);
+/* features of n pairs of LMc coefficient sets (complex128, (n, bw^2), 9 <= bw <= 64) into
+ * x (DEVICE float32 (n, 452)): the l <= 8 coefficients of f and g (re, im) and the log
+ * per-degree powers of f and g for l < 64, scaled by the l <= 8 RMS of f; standardised
+ * with mean/std (DEVICE float32 (452)) when given. */
+int so3ft_align_features(int64_t bw, const void *f_lms, const void *g_lms, int64_t n, const float *mean,
+                         const float *stdv, float *x, void *stream);
+/* bf16 GEMM test hook: C (M x N float32) = A (M x K) B (K x N), row-major bf16, DEVICE */
+int so3ft_gemm_bf16(const void *A, const void *B, float *C, int M, int N, int K, void *stream);
+/* local correlation refinement: for each pair, the correlation |C| (degrees < lmax <= 64) on
+ * the 5 x 5 x 5 window of the grid of bandwidth grid_bw around euler_in (DEVICE float64
+ * (n, 3)); euler_out (DEVICE (n, 3)) gets the window's argmax, score (DEVICE (n,), may be
+ * NULL) its |C|, on the scale of the full correlation grid. */
+int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_lms, int64_t n,
+                                   const double *euler_in, double *euler_out, double *score, int64_t grid_bw,
+                                   int64_t lmax, void *stream);
+
 /* ---- measurement hook (no counterpart in the reference): launches one kernel of
  * the pipeline `reps` times on `stream` between two HIP events and returns the mean
  * duration of one launch.  which: 0 inverse DWT (a = coeffs, b = T x (2bw)^3 scratch),

@@ -53,6 +53,17 @@ SIGNATURES = {
     "so3ft_cross_correlation_ylm_cmplx_3d": (_int, [_i64, _vp, _vp, _vp, _P(_dbl), _i64, _i64, _i64, _vp]),
     "so3ft_enforce_real_sym_mnl": (_int, [_i64, _vp, _i64, _vp]),
     "so3ft_rotate_ylm_cmplx": (_int, [_i64, _vp, _i64, _vp, _i64, _int, _vp, _vp]),
+    "so3ft_mlp_last_error": (C.c_char_p, []),
+    "so3ft_mlp_create": (_int, [_P(_i32), _int, _P(_i64)]),
+    "so3ft_mlp_destroy": (_int, [_i64]),
+    "so3ft_mlp_num_params": (_int, [_i64, _P(_i64)]),
+    "so3ft_mlp_set_params": (_int, [_i64, _vp]),
+    "so3ft_mlp_get_params": (_int, [_i64, _int, _vp]),
+    "so3ft_mlp_forward": (_int, [_i64, _vp, _i64, _vp, _vp, _vp]),
+    "so3ft_mlp_train_step": (_int, [_i64, _vp, _vp, _i64, C.c_float, _P(C.c_float), _vp]),
+    "so3ft_align_features": (_int, [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "so3ft_gemm_bf16": (_int, [_vp, _vp, _vp, _int, _int, _int, _vp]),
+    "so3ft_refine_correlation_peaks": (_int, [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _i64, _vp]),
     "so3ft_bench_kernel": (_int, [_i64, _int, _int, _vp, _vp, _i64, _vp, _P(C.c_float)]),
 }
 

@@ -221,3 +221,26 @@ def test_correlation_scale_at_true_rotation():
         cc = P.cross_correlation_ylm_cmplx(f, g)
         v = cc[idx[2], idx[0], idx[1]]
         assert abs(v - np.sum(np.abs(f) ** 2)) < 1e-11 * np.sum(np.abs(f) ** 2), (idx, v)
+
+
+def test_mlp_oracle_gradients_vs_finite_differences():
+    """The alignment MLP's FP64 oracle gradients (oracle/mlp_oracle.py) against central
+    differences of its own loss on a tiny network: a dropped chain-rule term or a wrong
+    sign in the Gram-Schmidt backward would fail."""
+    from oracle import mlp_oracle as M
+    rng = np.random.default_rng(9)
+    dims = (7, 5, 4, 6)
+    n = 6
+    sizes = sum(dims[i] * dims[i + 1] + dims[i + 1] for i in range(len(dims) - 1))
+    p = rng.standard_normal(sizes) * 0.5
+    X = rng.standard_normal((n, dims[0]))
+    Q, _ = np.linalg.qr(rng.standard_normal((n, 3, 3)))
+    T = Q.reshape(n, 9)
+    g = M.gradients(p, dims, X, T)
+    eps = 1e-6
+    fd = np.empty_like(p)
+    for i in range(p.size):
+        e = np.zeros_like(p)
+        e[i] = eps
+        fd[i] = (M.loss(p + e, dims, X, T) - M.loss(p - e, dims, X, T)) / (2 * eps)
+    assert np.max(np.abs(g - fd)) < 1e-7 * max(1.0, np.max(np.abs(fd)))
