@@ -7,7 +7,7 @@ with g ~ D(R) f:
   2. proposal: a small MLP (BF16 MFMA, FP32 accumulation; ``so3ft_mlp_*``) maps the
      features to a 6-D rotation representation -> rotation R0 -> ZYZ Euler angles;
   3. refinement: the FP64 correlation |C| on 5 x 5 x 5 windows of successively finer
-     grids (bandwidth 8, 16, 32, 64, 128 by default, +-2 steps each, degrees below the
+     grids (bandwidth 4, 8, 16, 32, 128 by default, +-2 steps each, degrees below the
      grid bandwidth up to 64) around the estimate
      (``so3ft_refine_correlation_peaks``);
   4. fallback: pairs whose refined peak, normalised by ||f|| ||g||, stays below a
@@ -28,7 +28,7 @@ from . import workloads as W
 
 N_FEAT = 452
 DIMS = (N_FEAT, 512, 512, 512, 6)
-LEVELS = ((8, 8), (16, 16), (32, 32), (64, 64), (128, 64))   # (grid bandwidth, degree limit)
+LEVELS = ((4, 4), (8, 8), (16, 16), (32, 32), (128, 64))   # (grid bandwidth, degree limit)
 
 
 def _torch():

@@ -10,7 +10,7 @@
 // d~^q_{p,q}(b) = sqrt((2q+1)/2) sqrt((2q)!/((q+p)!(q-p)!)) cos(b/2)^(q+p) sin(b/2)^(q-p)
 // (the seeds of wigner.f90:130-187 for 0 < b < pi), and the 8-variant symmetry table.
 //
-// One block per pair.  Work items (p, q, variant pair) run the recurrence for the 5 betas
+// One block (768 threads) per pair.  Work items (p, q, variant pair) run the recurrence for the 5 betas
 // (or pi - beta for the reversed variants), accumulate F(beta, m, n) over l in registers
 // and fold e^{-i n gamma_c} into LDS sums G[beta][m][c]; the alpha sums and the argmax
 // follow.  No counterpart in the reference (its callers evaluate the whole grid).
@@ -36,7 +36,8 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b)  // a * conj(b)
     return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 
-__global__ __launch_bounds__(256) void refine_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g,
+// dynamic LDS: G [W beta][NM m][W gamma] | eg [W gamma][NM n] | ea [W alpha][NM m] (double2)
+__global__ __launch_bounds__(768) void refine_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g,
                                                      int64_t nb, int n, const double *__restrict__ ein,
                                                      double *__restrict__ eout, double *__restrict__ score, int Bg,
                                                      int Lc, const double *__restrict__ rc,
@@ -48,8 +49,14 @@ __global__ __launch_bounds__(256) void refine_kernel(const double2 *__restrict__
     const int NM = 2 * Lc - 1;
     extern __shared__ double sm[];
     double2 *G = reinterpret_cast<double2 *>(sm);    // [W beta][NM m][W gamma]
+    double2 *eg = G + W * NM * W;                     // e^{-i n gamma_c}, [c][n + Lc - 1]
+    double2 *ea = eg + W * NM;                        // e^{-i m alpha_a}, [a][m + Lc - 1]
     __shared__ double Cabs[W * W * W];
-    __shared__ double bet[W], brev[W];
+    __shared__ double wl[RL_MAX];                     // 2 pi sqrt(2/(2l+1))
+    __shared__ double lfac[2 * RL_MAX + 1];           // log k!
+    __shared__ double lq[RL_MAX];                     // log sqrt((2q+1)/2)
+    __shared__ double lcb[2 * W], lsb[2 * W], cbv[2 * W];  // log cos(b/2), log sin(b/2), cos b; b, pi - b
+    __shared__ double bet[W];
     __shared__ int win[3][W];                         // grid indices of the window
     const int tid = threadIdx.x;
     const double pi = 3.14159265358979323846;
@@ -66,10 +73,27 @@ __global__ __launch_bounds__(256) void refine_kernel(const double2 *__restrict__
             win[1][i] = kb + i - 2;
             win[2][i] = ((jg + i - 2) % N2 + N2) % N2;
             bet[i] = (2.0 * win[1][i] + 1.0) * pi / (2.0 * N2);
-            brev[i] = pi - bet[i];
         }
     }
+    __syncthreads();
     for (int i = tid; i < W * NM * W; i += blockDim.x) G[i] = make_double2(0.0, 0.0);
+    for (int l = tid; l < Lc; l += blockDim.x) {
+        wl[l] = 2.0 * pi * sqrt(2.0 / (2.0 * l + 1.0));
+        lq[l] = 0.5 * log((2.0 * l + 1.0) * 0.5);
+    }
+    for (int k = tid; k <= 2 * Lc; k += blockDim.x) lfac[k] = lgamma((double)k + 1.0);
+    if (tid < 2 * W) {
+        const double be = tid < W ? bet[tid] : pi - bet[tid - W];
+        lcb[tid] = log(cos(0.5 * be));
+        lsb[tid] = log(sin(0.5 * be));
+        cbv[tid] = cos(be);
+    }
+    for (int i = tid; i < W * NM; i += blockDim.x) {
+        const int c = i / NM, m = i % NM - (Lc - 1);
+        const double tg = (double)m * win[2][c] * h, ta = (double)m * win[0][c] * h;
+        eg[i] = make_double2(cos(tg), -sin(tg));
+        ea[i] = make_double2(cos(ta), -sin(ta));
+    }
     __syncthreads();
 
     const double2 *fs = f + (int64_t)s * nb, *gs = g + (int64_t)s * nb;
@@ -78,54 +102,59 @@ __global__ __launch_bounds__(256) void refine_kernel(const double2 *__restrict__
     for (int it = tid; it < 4 * npq; it += blockDim.x) {
         const int vg = it & 3, P = it >> 2;
         int q = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
-        while ((int64_t)q * (q + 1) / 2 > P) --q;
-        while ((int64_t)(q + 1) * (q + 2) / 2 <= P) ++q;
+        while (q * (q + 1) / 2 > P) --q;
+        while ((q + 1) * (q + 2) / 2 <= P) ++q;
         const int p = P - q * (q + 1) / 2;
         const Variant v0 = variant(2 * vg, p, q), v1 = variant(2 * vg + 1, p, q);
         if (!v0.valid && !v1.valid) continue;
-        const bool rev = vg >= 2;
-        // seeds at the 5 betas (closed form), log-domain
+        const int ro = vg >= 2 ? W : 0;   // reversed variants use pi - beta
+        // seeds d~^q_{p,q} at the 5 betas (closed form, log domain)
+        const double lc = 0.5 * (lfac[2 * q] - lfac[q + p] - lfac[q - p]) + lq[q];
         double d[W], dp[W];
-        const double lc = 0.5 * (lgamma(2.0 * q + 1.0) - lgamma((double)(q + p) + 1.0) - lgamma((double)(q - p) + 1.0)) +
-                          0.5 * log((2.0 * q + 1.0) * 0.5);
+#pragma unroll
         for (int b = 0; b < W; ++b) {
-            const double be = rev ? brev[b] : bet[b];
-            double e = lc;
-            if (q + p > 0) e += (double)(q + p) * log(cos(0.5 * be));
-            if (q - p > 0) e += (double)(q - p) * log(sin(0.5 * be));
-            d[b] = exp(e);
+            d[b] = exp(lc + (double)(q + p) * lcb[ro + b] + (double)(q - p) * lsb[ro + b]);
             dp[b] = 0.0;
         }
-        double cb[W];
-        for (int b = 0; b < W; ++b) cb[b] = cos(rev ? brev[b] : bet[b]);
         double2 F0[W], F1[W];
+#pragma unroll
         for (int b = 0; b < W; ++b) { F0[b] = make_double2(0.0, 0.0); F1[b] = make_double2(0.0, 0.0); }
         const double *cst = rc + rcoff[rl_pair(p, q)];
+        const double s0 = (((v0.m + v0.n) & 1) ? -1.0 : 1.0) * v0.sign, s1 = (((v1.m + v1.n) & 1) ? -1.0 : 1.0) * v1.sign;
+        // operands of degree l (coefficients of both variants, recurrence constants of the
+        // step l -> l+1), loaded one degree ahead of their use
+        const int m0 = v0.valid ? v0.m : 0, n0 = v0.valid ? v0.n : 0;
+        const int m1 = v1.valid ? v1.m : 0, n1 = v1.valid ? v1.n : 0;
+        struct Ops {
+            double2 f0, g0, f1, g1;
+            double ca, cx, cbb;
+        };
+        auto load = [&](Ops &o, int l) {
+            const int r = l * (l + 1);
+            o.f0 = fs[r + m0]; o.g0 = gs[r + n0];
+            o.f1 = fs[r + m1]; o.g1 = gs[r + n1];
+            const int k = max(0, min(l, Lc - 2) - q);   // the last degree has no step: reread entry 0
+            o.ca = cst[3 * k]; o.cx = cst[3 * k + 1]; o.cbb = cst[3 * k + 2];
+        };
+        Ops nx;
+        load(nx, q);
         for (int l = q; l < Lc; ++l) {
-            const double wn = 2.0 * pi * sqrt(2.0 / (2.0 * l + 1.0));
-            double2 c0 = make_double2(0.0, 0.0), c1 = make_double2(0.0, 0.0);
-            if (v0.valid) {
-                const double sg = wn * ((((v0.m + v0.n) & 1) != 0) ? -1.0 : 1.0) * v0.sign *
-                                  ((v0.lsign && (l & 1)) ? -1.0 : 1.0);
-                const double2 z = cmulc(fs[l * (l + 1) + v0.m], gs[l * (l + 1) + v0.n]);
-                c0 = make_double2(sg * z.x, sg * z.y);
-            }
-            if (v1.valid) {
-                const double sg = wn * ((((v1.m + v1.n) & 1) != 0) ? -1.0 : 1.0) * v1.sign *
-                                  ((v1.lsign && (l & 1)) ? -1.0 : 1.0);
-                const double2 z = cmulc(fs[l * (l + 1) + v1.m], gs[l * (l + 1) + v1.n]);
-                c1 = make_double2(sg * z.x, sg * z.y);
-            }
+            const Ops cu = nx;
+            if (l + 1 < Lc) load(nx, l + 1);
+            const double wn = wl[l];
+            const double sl0 = v0.valid ? wn * s0 * ((v0.lsign && (l & 1)) ? -1.0 : 1.0) : 0.0;
+            const double sl1 = v1.valid ? wn * s1 * ((v1.lsign && (l & 1)) ? -1.0 : 1.0) : 0.0;
+            const double2 z0 = cmulc(cu.f0, cu.g0), z1 = cmulc(cu.f1, cu.g1);
+            const double2 c0 = make_double2(sl0 * z0.x, sl0 * z0.y), c1 = make_double2(sl1 * z1.x, sl1 * z1.y);
 #pragma unroll
             for (int b = 0; b < W; ++b) {
                 F0[b].x = fma(c0.x, d[b], F0[b].x); F0[b].y = fma(c0.y, d[b], F0[b].y);
                 F1[b].x = fma(c1.x, d[b], F1[b].x); F1[b].y = fma(c1.y, d[b], F1[b].y);
             }
             if (l < Lc - 1) {
-                const double ca = cst[3 * (l - q)], cx = cst[3 * (l - q) + 1], cbb = cst[3 * (l - q) + 2];
 #pragma unroll
                 for (int b = 0; b < W; ++b) {
-                    const double dn = (ca * (cb[b] - cx)) * d[b] - cbb * dp[b];
+                    const double dn = (cu.ca * (cbv[ro + b] - cu.cx)) * d[b] - cu.cbb * dp[b];
                     dp[b] = d[b];
                     d[b] = dn;
                 }
@@ -133,16 +162,15 @@ __global__ __launch_bounds__(256) void refine_kernel(const double2 *__restrict__
         }
         // fold e^{-i n gamma_c} into G[b][m][c]
         auto fold = [&](const Variant &vv, const double2 (&Fk)[W]) {
+#pragma unroll 1
             for (int c = 0; c < W; ++c) {
-                // e^{-i n gamma} = cos(n gamma) - i sin(n gamma)
-                const double th = (double)vv.n * win[2][c] * h;
-                const double cn = cos(th), sn = -sin(th);
+                const double2 e = eg[c * NM + vv.n + Lc - 1];
 #pragma unroll
                 for (int b = 0; b < W; ++b) {
                     const double2 z = Fk[b];
                     double2 *dst = &G[(b * NM + (vv.m + Lc - 1)) * W + c];
-                    atomicAdd(&dst->x, z.x * cn - z.y * sn);
-                    atomicAdd(&dst->y, z.x * sn + z.y * cn);
+                    atomicAdd(&dst->x, z.x * e.x - z.y * e.y);
+                    atomicAdd(&dst->y, z.x * e.y + z.y * e.x);
                 }
             }
         };
@@ -153,13 +181,11 @@ __global__ __launch_bounds__(256) void refine_kernel(const double2 *__restrict__
     // alpha sums: C(a, b, c) = (1/2pi) sum_m G[b][m][c] e^{-i m alpha_a}
     for (int o = tid; o < W * W * W; o += blockDim.x) {
         const int a = o / (W * W), b = (o / W) % W, c = o % W;
-        const double al = win[0][a] * h;
         double re = 0.0, im = 0.0;
-        for (int m = -(Lc - 1); m <= Lc - 1; ++m) {
-            const double2 z = G[(b * NM + (m + Lc - 1)) * W + c];
-            const double cm = cos(m * al), sm_ = -sin(m * al);
-            re += z.x * cm - z.y * sm_;
-            im += z.x * sm_ + z.y * cm;
+        for (int mi = 0; mi < NM; ++mi) {
+            const double2 z = G[(b * NM + mi) * W + c], e = ea[a * NM + mi];
+            re += z.x * e.x - z.y * e.y;
+            im += z.x * e.y + z.y * e.x;
         }
         Cabs[o] = sqrt(re * re + im * im) / (2.0 * pi);
     }
@@ -251,15 +277,15 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
     RlTables t;
     if (int rc = rl_tables(t)) return rc;
     const int NM = 2 * (int)lmax - 1;
-    const size_t bytes = sizeof(double2) * 25 * NM;
+    const size_t bytes = sizeof(double2) * 35 * NM;   // G (25 NM) + eg, ea (5 NM each)
     static bool attr = false;
     if (!attr) {
-        if (hipFuncSetAttribute((const void *)refine_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024) !=
+        if (hipFuncSetAttribute((const void *)refine_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024) !=
             hipSuccess)
             return SO3FT_EHIP;
         attr = true;
     }
-    hipLaunchKernelGGL(refine_kernel, dim3(n), dim3(256), bytes, reinterpret_cast<hipStream_t>(stream),
+    hipLaunchKernelGGL(refine_kernel, dim3(n), dim3(768), bytes, reinterpret_cast<hipStream_t>(stream),
                        reinterpret_cast<const double2 *>(f_lms), reinterpret_cast<const double2 *>(g_lms), bw * bw,
                        (int)n, euler_in, euler_out, score, (int)grid_bw, (int)lmax, t.rc, t.off);
     return hipGetLastError() == hipSuccess ? SO3FT_OK : SO3FT_EHIP;
