@@ -98,6 +98,71 @@ def learned_alignment(args, dev, full_rate):
             "final_train_loss": float(np.mean(hist[-10:]))}
 
 
+def ragged_batch(args, dev, timed):
+    """Grouped mixed-precision isoft+soft of transforms with bandwidths drawn uniformly
+    from {32, 48, 64, 96, 128}: one grouped launch per direction (tuned launch shapes,
+    replayed HIP graph) against the same transforms run bandwidth by bandwidth through
+    the single-bandwidth complex64 path, and per-transform relative L2 vs FP64."""
+    import torch
+    from pysofft_amd import Soft
+    from pysofft_amd.grouped import SoftGroup
+    rng = np.random.default_rng(256)
+    bws = [int(b) for b in rng.choice([32, 48, 64, 96, 128], args.ragged_n)]
+    G = SoftGroup(bws, device=dev)
+    g = torch.Generator(device="cuda").manual_seed(99)
+    cg = torch.view_as_complex(torch.randn(G.coeff_total, 2, generator=g, device=dev))
+    fg, rg = G.empty_grids(), G.empty_coeffs()
+    cfg = G.autotune()
+
+    def step():
+        G.isoft(cg, out=fg)
+        G.soft(fg, out=rg)
+
+    for _ in range(2):
+        step()
+    k = max(2, args.steps // 4)
+    ms, _ = timed(step, k)
+    ms /= k
+    # the same transforms bandwidth by bandwidth (each bandwidth's transforms are
+    # contiguous in the packed layout)
+    segs = []
+    for b in sorted(set(bws), reverse=True):
+        idx = [i for i, x in enumerate(bws) if x == b]
+        T, n_c, n_g = len(idx), (4 * b ** 3 - b) // 3, 8 * b ** 3
+        c0, g0 = int(G.coeff_off[idx[0]]), int(G.grid_off[idx[0]])
+        segs.append((Soft(b), cg[c0:c0 + T * n_c].view(T, n_c), fg[g0:g0 + T * n_g].view(T, 2 * b, 2 * b, 2 * b),
+                     rg[c0:c0 + T * n_c].view(T, n_c)))
+
+    def step_single():
+        for s, c, f, r in segs:
+            s.isoft_many_c64(c, out=f)
+            s.soft_many_c64(f, out=r)
+
+    step_single()
+    ms1, _ = timed(step_single, k)
+    ms1 /= k
+    step()
+    torch.cuda.synchronize()
+    plans = {s.bw: s for s, _, _, _ in segs}
+    e = [0.0, 0.0, 0.0]
+    cv, fv, rv = G.coeff_views(cg), G.grid_views(fg), G.coeff_views(rg)
+    for i, b in enumerate(bws):
+        f64 = plans[b].isoft(cv[i].to(torch.complex128))
+        e[0] = max(e[0], ((fv[i].to(torch.complex128) - f64).norm() / f64.norm()).item())
+        c64 = plans[b].soft(fv[i].to(torch.complex128))
+        e[1] = max(e[1], ((rv[i].to(torch.complex128) - c64).norm() / c64.norm()).item())
+        e[2] = max(e[2], ((rv[i] - cv[i]).norm() / cv[i].norm()).item())
+    print(f"# ragged: grouped {ms:.3f} ms/step, bandwidth by bandwidth {ms1:.3f} ms/step", file=sys.stderr,
+          flush=True)
+    return {"workload": f"{len(bws)} complex64 transforms, bw uniform over {{32,48,64,96,128}}, coefficients iid "
+                        f"complex N(0,1); isoft+soft",
+            "bw_counts": {str(b): bws.count(b) for b in sorted(set(bws))},
+            "ms_per_step": ms, "pairs_per_s": len(bws) / (ms * 1e-3),
+            "bandwidth_by_bandwidth_ms_per_step": ms1, "grouped_speedup": ms1 / ms,
+            "relL2_vs_fp64_max": {"isoft": e[0], "soft": e[1], "round_trip": e[2]},
+            "tuned_config": cfg}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +178,8 @@ def main():
     ap.add_argument("--align-bw", type=int, default=64)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal alignment")
     ap.add_argument("--train-pairs", type=int, default=8_192_000)
+    ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-bandwidth grouped batch")
+    ap.add_argument("--ragged-n", type=int, default=256)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -238,6 +305,10 @@ def main():
         roof["share_of_step"] = per_step[dom] / sum(per_step.values())
     del f64, r64
 
+    ragged = None
+    if not args.no_ragged:
+        ragged = ragged_batch(args, dev, timed)
+
     align = None
     if not args.no_align:
         from pysofft_amd import workloads as W
@@ -295,7 +366,7 @@ def main():
                    "bw": bw, "batch": B,
                    "relL2_vs_fp64_max": {"isoft": err_isoft, "soft": err_soft, "round_trip": err_rt},
                    "fp64_pairs_per_s": fp64_value, "mixed_over_fp64": value / fp64_value, "wall_s": wall},
-        "roofline": roof, "kernels": kernels, "alignment": align, "cpu_baseline": cpu,
+        "roofline": roof, "kernels": kernels, "ragged_batch": ragged, "alignment": align, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

@@ -93,6 +93,32 @@ int so3ft_rsoft_many(so3ft_handle h, const void *so3funcs_real, void *coeffs, in
 int so3ft_isoft_many_c64(so3ft_handle h, const void *coeffs, void *so3funcs, int64_t n, void *stream);
 int so3ft_soft_many_c64(so3ft_handle h, const void *so3funcs, void *coeffs, int64_t n, void *stream);
 
+/* ---- grouped mixed-precision transforms of uneven bandwidths (no counterpart in the
+ * reference, whose *_many take one bandwidth per instance).  A group fixes n bandwidths
+ * (2 <= bw <= 256, bw = 2^a or 3*2^a); its transforms run as one persistent DWT launch
+ * over the ragged work of all bandwidths plus per-bandwidth FFT passes (parallel graph
+ * branches), captured into a hipGraph per (input, output) pointer pair on first use and
+ * replayed after.  Arrays are PACKED: transform i's complex64 coefficients (NC(bw_i),
+ * mnl) start at element coeff_off[i], its complex64 SO(3) function ((2bw_i)^3, C-order
+ * (gamma, alpha, beta)) at element grid_off[i] (so3ft_group_layout; host only, no GPU).
+ * The first transform call (or so3ft_group_autotune) times the candidate launch shapes on
+ * the group's shapes, synchronising the stream; the choice is kept per (device,
+ * bandwidth multiset).  Precision and semantics as so3ft_isoft_many_c64 /
+ * so3ft_soft_many_c64.  A group may be used from one stream at a time.
+ * config (6 int32): inverse TB, inverse grid, forward TB, forward NW, forward grid
+ * (grid 0 = one block per work item, else a multiple of 8 blocks looping over the work
+ * list), tuned flag (get only). */
+int so3ft_group_layout(const int64_t *bws, int64_t n, int64_t *coeff_off, int64_t *grid_off,
+                       int64_t *coeff_total, int64_t *grid_total);
+int so3ft_group_create(const int64_t *bws, int64_t n, int64_t *group);
+int so3ft_group_destroy(int64_t group);
+int so3ft_group_isoft_c64(int64_t group, const void *coeffs, void *so3funcs, void *stream);
+int so3ft_group_soft_c64(int64_t group, const void *so3funcs, void *coeffs, void *stream);
+int so3ft_group_autotune(int64_t group, void *stream);
+int so3ft_group_get_config(int64_t group, int32_t *config);
+int so3ft_group_set_config(int64_t group, const int32_t *config);
+int so3ft_group_set_graphs(int64_t group, int enable);  /* 0: launch eagerly on the stream */
+
 /* ---- stage-level entry points: py_wrap.f90:190-233 -------------------------
  * fft_array has the SO(3)-function shape; entry (i_n, i_m, beta) holds
  * F(beta, m, n) with i = m>=0 ? m : 2bw+m (pysofft/utils.f90:45-57).
@@ -117,6 +143,25 @@ int so3ft_cross_correlation_ylm_cmplx(so3ft_handle h, const void *f_lm, const vo
 int so3ft_cross_correlation_ylm_cmplx_3d(so3ft_handle h, const void *f_lms, const void *g_lms, void *cc,
                                          const double *radial_sampling_points, int64_t n_radial,
                                          int64_t r_lo, int64_t r_hi, void *stream);
+
+/* ---- real correlation: py_wrap.f90:392-402, 403-415.  f_ml, g_ml: complex128 MLr
+ * coefficients of real functions (m >= 0, f^l_m at m(2bw-1-m)/2 + l; bw(bw+1)/2 entries
+ * per set), cc: float64 (2bw, 2bw, 2bw).  3-D: (n_radial, bw(bw+1)/2) sets, limits as for
+ * the complex 3-D call. */
+int so3ft_cross_correlation_ylm_real(so3ft_handle h, const void *f_ml, const void *g_ml, void *cc, void *stream);
+int so3ft_cross_correlation_ylm_real_3d(so3ft_handle h, const void *f_mls, const void *g_mls, void *cc,
+                                        const double *radial_sampling_points, int64_t n_radial,
+                                        int64_t r_lo, int64_t r_hi, void *stream);
+
+/* ---- stage-level correlation DWTs: py_wrap.f90:346-356 py_inverse_wigner_trf_corr_cmplx
+ * (LMc inputs, every valid (m,n) column written, the others zeroed) and :357-367
+ * py_inverse_wigner_trf_corr_real (MLr inputs, the half-complex columns m >= 0 and
+ * (0, -n) written, the others zeroed); fft_array complex128 (2bw, 2bw, 2bw) as for
+ * so3ft_inverse_wigner_trf_cmplx. */
+int so3ft_inverse_wigner_trf_corr_cmplx(so3ft_handle h, const void *f_lm, const void *g_lm, void *fft_array,
+                                        void *stream);
+int so3ft_inverse_wigner_trf_corr_real(so3ft_handle h, const void *f_ml, const void *g_ml, void *fft_array,
+                                       void *stream);
 
 /* ---- batched correlation (no counterpart in the reference, whose callers loop over
  * py_cross_correlation_ylm_cmplx): f_lms, g_lms (n, bw^2) LMc; cc (n, 2bw, 2bw, 2bw). */

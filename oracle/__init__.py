@@ -13,7 +13,7 @@ Contents
 ``brute``          numpy/scipy restatements from the closed-form definitions
                    (docs/defs/wigner.md:9-16), used to pin both of the above.
 
-Parity status (what pins each oracle function) is listed in DESIGN.md.
+What pins each oracle function is tested in tests/test_oracle_pins.py.
 """
 from .oracle import *  # noqa: F401,F403
 from . import brute  # noqa: F401

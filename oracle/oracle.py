@@ -145,6 +145,19 @@ def genwig(p, q, bw):
     return out
 
 
+def mlr_to_lmc(fml, bw):
+    """MLr coefficients of a real function -> LMc (index l(l+1)+m) with
+    f^l_{-m} = (-1)^m conj(f^l_m)."""
+    fml = np.asarray(fml, np.complex128)
+    out = np.zeros(fml.shape[:-1] + (bw * bw,), np.complex128)
+    for l in range(bw):
+        for m in range(l + 1):
+            v = fml[..., (m * (2 * bw - 1 - m)) // 2 + l]
+            out[..., l * (l + 1) + m] = v
+            out[..., l * (l + 1) - m] = (-1) ** m * np.conj(v)
+    return out
+
+
 def enforce_real_sym(coeff, bw):
     """In-place, utils.f90:471-488."""
     assert coeff.dtype == np.complex128 and coeff.flags.c_contiguous
@@ -280,6 +293,64 @@ class OraclePlan:
     def cross_correlation_ylm_cmplx(self, flm, glm, use_mp=True):
         out = np.zeros(self.grid_shape, np.complex128)
         _lib.or_cross_correlation_cmplx(self._h, _v(_c128(flm)), _v(_c128(glm)), _v(out), int(use_mp))
+        return out
+
+    def cross_correlation_ylm_real(self, fml, gml, stage=False):
+        """Real correlation from MLr sets (index m(2bw-1-m)/2 + l), restated loop by loop
+        from soft.f08:2806-2872 (half-complex DWT of the pairs 0 <= m1 <= m2), :2932-2945
+        (the (0, -n) columns by conjugation) and :3118-3131 (conj, c2r FFTW_BACKWARD,
+        /2pi, here written as the real part of the forward 2-D DFT of the Hermitian
+        completion).  stage=True returns the half-complex fft array (i_n, i_m, beta)
+        before the FFT (py_inverse_wigner_trf_corr_real).  Pure numpy over pairs: small
+        bandwidths only."""
+        bw, N = self.bw, self.N
+        fml = np.asarray(fml, np.complex128)
+        gml = np.asarray(gml, np.complex128)
+        lmax = self.lmax
+        F = np.zeros((N, N, N), np.complex128)          # (i_n, i_m, beta)
+        l_all = np.arange(bw)
+        wn_all = 2 * np.pi * np.sqrt(2.0 / (2 * l_all + 1))
+        sym_all = (-1.0) ** l_all
+        mlr = lambda m, ls: (m * (2 * bw - 1 - m)) // 2 + ls
+        for m1 in range(lmax + 1):
+            s1 = (-1.0) ** m1
+            for m2 in range(m1, lmax + 1):
+                s2 = (-1.0) ** m2
+                ls = np.arange(m2, bw)
+                wig = genwig(m1, m2, bw)                    # (bw - m2, N)
+                wn, sy = wn_all[m2:], sym_all[m2:]
+                f1, f2 = fml[mlr(m1, ls)], fml[mlr(m2, ls)]
+                g1, g2 = gml[mlr(m1, ls)], gml[mlr(m2, ls)]
+                F[m2, m1, :] = (wn * f1 * np.conj(g2) * s1 * s2) @ wig
+                if m1 == 0 and m2 == 0:
+                    continue
+                if m1 != m2:
+                    F[m1, m2, :] = (wn * f2 * np.conj(g1)) @ wig
+                if m1 == 0 or m2 == 0:
+                    continue
+                F[N - m2, m1, ::-1] = s1 * ((wn * f1 * g2 * s1 * sy) @ wig)
+                if m1 == m2:
+                    continue
+                F[N - m1, m2, ::-1] = s1 * ((wn * f2 * g1 * s2 * sy) @ wig)
+        for m2 in range(1, lmax + 1):
+            F[N - m2, 0, :] = np.conj(F[m2, 0, :])
+        if stage:
+            return F
+        # Hermitian completion of the m < 0 columns: X(-m, -n) = conj X(m, n)
+        for im in range(bw + 1, N):
+            for i_n in range(N):
+                F[i_n, im, :] = np.conj(F[(N - i_n) % N, N - im, :])
+        return np.fft.fft2(F, axes=(0, 1)).real / (2 * np.pi)
+
+    def cross_correlation_ylm_real_3d(self, fmls, gmls, radial_sampling_points, radial_limits):
+        """soft.f08:3164-3253: the radially weighted sum of the shells' real correlations,
+        radial_limits = (lo, hi) with Python semantics [lo, hi)."""
+        r = np.asarray(radial_sampling_points, np.float64)
+        lo, hi = int(radial_limits[0]), int(radial_limits[1]) - 1
+        inv = 1.0 / (r[hi] - r[lo] + (r[1] - r[0]))
+        out = np.zeros(self.grid_shape)
+        for k in range(lo, hi + 1):
+            out += r[k] ** 2 * inv * self.cross_correlation_ylm_real(fmls[k], gmls[k])
         return out
 
     def cross_correlation_ylm_cmplx_3d(self, flms, glms, radial_sampling_points, radial_limits, use_mp=True):

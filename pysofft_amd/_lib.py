@@ -51,6 +51,10 @@ SIGNATURES = {
     "so3ft_cross_correlation_ylm_cmplx_many": (_int, [_i64, _vp, _vp, _vp, _i64, _vp]),
     "so3ft_cross_correlation_argmax_many": (_int, [_i64, _vp, _vp, _i64, _vp, _vp, _vp]),
     "so3ft_cross_correlation_ylm_cmplx_3d": (_int, [_i64, _vp, _vp, _vp, _P(_dbl), _i64, _i64, _i64, _vp]),
+    "so3ft_cross_correlation_ylm_real": (_int, [_i64, _vp, _vp, _vp, _vp]),
+    "so3ft_cross_correlation_ylm_real_3d": (_int, [_i64, _vp, _vp, _vp, _P(_dbl), _i64, _i64, _i64, _vp]),
+    "so3ft_inverse_wigner_trf_corr_cmplx": (_int, [_i64, _vp, _vp, _vp, _vp]),
+    "so3ft_inverse_wigner_trf_corr_real": (_int, [_i64, _vp, _vp, _vp, _vp]),
     "so3ft_enforce_real_sym_mnl": (_int, [_i64, _vp, _i64, _vp]),
     "so3ft_rotate_ylm_cmplx": (_int, [_i64, _vp, _i64, _vp, _i64, _int, _vp, _vp]),
     "so3ft_mlp_last_error": (C.c_char_p, []),
@@ -64,6 +68,15 @@ SIGNATURES = {
     "so3ft_align_features": (_int, [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
     "so3ft_gemm_bf16": (_int, [_vp, _vp, _vp, _int, _int, _int, _vp]),
     "so3ft_refine_correlation_peaks": (_int, [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _i64, _vp]),
+    "so3ft_group_layout": (_int, [_vp, _i64, _vp, _vp, _P(_i64), _P(_i64)]),
+    "so3ft_group_create": (_int, [_vp, _i64, _P(_i64)]),
+    "so3ft_group_destroy": (_int, [_i64]),
+    "so3ft_group_isoft_c64": (_int, [_i64, _vp, _vp, _vp]),
+    "so3ft_group_soft_c64": (_int, [_i64, _vp, _vp, _vp]),
+    "so3ft_group_autotune": (_int, [_i64, _vp]),
+    "so3ft_group_get_config": (_int, [_i64, _vp]),
+    "so3ft_group_set_config": (_int, [_i64, _vp]),
+    "so3ft_group_set_graphs": (_int, [_i64, _int]),
     "so3ft_bench_kernel": (_int, [_i64, _int, _int, _vp, _vp, _i64, _vp, _P(C.c_float)]),
 }
 

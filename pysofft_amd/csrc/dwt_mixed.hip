@@ -64,28 +64,21 @@ __device__ __forceinline__ void split(double d, float &hi, float &lo)
 // ---------------------------------------------------------------------------
 // inverse
 // ---------------------------------------------------------------------------
+constexpr int INV_DS = 80;  // padded row of the inverse d tile (float2 entries)
+
+// one inverse work item: order pair (p, q), transforms t0 .. t0+TB-1 (< T), betas
+// 64 chunk .. 64 chunk + 63; cst / Dl are the block's LDS
 template <int TB>
-__global__ __launch_bounds__(64) void dwt_inverse_mix(const float2 *__restrict__ coeff, int64_t c_ts,
-                                                      float2 *__restrict__ F, int64_t f_ts, int T,
-                                                      const double *__restrict__ cosb, const double *__restrict__ seeds,
-                                                      const int64_t *__restrict__ slice,
-                                                      const int32_t *__restrict__ pairs, int bw, int N, int npair,
-                                                      int nch, int nunit)
+__device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, int64_t c_ts,
+                                             float2 *__restrict__ F, int64_t f_ts, int T,
+                                             const double *__restrict__ cosb, const double *__restrict__ seeds,
+                                             const int64_t *__restrict__ slice, int bw, int N, int p, int q, int t0,
+                                             int chunk, double *cst, float2 (*Dl)[4][INV_DS])
 {
-    constexpr int DS = 80;                 // padded row of the d tile (float2 entries)
-    __shared__ double cst[3 * 258];
-    __shared__ float2 Dl[2][4][DS];        // [step parity][degree in step][beta] = (hi, lo)
-    const int id = blockIdx.x;
-    const int chunk = (id >> 3) % nch;
-    const int unit = ((id >> 3) / nch) * 8 + (id & 7);  // chunks of a unit share an XCD
-    if (unit >= nunit) return;
     const int lane = threadIdx.x;
-    const int pq = pairs[unit % npair];
-    const int p = pq & 0xffff, q = pq >> 16;
     const int L = bw - q;
     const int b0 = chunk * 64;
     const int beta = b0 + lane;
-    const int t0 = (unit / npair) * TB;
     const int nt = min(TB, T - t0);
 
     // B operand of this lane: column j = lane & 15 (variant j >> 1, part j & 1), degree
@@ -206,36 +199,68 @@ __global__ __launch_bounds__(64) void dwt_inverse_mix(const float2 *__restrict__
     }
 }
 
+template <int TB>
+__global__ __launch_bounds__(64) void dwt_inverse_mix(const float2 *__restrict__ coeff, int64_t c_ts,
+                                                      float2 *__restrict__ F, int64_t f_ts, int T,
+                                                      const double *__restrict__ cosb, const double *__restrict__ seeds,
+                                                      const int64_t *__restrict__ slice,
+                                                      const int32_t *__restrict__ pairs, int bw, int N, int npair,
+                                                      int nch, int nunit)
+{
+    __shared__ double cst[3 * 258];
+    __shared__ float2 Dl[2][4][INV_DS];    // [step parity][degree in step][beta] = (hi, lo)
+    const int id = blockIdx.x;
+    const int chunk = (id >> 3) % nch;
+    const int unit = ((id >> 3) / nch) * 8 + (id & 7);  // chunks of a unit share an XCD
+    if (unit >= nunit) return;
+    const int pq = pairs[unit % npair];
+    inv_mix_item<TB>(coeff, c_ts, F, f_ts, T, cosb, seeds, slice, bw, N, pq & 0xffff, pq >> 16,
+                     (unit / npair) * TB, chunk, cst, Dl);
+}
+
+// grouped launch over transforms of several bandwidths: a persistent grid walks the
+// work list (item = segment, p | q << 16, first transform, chunk; segment -1 = padding)
+template <int TB>
+__global__ __launch_bounds__(64) void dwt_inverse_mix_grouped(const float2 *__restrict__ coeff,
+                                                              float2 *__restrict__ F,
+                                                              const MixSegment *__restrict__ segs,
+                                                              const int4 *__restrict__ items, int nitem)
+{
+    __shared__ double cst[3 * 258];
+    __shared__ float2 Dl[2][4][INV_DS];
+    for (int it = blockIdx.x; it < nitem; it += gridDim.x) {
+        const int4 w = items[it];
+        if (w.x < 0) continue;
+        const MixSegment sg = segs[w.x];
+        __syncthreads();  // the previous item's LDS reads are done
+        inv_mix_item<TB>(coeff + sg.coff, sg.nc, F + sg.goff, (int64_t)sg.N * sg.N * sg.N, sg.T, sg.cosb, sg.seeds,
+                         sg.slice, sg.bw, sg.N, w.y & 0xffff, w.y >> 16, w.z, w.w, cst, Dl);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------
+constexpr int FWD_DS = 66;  // padded row of the forward tile (float2 entries)
+
+// one forward work item: order pair (p, q), degrees 64 chunk .. 64 chunk + 63; wave wv
+// owns transforms tb0 + wv TB .. tb0 + wv TB + TB - 1 (< T)
 template <int TB, int NW>
-__global__ __launch_bounds__(64 * NW) void dwt_forward_mix(const float2 *__restrict__ F, int64_t f_ts,
-                                                           float2 *__restrict__ coeff, int64_t c_ts, int T,
-                                                           const double *__restrict__ w,
-                                                           const double *__restrict__ cosb,
-                                                           const double *__restrict__ seeds,
-                                                           const int64_t *__restrict__ slice,
-                                                           const int32_t *__restrict__ pairs, int bw, int N, int npair,
-                                                           int nch, int nunit)
+__device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64_t f_ts,
+                                             float2 *__restrict__ coeff, int64_t c_ts, int T,
+                                             const double *__restrict__ w, const double *__restrict__ cosb,
+                                             const double *__restrict__ seeds, const int64_t *__restrict__ slice,
+                                             int bw, int N, int p, int q, int tb0, int chunk, double *cst,
+                                             float2 (*Dt)[FWD_DS])
 {
-    constexpr int DS = 66;                 // padded row of the tile (float2 entries)
     constexpr int BPW = 64 / NW;           // betas of a tile whose recurrence one wave runs
-    __shared__ double cst[3 * 258];
-    __shared__ float2 Dt[64][DS];          // [beta in tile][degree in chunk] = w d (hi, lo)
-    const int id = blockIdx.x;
-    const int chunk = (id >> 3) % nch;
-    const int unit = ((id >> 3) / nch) * 8 + (id & 7);
-    if (unit >= nunit) return;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index, uniform
-    const int pq = pairs[unit % npair];
-    const int p = pq & 0xffff, q = pq >> 16;
     const int L = bw - q;
     const int lc0 = chunk * 64;
     if (lc0 >= L) return;
     const int lend = min(L, lc0 + 64);
-    const int t0 = ((unit / npair) * NW + wv) * TB;
+    const int t0 = tb0 + wv * TB;
     const int nt = min(TB, T - t0);        // <= 0 for spare waves of the last group
 
     const int j = lane & 15, kk = lane >> 4;
@@ -369,6 +394,45 @@ __global__ __launch_bounds__(64 * NW) void dwt_forward_mix(const float2 *__restr
     }
 }
 
+template <int TB, int NW>
+__global__ __launch_bounds__(64 * NW) void dwt_forward_mix(const float2 *__restrict__ F, int64_t f_ts,
+                                                           float2 *__restrict__ coeff, int64_t c_ts, int T,
+                                                           const double *__restrict__ w,
+                                                           const double *__restrict__ cosb,
+                                                           const double *__restrict__ seeds,
+                                                           const int64_t *__restrict__ slice,
+                                                           const int32_t *__restrict__ pairs, int bw, int N, int npair,
+                                                           int nch, int nunit)
+{
+    __shared__ double cst[3 * 258];
+    __shared__ float2 Dt[64][FWD_DS];      // [beta in tile][degree in chunk] = w d (hi, lo)
+    const int id = blockIdx.x;
+    const int chunk = (id >> 3) % nch;
+    const int unit = ((id >> 3) / nch) * 8 + (id & 7);
+    if (unit >= nunit) return;
+    const int pq = pairs[unit % npair];
+    fwd_mix_item<TB, NW>(F, f_ts, coeff, c_ts, T, w, cosb, seeds, slice, bw, N, pq & 0xffff, pq >> 16,
+                         (unit / npair) * NW * TB, chunk, cst, Dt);
+}
+
+template <int TB, int NW>
+__global__ __launch_bounds__(64 * NW) void dwt_forward_mix_grouped(const float2 *__restrict__ F,
+                                                                   float2 *__restrict__ coeff,
+                                                                   const MixSegment *__restrict__ segs,
+                                                                   const int4 *__restrict__ items, int nitem)
+{
+    __shared__ double cst[3 * 258];
+    __shared__ float2 Dt[64][FWD_DS];
+    for (int it = blockIdx.x; it < nitem; it += gridDim.x) {
+        const int4 wk = items[it];
+        if (wk.x < 0) continue;
+        const MixSegment sg = segs[wk.x];
+        __syncthreads();  // every wave is done with the previous item's LDS
+        fwd_mix_item<TB, NW>(F + sg.goff, (int64_t)sg.N * sg.N * sg.N, coeff + sg.coff, sg.nc, sg.T, sg.w, sg.cosb,
+                             sg.seeds, sg.slice, sg.bw, sg.N, wk.y & 0xffff, wk.y >> 16, wk.z, wk.w, cst, Dt);
+    }
+}
+
 static int grid_m(int nunit, int nch) { return (nunit + 7) / 8 * 8 * nch; }
 
 hipError_t launch_dwt_inverse_mixed(const DeviceTables &tb, int bw, int N, int npair, int mode, const float2 *coeff,
@@ -407,6 +471,61 @@ hipError_t launch_dwt_forward_mixed(const DeviceTables &tb, int bw, int N, int n
     else SO3_MFWD(1, 1);
 #undef SO3_MFWD
     return hipGetLastError();
+}
+
+// ---- grouped launches: TB (inverse), TB x NW (forward) from the tuned configuration
+hipError_t launch_dwt_inverse_mixed_grouped(int TB, const float2 *coeff, float2 *F, const MixSegment *segs,
+                                            const int4 *items, int nitem, int grid, hipStream_t s)
+{
+    if (nitem <= 0) return hipSuccess;
+    if (grid <= 0) return hipErrorInvalidValue;
+    if (TB == 2)
+        hipLaunchKernelGGL((dwt_inverse_mix_grouped<2>), dim3(grid), dim3(64), 0, s, coeff, F, segs, items, nitem);
+    else if (TB == 1)
+        hipLaunchKernelGGL((dwt_inverse_mix_grouped<1>), dim3(grid), dim3(64), 0, s, coeff, F, segs, items, nitem);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_dwt_forward_mixed_grouped(int TB, int NW, const float2 *F, float2 *coeff, const MixSegment *segs,
+                                            const int4 *items, int nitem, int grid, hipStream_t s)
+{
+    if (nitem <= 0) return hipSuccess;
+    if (grid <= 0) return hipErrorInvalidValue;
+#define SO3_GFWD(TBV, NWV)                                                                                     \
+    if (TB == TBV && NW == NWV) {                                                                              \
+        hipLaunchKernelGGL((dwt_forward_mix_grouped<TBV, NWV>), dim3(grid), dim3(64 * NWV), 0, s, F, coeff, segs, \
+                           items, nitem);                                                                      \
+        return hipGetLastError();                                                                              \
+    }
+    SO3_GFWD(2, 4)
+    SO3_GFWD(1, 4)
+    SO3_GFWD(1, 2)
+    SO3_GFWD(1, 1)
+#undef SO3_GFWD
+    return hipErrorInvalidValue;
+}
+
+// resident blocks per CU of a grouped kernel
+int mixed_grouped_occupancy(int inverse, int TB, int NW)
+{
+    int nb = 0;
+    hipError_t e = hipErrorInvalidValue;
+    if (inverse) {
+        if (TB == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_inverse_mix_grouped<2>, 64, 0);
+        else if (TB == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_inverse_mix_grouped<1>, 64, 0);
+    } else {
+        if (TB == 2 && NW == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<2, 4>, 256, 0);
+        else if (TB == 1 && NW == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<1, 4>, 256, 0);
+        else if (TB == 1 && NW == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<1, 2>, 128, 0);
+        else if (TB == 1 && NW == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<1, 1>, 64, 0);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return nb;
 }
 
 }  // namespace so3

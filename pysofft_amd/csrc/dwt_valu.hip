@@ -398,9 +398,28 @@ __global__ __launch_bounds__(64 * NW) void dwt_forward_valu(const double2 *__res
     }
 }
 
-// cc^l_{mn} = 2 pi sqrt(2/(2l+1)) (-1)^{m+n} f^l_m conj(g^l_n) in mnl order
-// (pysofft/soft.f08:2242-2341); f, g in SHTNS LMc order l(l+1)+m.  One block per (m, n).
-__global__ void corr_coeffs_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g, int64_t sh_ts,
+// f^l_m of one SH coefficient set: layout 0 = SHTNS LMc (index l(l+1)+m); layout 1 =
+// MLr of a real function (pysofft/utils.f90:758-772, m >= 0 at m(2bw-1-m)/2 + l), the
+// negative orders following from f^l_{-m} = (-1)^m conj(f^l_m)
+__device__ __forceinline__ double2 sh_coeff(const double2 *__restrict__ x, int layout, int l, int m, int bw)
+{
+    if (layout == 0) return x[l * (l + 1) + m];
+    const int am = m < 0 ? -m : m;
+    double2 v = x[(int64_t)am * (2 * bw - 1 - am) / 2 + l];
+    if (m < 0) {
+        v.y = -v.y;
+        if (am & 1) { v.x = -v.x; v.y = -v.y; }
+    }
+    return v;
+}
+
+// cc^l_{mn} = 2 pi sqrt(2/(2l+1)) (-1)^{m+n} sum_r w_r f^l_{m,r} conj(g^l_{n,r}) in mnl order
+// (pysofft/soft.f08:2242-2341; real inputs soft.f08:2806-2872, whose half-complex
+// branches are this formula with the negative orders completed).  Transform t = blockIdx.y
+// reads its sets at t * t_ts; nr shells r at stride r_ts are summed with weights w (w ==
+// nullptr: one shell of weight 1).  One block per (m, n).
+__global__ void corr_coeffs_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g, int64_t t_ts,
+                                   int layout, int nr, int64_t r_ts, const double *__restrict__ w,
                                    double2 *__restrict__ out, int64_t c_ts, const int64_t *__restrict__ slice,
                                    int bw)
 {
@@ -411,18 +430,26 @@ __global__ void corr_coeffs_kernel(const double2 *__restrict__ f, const double2 
     const int64_t t = blockIdx.y;
     const double sg = ((m + n) & 1) ? -1.0 : 1.0;
     for (int l = lmin + (int)threadIdx.x; l < bw; l += blockDim.x) {
-        const double2 a = f[t * sh_ts + l * (l + 1) + m];
-        const double2 b = g[t * sh_ts + l * (l + 1) + n];
+        double re = 0.0, im = 0.0;
+        for (int r = 0; r < nr; ++r) {
+            const double2 a = sh_coeff(f + t * t_ts + r * r_ts, layout, l, m, bw);
+            const double2 b = sh_coeff(g + t * t_ts + r * r_ts, layout, l, n, bw);
+            const double wr = w ? w[r] : 1.0;
+            re += wr * (a.x * b.x + a.y * b.y);
+            im += wr * (a.y * b.x - a.x * b.y);
+        }
         const double wn = sg * 2.0 * 3.14159265358979323846 * sqrt(2.0 / (double)(2 * l + 1));
-        out[t * c_ts + base + (l - lmin)] = make_double2(wn * (a.x * b.x + a.y * b.y), wn * (a.y * b.x - a.x * b.y));
+        out[t * c_ts + base + (l - lmin)] = make_double2(wn * re, wn * im);
     }
 }
 
-hipError_t launch_corr_coeffs(const DeviceTables &tb, int bw, const double2 *f, const double2 *g, int64_t sh_ts,
-                              double2 *out, int64_t c_ts, int T, hipStream_t s)
+hipError_t launch_corr_coeffs(const DeviceTables &tb, int bw, const double2 *f, const double2 *g, int64_t t_ts,
+                              int layout, int nr, int64_t r_ts, const double *w, double2 *out, int64_t c_ts, int T,
+                              hipStream_t s)
 {
     const int nsl = (2 * bw - 1) * (2 * bw - 1);
-    hipLaunchKernelGGL(corr_coeffs_kernel, dim3(nsl, T), dim3(64), 0, s, f, g, sh_ts, out, c_ts, tb.slice, bw);
+    hipLaunchKernelGGL(corr_coeffs_kernel, dim3(nsl, T), dim3(64), 0, s, f, g, t_ts, layout, nr, r_ts, w, out, c_ts,
+                       tb.slice, bw);
     return hipGetLastError();
 }
 

@@ -174,22 +174,24 @@ hipError_t launch_fft_axis(const FftDesc &fd, const double2 *tw, int bw, int lma
 // small helpers
 // ---------------------------------------------------------------------------
 
-__global__ void zero_invalid_kernel(double2 *F, int64_t f_ts, int bw, int N, int lmax)
+__global__ void zero_invalid_kernel(double2 *F, int64_t f_ts, int bw, int N, int lmax, int half)
 {
     const int64_t t = blockIdx.y;
     const int64_t n3 = (int64_t)N * N * N;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x) {
         const int im = (int)((i / N) % N), in = (int)(i / ((int64_t)N * N));
-        if (!order_ok(im, N, bw, lmax) || !order_ok(in, N, bw, lmax)) F[t * f_ts + i] = make_double2(0.0, 0.0);
+        if (!order_ok(im, N, bw, lmax) || !order_ok(in, N, bw, lmax) || (half && im > bw))
+            F[t * f_ts + i] = make_double2(0.0, 0.0);
     }
 }
 
-hipError_t launch_zero_invalid_columns(double2 *F, int64_t f_ts, int bw, int N, int lmax, int T, hipStream_t s)
+hipError_t launch_zero_invalid_columns(double2 *F, int64_t f_ts, int bw, int N, int lmax, int T, int half,
+                                       hipStream_t s)
 {
     const int64_t n3 = (int64_t)N * N * N;
     int nb = (int)((n3 + 255) / 256);
     if (nb > 4096) nb = 4096;
-    hipLaunchKernelGGL(zero_invalid_kernel, dim3(nb, T), dim3(256), 0, s, F, f_ts, bw, N, lmax);
+    hipLaunchKernelGGL(zero_invalid_kernel, dim3(nb, T), dim3(256), 0, s, F, f_ts, bw, N, lmax, half);
     return hipGetLastError();
 }
 
@@ -292,23 +294,6 @@ hipError_t launch_enforce_real_sym(double2 *c, int64_t c_ts, const int64_t *slic
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
-}
-
-__global__ void radial_acc_kernel(double2 *acc, const double2 *x, int64_t n, double w, int first)
-{
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double2 v = x[i];
-        if (first) acc[i] = make_double2(w * v.x, w * v.y);
-        else { double2 a = acc[i]; acc[i] = make_double2(a.x + w * v.x, a.y + w * v.y); }
-    }
-}
-
-hipError_t launch_radial_accumulate(double2 *acc, const double2 *x, int64_t n, double w, int first, hipStream_t s)
-{
-    int nb = (int)((n + 255) / 256);
-    if (nb > 8192) nb = 8192;
-    hipLaunchKernelGGL(radial_acc_kernel, dim3(nb), dim3(256), 0, s, acc, x, n, w, first);
-    return hipGetLastError();
 }
 
 }  // namespace so3

@@ -51,6 +51,8 @@ struct Plan {
     int64_t *amax_i = nullptr;
     int64_t amax_cap = 0;
     double *red = nullptr;  // integration partials + result
+    double *radw = nullptr; // radial shell weights (3-D correlation)
+    int64_t radw_cap = 0;
 };
 
 std::mutex g_mu;
@@ -73,6 +75,8 @@ void free_plan_device(Plan *P)
     P->cstage = nullptr; P->cstage_T = 0;
     hipFree(P->amax_v); hipFree(P->amax_i);
     P->amax_v = nullptr; P->amax_i = nullptr; P->amax_cap = 0;
+    hipFree(P->radw);
+    P->radw = nullptr; P->radw_cap = 0;
 }
 
 template <class T>
@@ -287,7 +291,7 @@ int dwt_inverse(Plan *P, int mode, const double2 *c, int64_t c_ts, const double2
     }
     if (mode == DWT_CORR) {
         if (int rc = ensure_cstage(P, T)) return rc;
-        HIPCHK(launch_corr_coeffs(P->tb, bw, c, aux, c_ts, P->cstage, P->nc, (int)T, s));
+        HIPCHK(launch_corr_coeffs(P->tb, bw, c, aux, c_ts, 0, 1, 0, nullptr, P->cstage, P->nc, (int)T, s));
         c = P->cstage; c_ts = P->nc; mode = DWT_COMPLEX;
     }
     HIPCHK(launch_dwt_inverse_valu(P->tb, bw, N, np, mode, c, c_ts, F, f_ts, (int)T, s));
@@ -337,6 +341,45 @@ int run_forward(Plan *P, int mode, const void *in, int in_real, void *coeff, int
         double2 *c = reinterpret_cast<double2 *>(coeff) + i0 * P->nc;
         if ((rc = dwt_forward(P, mode, P->scratch, N3, c, P->nc, T, s))) return rc;
     }
+    return SO3FT_OK;
+}
+
+// correlation coefficients of T pairs (layout 0 LMc, bw^2 per set; 1 MLr, bw(bw+1)/2 per
+// set) into the plan's staging array; with host weights hw, the weighted sum over nr
+// consecutive shells of one pair
+int stage_corr(Plan *P, int layout, const void *f, const void *g, int64_t T, int nr, const double *hw, hipStream_t s)
+{
+    if (int rc = ensure_cstage(P, T)) return rc;
+    const int64_t nb = layout == 0 ? P->bw * P->bw : P->bw * (P->bw + 1) / 2;
+    const double *w = nullptr;
+    if (hw) {
+        if (P->radw_cap < nr) {
+            HIPCHK(hipStreamSynchronize(s));
+            hipFree(P->radw);
+            P->radw = nullptr; P->radw_cap = 0;
+            HIPCHK(hipMalloc((void **)&P->radw, sizeof(double) * nr));
+            P->radw_cap = nr;
+        }
+        HIPCHK(hipStreamSynchronize(s));  // the previous call's staging has read the weights
+        HIPCHK(hipMemcpy(P->radw, hw, sizeof(double) * nr, hipMemcpyHostToDevice));
+        w = P->radw;
+    }
+    HIPCHK(launch_corr_coeffs(P->tb, (int)P->bw, reinterpret_cast<const double2 *>(f),
+                              reinterpret_cast<const double2 *>(g), nb, layout, hw ? nr : 1, nb, w, P->cstage, P->nc,
+                              (int)T, s));
+    return SO3FT_OK;
+}
+
+// soft.f08:2696-2697,2737: weights r^2 / (r_hi - r_lo + dr) of the shells r_lo..r_hi
+int radial_weights(const double *rad, int64_t n_radial, int64_t r_lo, int64_t r_hi, std::vector<double> &w)
+{
+    if (!rad) return fail(SO3FT_EINVAL, "null radial_sampling_points");
+    if (n_radial < 2 || r_lo < 0 || r_hi >= n_radial || r_lo > r_hi)
+        return fail(SO3FT_EINVAL, "radial_limits out of bounds for radial_sampling_points");
+    const double step = rad[1] - rad[0];
+    const double inv = 1.0 / (rad[r_hi] - rad[r_lo] + step);
+    w.clear();
+    for (int64_t r = r_lo; r <= r_hi; ++r) w.push_back(rad[r] * rad[r] * inv);
     return SO3FT_OK;
 }
 
@@ -611,7 +654,7 @@ int so3ft_inverse_wigner_trf_cmplx(so3ft_handle h, const void *coeff, void *fft_
     double2 *F = reinterpret_cast<double2 *>(fft_array);
     if (int rc = dwt_inverse(P, DWT_COMPLEX, reinterpret_cast<const double2 *>(coeff), P->nc, nullptr, F, N3, 1, S(stream)))
         return rc;
-    HIPCHK(launch_zero_invalid_columns(F, N3, (int)P->bw, (int)P->N, (int)P->lmax, 1, S(stream)));
+    HIPCHK(launch_zero_invalid_columns(F, N3, (int)P->bw, (int)P->N, (int)P->lmax, 1, 0, S(stream)));
     return SO3FT_OK;
 }
 
@@ -734,25 +777,69 @@ int so3ft_cross_correlation_ylm_cmplx_3d(so3ft_handle h, const void *f_lms, cons
                                          void *stream)
 {
     GETP(h);
-    if (!f_lms || !g_lms || !cc || !rad) return fail(SO3FT_EINVAL, "null pointer");
-    if (n_radial < 2 || r_lo < 0 || r_hi >= n_radial || r_lo > r_hi)
-        return fail(SO3FT_EINVAL, "radial_limits out of bounds for radial_sampling_points");
-    const int64_t N3 = P->N * P->N * P->N, nb = P->bw * P->bw;
-    // soft.f08:2696-2737: sum_r r^2 C_r / (r_hi - r_lo + dr); one shell at a time through a
-    // second scratch slot
-    int rc = ensure_scratch(P, 2);
-    if (rc) return rc;
-    double2 *tmp = P->scratch + N3;
-    const double step = rad[1] - rad[0];
-    const double inv = 1.0 / (rad[r_hi] - rad[r_lo] + step);
-    for (int64_t r = r_lo; r <= r_hi; ++r) {
-        const double2 *f = reinterpret_cast<const double2 *>(f_lms) + r * nb;
-        const double2 *g = reinterpret_cast<const double2 *>(g_lms) + r * nb;
-        if ((rc = dwt_inverse(P, DWT_CORR, f, nb, g, P->scratch, N3, 1, S(stream)))) return rc;
-        if ((rc = inverse_fft_passes(P, P->scratch, tmp, 0, 1, S(stream)))) return rc;
-        HIPCHK(launch_radial_accumulate(reinterpret_cast<double2 *>(cc), tmp, N3, rad[r] * rad[r] * inv,
-                                        r == r_lo, S(stream)));
-    }
+    if (!f_lms || !g_lms || !cc) return fail(SO3FT_EINVAL, "null pointer");
+    // soft.f08:2657-2738 sums the shells' correlation grids; the correlation is linear in
+    // its coefficients, so the weighted shell sum is taken over the coefficients and one
+    // transform follows
+    std::vector<double> w;
+    if (int rc = radial_weights(rad, n_radial, r_lo, r_hi, w)) return rc;
+    const int64_t nb = P->bw * P->bw;
+    if (int rc = stage_corr(P, 0, reinterpret_cast<const double2 *>(f_lms) + r_lo * nb,
+                            reinterpret_cast<const double2 *>(g_lms) + r_lo * nb, 1, (int)w.size(), w.data(),
+                            S(stream)))
+        return rc;
+    return run_inverse(P, DWT_COMPLEX, P->cstage, P->nc, nullptr, cc, 0, 1, S(stream));
+}
+
+int so3ft_cross_correlation_ylm_real(so3ft_handle h, const void *f_ml, const void *g_ml, void *cc, void *stream)
+{
+    GETP(h);
+    if (!f_ml || !g_ml || !cc) return fail(SO3FT_EINVAL, "null pointer");
+    // soft.f08:2741-2946,3118-3131: the coefficients of a real pair are Hermitian
+    // (cc^l_{-m,-n} = (-1)^{m+n} conj cc^l_{mn}), so the correlation is irsoft of them
+    if (int rc = stage_corr(P, 1, f_ml, g_ml, 1, 1, nullptr, S(stream))) return rc;
+    return run_inverse(P, DWT_REAL, P->cstage, P->nc, nullptr, cc, 1, 1, S(stream));
+}
+
+int so3ft_cross_correlation_ylm_real_3d(so3ft_handle h, const void *f_mls, const void *g_mls, void *cc,
+                                        const double *rad, int64_t n_radial, int64_t r_lo, int64_t r_hi, void *stream)
+{
+    GETP(h);
+    if (!f_mls || !g_mls || !cc) return fail(SO3FT_EINVAL, "null pointer");
+    std::vector<double> w;
+    if (int rc = radial_weights(rad, n_radial, r_lo, r_hi, w)) return rc;
+    const int64_t nb = P->bw * (P->bw + 1) / 2;
+    if (int rc = stage_corr(P, 1, reinterpret_cast<const double2 *>(f_mls) + r_lo * nb,
+                            reinterpret_cast<const double2 *>(g_mls) + r_lo * nb, 1, (int)w.size(), w.data(),
+                            S(stream)))
+        return rc;
+    return run_inverse(P, DWT_REAL, P->cstage, P->nc, nullptr, cc, 1, 1, S(stream));
+}
+
+int so3ft_inverse_wigner_trf_corr_cmplx(so3ft_handle h, const void *f_lm, const void *g_lm, void *fft_array,
+                                        void *stream)
+{
+    GETP(h);
+    if (!f_lm || !g_lm || !fft_array) return fail(SO3FT_EINVAL, "null pointer");
+    if (int rc = stage_corr(P, 0, f_lm, g_lm, 1, 1, nullptr, S(stream))) return rc;
+    const int64_t N3 = P->N * P->N * P->N;
+    double2 *F = reinterpret_cast<double2 *>(fft_array);
+    if (int rc = dwt_inverse(P, DWT_COMPLEX, P->cstage, P->nc, nullptr, F, N3, 1, S(stream))) return rc;
+    HIPCHK(launch_zero_invalid_columns(F, N3, (int)P->bw, (int)P->N, (int)P->lmax, 1, 0, S(stream)));
+    return SO3FT_OK;
+}
+
+int so3ft_inverse_wigner_trf_corr_real(so3ft_handle h, const void *f_ml, const void *g_ml, void *fft_array,
+                                       void *stream)
+{
+    GETP(h);
+    if (!f_ml || !g_ml || !fft_array) return fail(SO3FT_EINVAL, "null pointer");
+    if (int rc = stage_corr(P, 1, f_ml, g_ml, 1, 1, nullptr, S(stream))) return rc;
+    const int64_t N3 = P->N * P->N * P->N;
+    double2 *F = reinterpret_cast<double2 *>(fft_array);
+    if (int rc = dwt_inverse(P, DWT_REAL, P->cstage, P->nc, nullptr, F, N3, 1, S(stream))) return rc;
+    // the half-complex array of soft.f08:2874-2945: columns m >= 0 (and (0, -n)), zero elsewhere
+    HIPCHK(launch_zero_invalid_columns(F, N3, (int)P->bw, (int)P->N, (int)P->lmax, 1, 1, S(stream)));
     return SO3FT_OK;
 }
 
@@ -858,3 +945,19 @@ int so3ft_enforce_real_sym_mnl(int64_t bw, void *coeffs, int64_t n, void *stream
 }
 
 }  // extern "C"
+
+namespace so3 {
+
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+
+int plan_device_tables(int64_t h, DeviceTables *tb, int64_t *bw, int64_t *lmax)
+{
+    Plan *P = get(h);
+    if (!P) return fail(SO3FT_EHANDLE, "unknown handle");
+    *tb = P->tb;
+    *bw = P->bw;
+    *lmax = P->lmax;
+    return SO3FT_OK;
+}
+
+}  // namespace so3

@@ -81,9 +81,12 @@ hipError_t launch_dwt_forward_valu(const DeviceTables &t, int bw, int N, int npa
                                    const double2 *F, int64_t f_ts, double2 *coeff, int64_t c_ts, int T,
                                    hipStream_t s);
 
-// mnl correlation coefficients of T pairs of SH sets (LMc, stride sh_ts) into out (stride c_ts)
-hipError_t launch_corr_coeffs(const DeviceTables &t, int bw, const double2 *f, const double2 *g, int64_t sh_ts,
-                              double2 *out, int64_t c_ts, int T, hipStream_t s);
+// mnl correlation coefficients of T pairs of SH sets (transform stride t_ts; layout 0 LMc,
+// 1 MLr of real functions), each the w-weighted sum over nr shells at stride r_ts (w ==
+// nullptr: nr = 1, weight 1), into out (stride c_ts)
+hipError_t launch_corr_coeffs(const DeviceTables &t, int bw, const double2 *f, const double2 *g, int64_t t_ts,
+                              int layout, int nr, int64_t r_ts, const double *w, double2 *out, int64_t c_ts, int T,
+                              hipStream_t s);
 
 // mixed precision: complex64 coefficients and DWT arrays, FP64 recurrence, FP32 MFMA
 // with d split into hi + lo, FP64 accumulation across 16-term blocks
@@ -92,6 +95,23 @@ hipError_t launch_dwt_inverse_mixed(const DeviceTables &t, int bw, int N, int np
                                     int T, hipStream_t s);
 hipError_t launch_dwt_forward_mixed(const DeviceTables &t, int bw, int N, int npair, int mode, const float2 *F,
                                     int64_t f_ts, float2 *coeff, int64_t c_ts, int T, hipStream_t s);
+
+// one bandwidth of a grouped mixed-precision batch: its plan tables and the element
+// offsets of its first transform in the packed coefficient / grid arrays (the segment's
+// transforms are contiguous, nc resp. N^3 apart)
+struct MixSegment {
+    const double *cosb, *w, *seeds;
+    const int64_t *slice;
+    int bw, N, T, pad;
+    int64_t coff, goff, nc;
+};
+// work items: x = segment (-1 = padding), y = p | q << 16, z = first transform, w = chunk
+// (64 betas for the inverse, 64 degrees for the forward)
+hipError_t launch_dwt_inverse_mixed_grouped(int TB, const float2 *coeff, float2 *F, const MixSegment *segs,
+                                            const int4 *items, int nitem, int grid, hipStream_t s);
+hipError_t launch_dwt_forward_mixed_grouped(int TB, int NW, const float2 *F, float2 *coeff, const MixSegment *segs,
+                                            const int4 *items, int nitem, int grid, hipStream_t s);
+int mixed_grouped_occupancy(int inverse, int TB, int NW);
 
 struct FftAxis {
     int64_t in_t, in_o, in_r;      // element strides: transform, outer line, fft row
@@ -111,21 +131,26 @@ bool fft_reg_supported(int N);
 hipError_t launch_fft_reg(int N, const double2 *tw, int bw, int lmax, const void *in, void *out,
                           const FftAxis &ax, int T, int in_c64, int out_c64, int col0, int ncols, hipStream_t s);
 
-hipError_t launch_zero_invalid_columns(double2 *F, int64_t f_ts, int bw, int N, int lmax, int T,
+// zero the fft-array columns whose order is Nyquist or beyond lmax; half: also every
+// column with m < 0 (the half-complex array of the real transforms)
+hipError_t launch_zero_invalid_columns(double2 *F, int64_t f_ts, int bw, int N, int lmax, int T, int half,
                                        hipStream_t s);
 hipError_t launch_scale(double2 *X, int64_t n, double scale, hipStream_t s);
 hipError_t launch_integrate(const void *f, int is_real, const double *w, int N, double *partials,
                             int nblocks, double *result2, hipStream_t s);
 hipError_t launch_enforce_real_sym(double2 *c, int64_t c_ts, const int64_t *slice, int bw, int T,
                                    hipStream_t s);
-hipError_t launch_radial_accumulate(double2 *acc, const double2 *x, int64_t n, double w, int first,
-                                    hipStream_t s);
 
 // per transform: first flat index (and value) of max |x| over n_per complex values;
 // pv / pi hold argmax_partials(n_per) * T partials
 int argmax_partials(int64_t n_per);
 hipError_t launch_argmax_abs(const double2 *x, int64_t n_per, int T, double *pv, int64_t *pi, int64_t *idx,
                              double *val, hipStream_t s);
+
+// shared with the other translation units of the C ABI: set the calling thread's
+// so3ft_last_error() message (returns code); device tables of a plan handle
+int set_error(int code, const std::string &msg);
+int plan_device_tables(int64_t h, DeviceTables *tb, int64_t *bw, int64_t *lmax);
 
 int dwt_lds_bytes_inverse(int bw, int N);
 int dwt_lds_bytes_forward(int bw, int N);

@@ -421,6 +421,74 @@ class Soft:
                 int(radial_limits[1]) - 1, _stream_ptr(self.device)))
             return self._finish(o, f_lms, out)
 
+    def _mlr_sets(self, x, many):
+        """Complex128 MLr sets on the device: (bw(bw+1)/2,) (or (n, .)); longer trailing
+        axes (the reference documents (bw**2,)) are cut to the used entries."""
+        torch = _torch()
+        nr = self.bw * (self.bw + 1) // 2
+        d = _Dev(x, torch.complex128, self.device, None)
+        t = d.t
+        if t.ndim != (2 if many else 1) or t.shape[-1] < nr:
+            raise ValueError(f"expected MLr coefficient sets with >= {nr} entries, got shape {tuple(t.shape)}")
+        d.t = t[..., :nr].contiguous()
+        return d
+
+    def cross_correlation_ylm_real(self, f_lm, g_lm, out=None, use_mp=False):
+        """Correlation of two real functions from MLr coefficients (pysofft/soft.py:594-626)."""
+        torch = _torch()
+        with torch.cuda.device(self.device):
+            f = self._mlr_sets(f_lm, False)
+            g = self._mlr_sets(g_lm, False)
+            o = torch.empty(self.grid_shape, dtype=torch.float64, device=self.device)
+            _lib.check(_lib.lib().so3ft_cross_correlation_ylm_real(self._handle, f.ptr, g.ptr, C.c_void_p(o.data_ptr()),
+                                                                  _stream_ptr(self.device)))
+            return self._finish(o, f_lm, out)
+
+    def cross_correlation_ylm_real_3d(self, f_lms, g_lms, out=None, radial_sampling_points=None,
+                                      radial_limits=None, use_mp=False):
+        """Radially weighted correlation of real functions given per shell (pysofft/soft.py:666-703)."""
+        torch = _torch()
+        n = len(f_lms)
+        if radial_sampling_points is None:
+            radial_sampling_points = np.linspace(0, 1, n)
+        if radial_limits is None:
+            radial_limits = [0, n]
+        rad = np.ascontiguousarray(radial_sampling_points, dtype=np.float64)
+        with torch.cuda.device(self.device):
+            f = self._mlr_sets(f_lms, True)
+            g = self._mlr_sets(g_lms, True)
+            o = torch.empty(self.grid_shape, dtype=torch.float64, device=self.device)
+            _lib.check(_lib.lib().so3ft_cross_correlation_ylm_real_3d(
+                self._handle, f.ptr, g.ptr, C.c_void_p(o.data_ptr()),
+                rad.ctypes.data_as(C.POINTER(C.c_double)), len(rad), int(radial_limits[0]),
+                int(radial_limits[1]) - 1, _stream_ptr(self.device)))
+            return self._finish(o, f_lms, out)
+
+    def _inverse_wigner_trf_corr_cmplx(self, f_lm, g_lm, out=None, use_mp=False):
+        """Correlation DWT stage (pysofft/soft.py:550-554): the fft array before the FFT."""
+        torch = _torch()
+        nb = self.bw ** 2
+        with torch.cuda.device(self.device):
+            f = _Dev(f_lm, torch.complex128, self.device, (nb,))
+            g = _Dev(g_lm, torch.complex128, self.device, (nb,))
+            o = torch.empty(self.grid_shape, dtype=torch.complex128, device=self.device)
+            _lib.check(_lib.lib().so3ft_inverse_wigner_trf_corr_cmplx(self._handle, f.ptr, g.ptr,
+                                                                     C.c_void_p(o.data_ptr()),
+                                                                     _stream_ptr(self.device)))
+            return self._finish(o, f_lm, out)
+
+    def _inverse_wigner_trf_corr_real(self, f_ml, g_ml, out=None, use_mp=False):
+        """Real correlation DWT stage (pysofft/soft.py:555-559): half-complex fft array."""
+        torch = _torch()
+        with torch.cuda.device(self.device):
+            f = self._mlr_sets(f_ml, False)
+            g = self._mlr_sets(g_ml, False)
+            o = torch.empty(self.grid_shape, dtype=torch.complex128, device=self.device)
+            _lib.check(_lib.lib().so3ft_inverse_wigner_trf_corr_real(self._handle, f.ptr, g.ptr,
+                                                                    C.c_void_p(o.data_ptr()),
+                                                                    _stream_ptr(self.device)))
+            return self._finish(o, f_ml, out)
+
     def _finish(self, o, like, out):
         if _is_torch(like):
             if out is not None:

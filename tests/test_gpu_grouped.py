@@ -1,0 +1,142 @@
+"""GPU tests of the grouped mixed-precision transforms over uneven bandwidths.
+
+The grouped launch runs the same per-item kernel bodies and FFT passes as the
+single-bandwidth complex64 path, so its results must equal that path's bit for bit
+for every launch shape; against the FP64 path (itself pinned to the oracle in
+test_gpu_transforms.py) each transform meets relative L2 <= 1e-7.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+TOL = 1e-7          # per-transform relative L2 vs FP64 (isoft grids, soft coefficients)
+TOL_RT = 2e-7       # round trip vs the input coefficients
+
+
+def _nc(b):
+    return (4 * b ** 3 - b) // 3
+
+
+def _coeffs(group, seed):
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    c = torch.randn(group.coeff_total, 2, generator=g, device="cuda", dtype=torch.float32)
+    return torch.view_as_complex(c.contiguous())
+
+
+def _single_path(bws, views, inverse):
+    """Per-bandwidth complex64 *_many results for each transform (caller's order)."""
+    from pysofft_amd import Soft
+    out = [None] * len(bws)
+    for b in sorted(set(bws)):
+        idx = [i for i, x in enumerate(bws) if x == b]
+        x = torch.stack([views[i] for i in idx])
+        s = Soft(b)
+        y = s.isoft_many_c64(x) if inverse else s.soft_many_c64(x)
+        for k, i in enumerate(idx):
+            out[i] = y[k]
+    return out
+
+
+BWS_SMALL = [8, 16, 6, 12, 8, 4, 16, 24, 2, 8, 3]
+
+
+def test_grouped_equals_single_bandwidth_path():
+    from pysofft_amd.grouped import SoftGroup
+    g = SoftGroup(BWS_SMALL)
+    c = _coeffs(g, 1)
+    f = g.isoft(c)
+    ref = _single_path(BWS_SMALL, g.coeff_views(c), True)
+    for i, (a, b) in enumerate(zip(g.grid_views(f), ref)):
+        assert torch.equal(a, b), (i, BWS_SMALL[i])
+    c2 = g.soft(f)
+    ref = _single_path(BWS_SMALL, g.grid_views(f), False)
+    for i, (a, b) in enumerate(zip(g.coeff_views(c2), ref)):
+        assert torch.equal(a, b), (i, BWS_SMALL[i])
+
+
+def test_grouped_every_launch_shape_same_result():
+    """Every tunable launch shape (transforms per wave, waves per block, one block per
+    item or a persistent grid looping over the work list) gives identical results."""
+    from pysofft_amd.grouped import SoftGroup
+    g = SoftGroup(BWS_SMALL)
+    g.config = dict(inv_tb=2, inv_grid=0, fwd_tb=2, fwd_nw=4, fwd_grid=0)
+    c = _coeffs(g, 2)
+    f0 = g.isoft(c).clone()
+    c0 = g.soft(f0).clone()
+    for inv_tb in (1, 2):
+        for (tb, nw) in ((2, 4), (1, 4), (1, 2), (1, 1)):
+            for grid in (0, 64):
+                g.config = dict(inv_tb=inv_tb, inv_grid=grid, fwd_tb=tb, fwd_nw=nw, fwd_grid=grid)
+                assert torch.equal(g.isoft(c), f0), (inv_tb, grid)
+                assert torch.equal(g.soft(f0), c0), (tb, nw, grid)
+    cfg = g.autotune()
+    assert cfg["tuned"] == 1 and cfg["inv_tb"] in (1, 2) and cfg["fwd_nw"] in (1, 2, 4)
+    assert torch.equal(g.isoft(c), f0) and torch.equal(g.soft(f0), c0)
+
+
+def test_grouped_graph_replays_match_eager():
+    """The captured graph, replayed several times with eager work and synchronisations
+    in between, reproduces the eager launches each time."""
+    from pysofft_amd.grouped import SoftGroup
+    g = SoftGroup(BWS_SMALL)
+    c = _coeffs(g, 3)
+    g.set_graphs(False)
+    fe = g.isoft(c).clone()
+    ce = g.soft(fe).clone()
+    g.set_graphs(True)
+    f = g.empty_grids()
+    c2 = g.empty_coeffs()
+    for r in range(4):
+        f.zero_()
+        c2.fill_(1.0)
+        torch.cuda.synchronize()
+        g.isoft(c, out=f)
+        g.soft(f, out=c2)
+        torch.cuda.synchronize()
+        assert torch.equal(f, fe), r
+        assert torch.equal(c2, ce), r
+
+
+def test_grouped_rejects_bad_inputs():
+    from pysofft_amd import _lib
+    from pysofft_amd.grouped import SoftGroup
+    with pytest.raises(_lib.So3ftError):
+        SoftGroup([32, 40])
+    g = SoftGroup([4, 8])
+    with pytest.raises(ValueError):
+        g.isoft(torch.zeros(5, dtype=torch.complex64, device="cuda"))
+    with pytest.raises(TypeError):
+        g.isoft(torch.zeros(g.coeff_total, dtype=torch.complex128, device="cuda"))
+
+
+def test_workload_ragged_256_vs_fp64():
+    """256 complex64 transforms, bandwidths drawn uniformly from {32, 48, 64, 96, 128},
+    coefficients iid complex N(0, 1): grouped isoft and soft against the FP64 path, per
+    transform, and the round trip."""
+    from pysofft_amd import Soft
+    from pysofft_amd.grouped import SoftGroup
+    rng = np.random.default_rng(256)
+    bws = [int(b) for b in rng.choice([32, 48, 64, 96, 128], 256)]
+    g = SoftGroup(bws)
+    c = _coeffs(g, 4)
+    f = g.isoft(c)
+    c2 = g.soft(f)
+    torch.cuda.synchronize()
+    plans = {b: Soft(b) for b in set(bws)}
+    worst = [0.0, 0.0, 0.0]
+    for i, b in enumerate(bws):
+        ci = g.coeff_views(c)[i]
+        fi = g.grid_views(f)[i]
+        f64 = plans[b].isoft(ci.to(torch.complex128))
+        e0 = (torch.linalg.vector_norm(fi.to(torch.complex128) - f64) / torch.linalg.vector_norm(f64)).item()
+        c64 = plans[b].soft(fi.to(torch.complex128))
+        c2i = g.coeff_views(c2)[i]
+        e1 = (torch.linalg.vector_norm(c2i.to(torch.complex128) - c64) / torch.linalg.vector_norm(c64)).item()
+        e2 = (torch.linalg.vector_norm(c2i - ci) / torch.linalg.vector_norm(ci)).item()
+        worst = [max(worst[0], e0), max(worst[1], e1), max(worst[2], e2)]
+        assert e0 <= TOL and e1 <= TOL and e2 <= TOL_RT, (i, b, e0, e1, e2)
+    print("ragged 256: max relL2 isoft %.2e soft %.2e round trip %.2e" % tuple(worst), flush=True)

@@ -215,6 +215,65 @@ def test_cross_correlation_3d_vs_oracle():
     assert rel_l2(got, P.cross_correlation_ylm_cmplx_3d(f, g, rad, (1, nr))) < 1e-12
 
 
+def _real_mlr(rng, bw, shape=()):
+    nr = bw * (bw + 1) // 2
+    f = rng.standard_normal(shape + (nr,)) + 1j * rng.standard_normal(shape + (nr,))
+    f[..., :bw] = f[..., :bw].real
+    return f
+
+
+@pytest.mark.parametrize("engine", ["valu", "mfma"])
+@pytest.mark.parametrize("bw", [1, 2, 5, 8, 13])
+def test_real_correlation_vs_oracle(bw, engine):
+    """cross_correlation_ylm_real (MLr inputs, float64 grid) and its DWT stage against
+    the oracle's loop-by-loop restatement of soft.f08:2806-2946, 3118-3131."""
+    rng = np.random.default_rng(100 + bw)
+    s = _soft(bw)
+    s.dwt_engine = engine
+    P = O.OraclePlan(bw)
+    f, g = _real_mlr(rng, bw), _real_mlr(rng, bw)
+    got = s.cross_correlation_ylm_real(f, g)
+    assert got.dtype == np.float64 and got.shape == tuple(P.grid_shape)
+    assert rel_l2(got, P.cross_correlation_ylm_real(f, g)) < 1e-12
+    st = s._inverse_wigner_trf_corr_real(f, g)
+    assert rel_l2(st, P.cross_correlation_ylm_real(f, g, stage=True)) < 1e-12
+    # the complex stage against the oracle's inverse DWT of its correlation coefficients
+    fc, gc = _rand_c(rng, bw * bw), _rand_c(rng, bw * bw)
+    st = s._inverse_wigner_trf_corr_cmplx(fc, gc)
+    assert rel_l2(st, P.inverse_dwt(O.corr_coeffs(bw, fc, gc))) < 1e-12
+
+
+def test_real_correlation_3d_vs_oracle():
+    bw, nr = 8, 5
+    rng = np.random.default_rng(78)
+    s = _soft(bw)
+    P = O.OraclePlan(bw)
+    f, g = _real_mlr(rng, bw, (nr,)), _real_mlr(rng, bw, (nr,))
+    rad = np.arange(nr).astype(float) + 0.25
+    got = s.cross_correlation_ylm_real_3d(f, g, radial_sampling_points=rad, radial_limits=[1, nr])
+    assert rel_l2(got, P.cross_correlation_ylm_real_3d(f, g, rad, (1, nr))) < 1e-12
+
+
+def test_real_correlation_peak_bw32():
+    """tests/test_fortran_backend.py:723-750: real f (rng.random MLr coefficients) and
+    its rotation by grid rotations (rotate_ylm_real); the peak of the real correlation is
+    the rotation's grid index."""
+    from pysofft_amd import Soft
+    bw = 32
+    rng = np.random.default_rng(32)
+    nr = bw * (bw + 1) // 2
+    f = rng.random(nr).astype(complex)
+    s = _soft(bw)
+    a = np.arange(2 * bw) * np.pi / bw
+    b = O.beta_samples(2 * bw)
+    for _ in range(5):
+        idx = rng.integers(0, 2 * bw, 3)
+        g = Soft.rotate_ylm_real(f[None], (a[idx[0]], b[idx[1]], a[idx[2]]))
+        cc = s.cross_correlation_ylm_real(f, np.asarray(g).reshape(-1)[:nr])
+        ig, ia, ib = np.unravel_index(np.argmax(np.abs(cc)), cc.shape)
+        assert [ia, ib, ig] == list(idx)
+
+
 def test_rotate_ylm_vs_oracle():
     bw = 12
     rng = np.random.default_rng(12)

@@ -73,3 +73,27 @@ def test_soft_fails_loudly_without_gpu():
     from pysofft_amd import Soft
     with pytest.raises(RuntimeError):
         Soft(8)
+
+
+def test_group_layout_host():
+    """Packed layout of a ragged batch: bandwidths in decreasing order, each bandwidth's
+    transforms contiguous in the caller's order; unsupported bandwidths rejected."""
+    from pysofft_amd.grouped import group_layout
+    bws = [32, 128, 48, 32, 96, 64, 128, 48]
+    co, go, ct, gt = group_layout(bws)
+    nc = lambda b: (4 * b ** 3 - b) // 3
+    order = sorted(range(len(bws)), key=lambda i: (-bws[i], i))
+    c = g = 0
+    for i in order:
+        assert co[i] == c and go[i] == g, i
+        c += nc(bws[i])
+        g += (2 * bws[i]) ** 3
+    assert (ct, gt) == (c, g)
+    L = _lib.lib()
+    b = np.array([32, 40], np.int64)
+    assert L.so3ft_group_layout(b.ctypes.data, 2, None, None, None, None) == -5
+    assert b"3*2^a" in L.so3ft_last_error()
+    b = np.array([1], np.int64)
+    assert L.so3ft_group_layout(b.ctypes.data, 1, None, None, None, None) == -5
+    assert L.so3ft_group_layout(b.ctypes.data, 0, None, None, None, None) == -1
+    assert L.so3ft_group_destroy(99999) == -2

@@ -223,6 +223,46 @@ def test_correlation_scale_at_true_rotation():
         assert abs(v - np.sum(np.abs(f) ** 2)) < 1e-11 * np.sum(np.abs(f) ** 2), (idx, v)
 
 
+def _real_mlr(rng, bw, shape=()):
+    """Random MLr coefficients of a real function (the m = 0 entries real)."""
+    nr = bw * (bw + 1) // 2
+    f = rng.standard_normal(shape + (nr,)) + 1j * rng.standard_normal(shape + (nr,))
+    f[..., :bw] = f[..., :bw].real
+    return f
+
+
+def test_real_correlation_restatement_vs_complex_correlation():
+    """The loop-by-loop restatement of the reference's half-complex real correlation
+    (soft.f08:2806-2946, 3118-3131) equals the complex correlation of the same functions
+    (LMc with f^l_{-m} = (-1)^m conj f^l_m), whose imaginary part vanishes; a wrong sign or
+    beta reversal in any of the four half-complex branches breaks the equality."""
+    for bw in (1, 2, 5, 8):
+        rng = np.random.default_rng(70 + bw)
+        f, g = _real_mlr(rng, bw), _real_mlr(rng, bw)
+        P = O.OraclePlan(bw)
+        cr = P.cross_correlation_ylm_real(f, g)
+        cc = P.cross_correlation_ylm_cmplx(O.mlr_to_lmc(f, bw), O.mlr_to_lmc(g, bw))
+        assert rel_l2(cr, cc.real) < 1e-12, bw
+        assert np.abs(cc.imag).max() < 1e-12 * np.abs(cc).max(), bw
+
+
+def test_real_correlation_peak_at_rotation():
+    """tests/test_fortran_backend.py:723-750 on the oracle: the real correlation of f and
+    its rotation by a grid rotation peaks at that rotation's grid index."""
+    bw = 8
+    P = O.OraclePlan(bw)
+    rng = np.random.default_rng(77)
+    f = _real_mlr(rng, bw)
+    a = np.arange(2 * bw) * np.pi / bw
+    b = O.beta_samples(2 * bw)
+    for idx in ([3, 5, 11], [14, 2, 7]):
+        gl = O.rotate_ylm_cmplx(O.mlr_to_lmc(f, bw), [a[idx[0]], b[idx[1]], a[idx[2]]])[0, 0]
+        g = np.array([gl[l * (l + 1) + m] for m in range(bw) for l in range(m, bw)])
+        cr = P.cross_correlation_ylm_real(f, g)
+        ig, ia, ib = np.unravel_index(np.argmax(np.abs(cr)), cr.shape)
+        assert [ia, ib, ig] == idx
+
+
 def test_mlp_oracle_gradients_vs_finite_differences():
     """The alignment MLP's FP64 oracle gradients (oracle/mlp_oracle.py) against central
     differences of its own loss on a tiny network: a dropped chain-rule term or a wrong
