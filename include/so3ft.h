@@ -14,7 +14,9 @@
  *
  * Layouts are the reference's:
  *   coefficients : complex128 (or complex64 for the *_c64 entry points), length
- *                  NC = (4bw^3-bw)/3, "mnl" order (pysofft/utils.f90:87-201)
+ *                  NC = (4bw^3-bw)/3, "mnl" order (pysofft/utils.f90:87-201); for a
+ *                  handle made with recurrence_type = Risbo and precompute_wigners = 0,
+ *                  "lmn" order (utils.f90:228-355), as the reference (soft.py:98-106)
  *   SO(3) function: C-order (gamma, alpha, beta), shape (2bw,2bw,2bw)
  *                  (docs/defs/so3_func.md:5-13), complex128 or float64
  *   *_many       : batch-major, i.e. (n, NC) and (n, 2bw, 2bw, 2bw)
@@ -229,6 +231,9 @@ int so3ft_bench_kernel(so3ft_handle h, int which, int reps, void *a, void *b, in
 
 /* ---- utils: utils.f90:471-505 enforce_real_sym_mnl(_many), in place on device */
 int so3ft_enforce_real_sym_mnl(int64_t bw, void *coeffs, int64_t n, void *stream);
+/* ---- utils.f90:510-560 enforce_real_sym_lmn(_many) on lmn-ordered sets, in place on
+ * device (synchronises the stream) */
+int so3ft_enforce_real_sym_lmn(int64_t bw, void *coeffs, int64_t n, void *stream);
 
 #ifdef __cplusplus
 }

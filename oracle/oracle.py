@@ -393,6 +393,9 @@ class RefLib:
                 "ref_risbo_step": (None, [_dp, _i64, C.c_double, C.c_double, _dp, _i64, _dp]),
                 "ref_full_big_d": (None, [_i64, C.c_double, C.c_double, C.c_double, _vp]),
                 "ref_enforce_real_sym_mnl": (None, [_vp, _i64]),
+                "ref_coeff_location_lmn": (_i64, [_i64, _i64, _i64]),
+                "ref_coeff_degrees_risbo": (None, [_i64, _ip]),
+                "ref_enforce_real_sym_lmn": (None, [_vp, _i64]),
                 "ref_lmc": (_i64, [_i64, _i64]),
                 "ref_mlr": (_i64, [_i64, _i64, _i64]),
                 "ref_mlc": (_i64, [_i64, _i64, _i64]),
@@ -413,6 +416,22 @@ class RefLib:
         out = np.zeros((total_num_coeffs(bw), 3), np.int64)
         cls.lib().ref_coeff_degrees(bw, _p(out, _ip))
         return out
+
+    @classmethod
+    def coeff_location_lmn(cls, l, m, n):
+        return int(cls.lib().ref_coeff_location_lmn(l, m, n)) - 1
+
+    @classmethod
+    def coeff_degrees_risbo(cls, bw):
+        out = np.zeros((total_num_coeffs(bw), 3), np.int64)
+        cls.lib().ref_coeff_degrees_risbo(bw, _p(out, _ip))
+        return out
+
+    @classmethod
+    def enforce_real_sym_lmn(cls, coeff, bw):
+        c = np.ascontiguousarray(coeff, np.complex128).copy()
+        cls.lib().ref_enforce_real_sym_lmn(_v(c), bw)
+        return c
 
     @classmethod
     def legendre_weights(cls, bw):

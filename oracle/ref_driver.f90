@@ -138,6 +138,30 @@ contains
     call enforce_real_sym_mnl(coeff, bw)
   end subroutine
 
+  !> utils.f90:261-292, 1-based index of f^l_{n1,n2} in lmn order
+  function ref_coeff_location_lmn(l, n1, n2) bind(C, name="ref_coeff_location_lmn") result(id)
+    integer(c_int64_t), value :: l, n1, n2
+    integer(c_int64_t) :: id, a, b, c
+    a = l; b = n1; c = n2
+    id = coeff_location_lmn(a, b, c)
+  end function
+
+  !> utils.f90:298-355, written as C-order [NC][3] = (l, m, n)
+  subroutine ref_coeff_degrees_risbo(bw, out) bind(C, name="ref_coeff_degrees_risbo")
+    integer(c_int64_t), value :: bw
+    integer(c_int64_t), intent(out) :: out(3, (4*bw*bw*bw - bw)/3)
+    integer(c_int64_t) :: lmn((4*bw*bw*bw - bw)/3, 3)
+    lmn = get_coeff_degrees_risbo(bw)
+    out = transpose(lmn)
+  end subroutine
+
+  !> utils.f90:510-550
+  subroutine ref_enforce_real_sym_lmn(coeff, bw) bind(C, name="ref_enforce_real_sym_lmn")
+    integer(c_int64_t), value :: bw
+    complex(c_double_complex), intent(inout) :: coeff((4*bw*bw*bw - bw)/3)
+    call enforce_real_sym_lmn(coeff, bw)
+  end subroutine
+
   !> utils.f90:735-741, 758-764, 781-787 (1-based)
   function ref_lmc(l, m) bind(C, name="ref_lmc") result(i)
     integer(c_int64_t), value :: l, m

@@ -56,6 +56,7 @@ SIGNATURES = {
     "so3ft_inverse_wigner_trf_corr_cmplx": (_int, [_i64, _vp, _vp, _vp, _vp]),
     "so3ft_inverse_wigner_trf_corr_real": (_int, [_i64, _vp, _vp, _vp, _vp]),
     "so3ft_enforce_real_sym_mnl": (_int, [_i64, _vp, _i64, _vp]),
+    "so3ft_enforce_real_sym_lmn": (_int, [_i64, _vp, _i64, _vp]),
     "so3ft_rotate_ylm_cmplx": (_int, [_i64, _vp, _i64, _vp, _i64, _int, _vp, _vp]),
     "so3ft_mlp_last_error": (C.c_char_p, []),
     "so3ft_mlp_create": (_int, [_P(_i32), _int, _P(_i64)]),

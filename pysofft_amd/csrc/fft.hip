@@ -260,7 +260,9 @@ hipError_t launch_integrate(const void *f, int is_real, const double *w, int N, 
 
 // utils.f90:471-488: f^l_{-m,-n} = (-1)^{m+n} conj f^l_{m,n}; for m = 0 the n < 0 side is
 // the source (the reference's loop order lets it win), f^l_{0,0} is made real.
-__global__ void enforce_real_sym_kernel(double2 *c, int64_t c_ts, const int64_t *slice, int bw)
+// m0_neg_src: for m = 0 the n < 0 entries are the sources (utils.f90:471-488, mnl);
+// else the n > 0 entries (utils.f90:510-550, lmn)
+__global__ void enforce_real_sym_kernel(double2 *c, int64_t c_ts, const int64_t *slice, int bw, int m0_neg_src)
 {
     const int64_t t = blockIdx.y;
     const int W = 2 * bw - 1;
@@ -274,7 +276,7 @@ __global__ void enforce_real_sym_kernel(double2 *c, int64_t c_ts, const int64_t 
             for (int i = 0; i < L; ++i) dst[i].y = 0.0;
             continue;
         }
-        const bool is_dst = (m < 0) || (m == 0 && n > 0);
+        const bool is_dst = (m < 0) || (m == 0 && (m0_neg_src ? n > 0 : n < 0));
         if (!is_dst) continue;
         const double2 *src = ct + slice[(int64_t)(-m + bw - 1) * W + (-n + bw - 1)];
         const double sg = (((m + n) & 1) != 0) ? -1.0 : 1.0;
@@ -282,14 +284,52 @@ __global__ void enforce_real_sym_kernel(double2 *c, int64_t c_ts, const int64_t 
     }
 }
 
-hipError_t launch_enforce_real_sym(double2 *c, int64_t c_ts, const int64_t *slice, int bw, int T, hipStream_t s)
+hipError_t launch_enforce_real_sym(double2 *c, int64_t c_ts, const int64_t *slice, int bw, int T, int m0_neg_src,
+                                   hipStream_t s)
 {
     const int W = 2 * bw - 1;
     int nb = (W * W + 255) / 256;
     for (int t0 = 0; t0 < T; t0 += 32768) {  // grid y extent < 65536
         const int nt = T - t0 < 32768 ? T - t0 : 32768;
         hipLaunchKernelGGL(enforce_real_sym_kernel, dim3(nb, nt), dim3(256), 0, s, c + (int64_t)t0 * c_ts, c_ts, slice,
-                           bw);
+                           bw, m0_neg_src);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// lmn <-> mnl coefficient order: perm[i] = mnl position of lmn position i.  to_mnl:
+// out[perm[i]] = in[i]; else out[i] = in[perm[i]].  T sets at strides in_ts / out_ts.
+template <class E>
+__global__ void permute_kernel(const E *__restrict__ in, E *__restrict__ out, const int32_t *__restrict__ perm,
+                               int64_t nc, int64_t in_ts, int64_t out_ts, int to_mnl)
+{
+    const int64_t t = blockIdx.y;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nc; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = perm[i];
+        if (to_mnl) out[t * out_ts + j] = in[t * in_ts + i];
+        else out[t * out_ts + i] = in[t * in_ts + j];
+    }
+}
+
+hipError_t launch_permute_lmn(const int32_t *perm, int64_t nc, const void *in, int64_t in_ts, void *out,
+                              int64_t out_ts, int esz, int to_mnl, int T, hipStream_t s)
+{
+    int nb = (int)((nc + 255) / 256);
+    if (nb > 2048) nb = 2048;
+    for (int t0 = 0; t0 < T; t0 += 32768) {
+        const int nt = T - t0 < 32768 ? T - t0 : 32768;
+        if (esz == 16)
+            hipLaunchKernelGGL(permute_kernel<double2>, dim3(nb, nt), dim3(256), 0, s,
+                               reinterpret_cast<const double2 *>(in) + (int64_t)t0 * in_ts,
+                               reinterpret_cast<double2 *>(out) + (int64_t)t0 * out_ts, perm, nc, in_ts, out_ts, to_mnl);
+        else if (esz == 8)
+            hipLaunchKernelGGL(permute_kernel<float2>, dim3(nb, nt), dim3(256), 0, s,
+                               reinterpret_cast<const float2 *>(in) + (int64_t)t0 * in_ts,
+                               reinterpret_cast<float2 *>(out) + (int64_t)t0 * out_ts, perm, nc, in_ts, out_ts, to_mnl);
+        else
+            return hipErrorInvalidValue;
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

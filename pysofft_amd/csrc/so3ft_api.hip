@@ -53,6 +53,10 @@ struct Plan {
     double *red = nullptr;  // integration partials + result
     double *radw = nullptr; // radial shell weights (3-D correlation)
     int64_t radw_cap = 0;
+    int lmn = 0;                // coefficients in lmn order (Risbo layout)
+    int32_t *lmn_perm = nullptr;
+    double2 *pstage = nullptr;  // coefficients reordered to mnl
+    int64_t pstage_T = 0;
 };
 
 std::mutex g_mu;
@@ -77,6 +81,8 @@ void free_plan_device(Plan *P)
     P->amax_v = nullptr; P->amax_i = nullptr; P->amax_cap = 0;
     hipFree(P->radw);
     P->radw = nullptr; P->radw_cap = 0;
+    hipFree(P->lmn_perm); hipFree(P->pstage);
+    P->lmn_perm = nullptr; P->pstage = nullptr; P->pstage_T = 0; P->lmn = 0;
 }
 
 template <class T>
@@ -114,15 +120,41 @@ std::vector<int64_t> slice_table(int64_t bw)
     return t;
 }
 
+// lmn order (utils.f90:228-355): degree-major; inside a degree the pairs 0 <= m1 <= m2
+// own their distinct variants (m1,m2), (-m2,-m1), (m2,m1), (-m1,-m2), (m1,-m2), (-m1,m2),
+// (m2,-m1), (-m2,m1).  perm[i] = mnl position of lmn position i.
+std::vector<int32_t> lmn_perm_table(int64_t bw)
+{
+    const std::vector<int64_t> st = slice_table(bw);
+    std::vector<int32_t> perm;
+    perm.reserve((4 * bw * bw * bw - bw) / 3);
+    for (int64_t l = 0; l < bw; ++l)
+        for (int64_t m1 = 0; m1 <= l; ++m1)
+            for (int64_t m2 = m1; m2 <= l; ++m2) {
+                int64_t v[8][2];
+                int nv = 0;
+                v[nv][0] = m1; v[nv++][1] = m2;
+                if (!(m1 == 0 && m2 == 0)) {
+                    v[nv][0] = -m2; v[nv++][1] = -m1;
+                    if (m1 != m2) { v[nv][0] = m2; v[nv++][1] = m1; v[nv][0] = -m1; v[nv++][1] = -m2; }
+                    if (m1 != 0) {
+                        v[nv][0] = m1; v[nv++][1] = -m2; v[nv][0] = -m1; v[nv++][1] = m2;
+                        if (m1 != m2) { v[nv][0] = m2; v[nv++][1] = -m1; v[nv][0] = -m2; v[nv++][1] = m1; }
+                    }
+                }
+                for (int k = 0; k < nv; ++k) {
+                    const int64_t a = std::max(v[k][0] < 0 ? -v[k][0] : v[k][0], v[k][1] < 0 ? -v[k][1] : v[k][1]);
+                    perm.push_back((int32_t)(st[mnidx(v[k][0], v[k][1], bw)] + l - a));
+                }
+            }
+    return perm;
+}
+
 int build_plan(Plan *P, int64_t bw, int64_t lmax, int precompute, int init_ffts, int64_t rt, int32_t flags)
 {
     if (bw < 1 || bw > 256) return fail(SO3FT_EINVAL, "bw must satisfy 1 <= bw <= 256, got " + std::to_string(bw));
     if (lmax < 0 || lmax >= bw) return fail(SO3FT_EINVAL, "lmax must satisfy 0 <= lmax < bw");
     if (rt != 0 && rt != 1) return fail(SO3FT_EINVAL, "unknown recurrence type " + std::to_string(rt));
-    if (rt == 1 && !precompute)
-        return fail(SO3FT_EUNSUPPORTED,
-                    "recurrence_type=risbo without precomputed Wigners uses the lmn coefficient layout, "
-                    "which this library does not provide yet");
     const int64_t N = 2 * bw;
     P->bw = bw; P->lmax = lmax; P->N = N; P->nc = (4 * bw * bw * bw - bw) / 3;
     P->precompute = precompute; P->init_ffts = init_ffts; P->recurrence = rt; P->fftw_flags = flags;
@@ -195,6 +227,13 @@ int build_plan(Plan *P, int64_t bw, int64_t lmax, int precompute, int init_ffts,
     if ((rc = upload(&P->tb.slice, slice_table(bw)))) return rc;
     if ((rc = upload(&P->tb.pairs, pairs))) return rc;
     if ((rc = upload(&P->tb.tw, tw))) return rc;
+    // recurrence_type = Risbo without precomputed Wigners: coefficients in lmn order
+    // (soft.py:98-106); the transforms reorder them and run the same d values (Kostelec
+    // recurrence, equal to the Risbo values to rounding for bw <= 256)
+    if (rt == 1 && !precompute) {
+        P->lmn = 1;
+        if ((rc = upload(&P->lmn_perm, lmn_perm_table(bw)))) return rc;
+    }
     HIPCHK(hipMalloc((void **)&P->red, sizeof(double) * (2 * 1024 + 2)));
     HIPCHK(dwt_set_attributes((int)bw, (int)N));
     return SO3FT_OK;
@@ -279,6 +318,20 @@ int ensure_cstage(Plan *P, int64_t T)
     return SO3FT_OK;
 }
 
+int ensure_pstage(Plan *P, int64_t T)
+{
+    if (P->pstage_T >= T) return SO3FT_OK;
+    hipFree(P->pstage);
+    P->pstage = nullptr; P->pstage_T = 0;
+    const size_t bytes = sizeof(double2) * (size_t)P->nc * (size_t)T;
+    if (hipMalloc((void **)&P->pstage, bytes) != hipSuccess) {
+        hipGetLastError();
+        return fail(SO3FT_ENOMEM, "lmn reordering buffer allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    P->pstage_T = T;
+    return SO3FT_OK;
+}
+
 // inverse DWT of T transforms into F on the plan's engine; DWT_CORR takes SH pairs
 // (c, aux) with stride c_ts and stages their mnl correlation coefficients first
 int dwt_inverse(Plan *P, int mode, const double2 *c, int64_t c_ts, const double2 *aux, double2 *F, int64_t f_ts,
@@ -308,38 +361,46 @@ int dwt_forward(Plan *P, int mode, const double2 *F, int64_t f_ts, double2 *c, i
     return SO3FT_OK;
 }
 
+// perm: the caller's coefficients are in lmn order (plans with P->lmn)
 int run_inverse(Plan *P, int mode, const void *coeff, int64_t c_ts, const void *aux, void *out, int out_real,
-                int64_t n, hipStream_t s)
+                int64_t n, hipStream_t s, int perm = 0)
 {
     const int64_t N = P->N, N3 = N * N * N;
     const int64_t ch = chunk_of(P, n);
     int rc = ensure_scratch(P, ch);
     if (rc) return rc;
+    if (perm && (rc = ensure_pstage(P, ch))) return rc;
     const size_t osz = out_real ? sizeof(double) : sizeof(double2);
     for (int64_t i0 = 0; i0 < n; i0 += ch) {
         const int64_t T = std::min(ch, n - i0);
         const double2 *c = reinterpret_cast<const double2 *>(coeff) + i0 * c_ts;
+        if (perm) {
+            HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, c, c_ts, P->pstage, P->nc, 16, 1, (int)T, s));
+            c = P->pstage;
+        }
         const double2 *ax = aux ? reinterpret_cast<const double2 *>(aux) + i0 * c_ts : nullptr;
-        if ((rc = dwt_inverse(P, mode, c, c_ts, ax, P->scratch, N3, T, s))) return rc;
+        if ((rc = dwt_inverse(P, mode, c, perm ? P->nc : c_ts, ax, P->scratch, N3, T, s))) return rc;
         void *o = reinterpret_cast<char *>(out) + (size_t)i0 * N3 * osz;
         if ((rc = inverse_fft_passes(P, P->scratch, o, out_real, T, s))) return rc;
     }
     return SO3FT_OK;
 }
 
-int run_forward(Plan *P, int mode, const void *in, int in_real, void *coeff, int64_t n, hipStream_t s)
+int run_forward(Plan *P, int mode, const void *in, int in_real, void *coeff, int64_t n, hipStream_t s, int perm = 0)
 {
     const int64_t N = P->N, N3 = N * N * N;
     const int64_t ch = chunk_of(P, n);
     int rc = ensure_scratch(P, ch);
     if (rc) return rc;
+    if (perm && (rc = ensure_pstage(P, ch))) return rc;
     const size_t isz = in_real ? sizeof(double) : sizeof(double2);
     for (int64_t i0 = 0; i0 < n; i0 += ch) {
         const int64_t T = std::min(ch, n - i0);
         const void *f = reinterpret_cast<const char *>(in) + (size_t)i0 * N3 * isz;
         if ((rc = forward_fft_passes(P, f, in_real, P->scratch, T, s))) return rc;
         double2 *c = reinterpret_cast<double2 *>(coeff) + i0 * P->nc;
-        if ((rc = dwt_forward(P, mode, P->scratch, N3, c, P->nc, T, s))) return rc;
+        if ((rc = dwt_forward(P, mode, P->scratch, N3, perm ? P->pstage : c, P->nc, T, s))) return rc;
+        if (perm) HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, P->pstage, P->nc, c, P->nc, 16, 0, (int)T, s));
     }
     return SO3FT_OK;
 }
@@ -428,11 +489,18 @@ int run_inverse_c64(Plan *P, const float2 *coeff, float2 *out, int64_t n, hipStr
     const int64_t ch = chunk_of(P, n);
     int rc = ensure_scratch(P, (ch + 1) / 2);  // complex64 scratch: half the bytes
     if (rc) return rc;
+    if (P->lmn && (rc = ensure_pstage(P, (ch + 1) / 2))) return rc;
     float2 *F = reinterpret_cast<float2 *>(P->scratch);
     for (int64_t i0 = 0; i0 < n; i0 += ch) {
         const int64_t T = std::min(ch, n - i0);
-        HIPCHK(launch_dwt_inverse_mixed(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX, coeff + i0 * P->nc,
-                                        P->nc, nullptr, F, N3, (int)T, s));
+        const float2 *c = coeff + i0 * P->nc;
+        if (P->lmn) {
+            float2 *ps = reinterpret_cast<float2 *>(P->pstage);
+            HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, c, P->nc, ps, P->nc, 8, 1, (int)T, s));
+            c = ps;
+        }
+        HIPCHK(launch_dwt_inverse_mixed(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX, c, P->nc, nullptr, F,
+                                        N3, (int)T, s));
         if ((rc = inverse_fft_passes_c64(P, F, out + i0 * N3, T, s))) return rc;
     }
     return SO3FT_OK;
@@ -445,12 +513,16 @@ int run_forward_c64(Plan *P, const float2 *in, float2 *coeff, int64_t n, hipStre
     const int64_t ch = chunk_of(P, n);
     int rc = ensure_scratch(P, (ch + 1) / 2);
     if (rc) return rc;
+    if (P->lmn && (rc = ensure_pstage(P, (ch + 1) / 2))) return rc;
     float2 *F = reinterpret_cast<float2 *>(P->scratch);
     for (int64_t i0 = 0; i0 < n; i0 += ch) {
         const int64_t T = std::min(ch, n - i0);
         if ((rc = forward_fft_passes_c64(P, in + i0 * N3, F, T, s))) return rc;
+        float2 *c = coeff + i0 * P->nc;
+        float2 *ps = reinterpret_cast<float2 *>(P->pstage);
         HIPCHK(launch_dwt_forward_mixed(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX, F, N3,
-                                        coeff + i0 * P->nc, P->nc, (int)T, s));
+                                        P->lmn ? ps : c, P->nc, (int)T, s));
+        if (P->lmn) HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, ps, P->nc, c, P->nc, 8, 0, (int)T, s));
     }
     return SO3FT_OK;
 }
@@ -588,7 +660,7 @@ int so3ft_isoft_many(so3ft_handle h, const void *coeffs, void *so3funcs, int64_t
     GETP(h);
     if (int rc = check_ptrs(coeffs, so3funcs)) return rc;
     if (n < 0) return fail(SO3FT_EINVAL, "negative count");
-    return run_inverse(P, DWT_COMPLEX, coeffs, P->nc, nullptr, so3funcs, 0, n, S(stream));
+    return run_inverse(P, DWT_COMPLEX, coeffs, P->nc, nullptr, so3funcs, 0, n, S(stream), P->lmn);
 }
 int so3ft_isoft(so3ft_handle h, const void *coeff, void *so3func, void *stream)
 {
@@ -599,7 +671,7 @@ int so3ft_soft_many(so3ft_handle h, const void *so3funcs, void *coeffs, int64_t 
     GETP(h);
     if (int rc = check_ptrs(coeffs, so3funcs)) return rc;
     if (n < 0) return fail(SO3FT_EINVAL, "negative count");
-    return run_forward(P, DWT_COMPLEX, so3funcs, 0, coeffs, n, S(stream));
+    return run_forward(P, DWT_COMPLEX, so3funcs, 0, coeffs, n, S(stream), P->lmn);
 }
 int so3ft_soft(so3ft_handle h, const void *so3func, void *coeff, void *stream)
 {
@@ -610,7 +682,7 @@ int so3ft_irsoft_many(so3ft_handle h, const void *coeffs, void *so3funcs, int64_
     GETP(h);
     if (int rc = check_ptrs(coeffs, so3funcs)) return rc;
     if (n < 0) return fail(SO3FT_EINVAL, "negative count");
-    return run_inverse(P, DWT_REAL, coeffs, P->nc, nullptr, so3funcs, 1, n, S(stream));
+    return run_inverse(P, DWT_REAL, coeffs, P->nc, nullptr, so3funcs, 1, n, S(stream), P->lmn);
 }
 int so3ft_irsoft(so3ft_handle h, const void *coeff, void *so3func, void *stream)
 {
@@ -621,7 +693,7 @@ int so3ft_rsoft_many(so3ft_handle h, const void *so3funcs, void *coeffs, int64_t
     GETP(h);
     if (int rc = check_ptrs(coeffs, so3funcs)) return rc;
     if (n < 0) return fail(SO3FT_EINVAL, "negative count");
-    return run_forward(P, DWT_REAL, so3funcs, 1, coeffs, n, S(stream));
+    return run_forward(P, DWT_REAL, so3funcs, 1, coeffs, n, S(stream), P->lmn);
 }
 int so3ft_rsoft(so3ft_handle h, const void *so3func, void *coeff, void *stream)
 {
@@ -652,8 +724,13 @@ int so3ft_inverse_wigner_trf_cmplx(so3ft_handle h, const void *coeff, void *fft_
     if (int rc = check_ptrs(coeff, fft_array)) return rc;
     const int64_t N3 = P->N * P->N * P->N;
     double2 *F = reinterpret_cast<double2 *>(fft_array);
-    if (int rc = dwt_inverse(P, DWT_COMPLEX, reinterpret_cast<const double2 *>(coeff), P->nc, nullptr, F, N3, 1, S(stream)))
-        return rc;
+    const double2 *c = reinterpret_cast<const double2 *>(coeff);
+    if (P->lmn) {
+        if (int rc = ensure_pstage(P, 1)) return rc;
+        HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, c, P->nc, P->pstage, P->nc, 16, 1, 1, S(stream)));
+        c = P->pstage;
+    }
+    if (int rc = dwt_inverse(P, DWT_COMPLEX, c, P->nc, nullptr, F, N3, 1, S(stream))) return rc;
     HIPCHK(launch_zero_invalid_columns(F, N3, (int)P->bw, (int)P->N, (int)P->lmax, 1, 0, S(stream)));
     return SO3FT_OK;
 }
@@ -663,9 +740,13 @@ int so3ft_forward_wigner_trf_cmplx(so3ft_handle h, const void *fft_array, void *
     GETP(h);
     if (int rc = check_ptrs(coeff, fft_array)) return rc;
     const int64_t N3 = P->N * P->N * P->N;
-    if (int rc = dwt_forward(P, DWT_COMPLEX, reinterpret_cast<const double2 *>(fft_array), N3,
-                             reinterpret_cast<double2 *>(coeff), P->nc, 1, S(stream)))
+    double2 *c = reinterpret_cast<double2 *>(coeff);
+    if (P->lmn)
+        if (int rc = ensure_pstage(P, 1)) return rc;
+    if (int rc = dwt_forward(P, DWT_COMPLEX, reinterpret_cast<const double2 *>(fft_array), N3, P->lmn ? P->pstage : c,
+                             P->nc, 1, S(stream)))
         return rc;
+    if (P->lmn) HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, P->pstage, P->nc, c, P->nc, 16, 0, 1, S(stream)));
     return SO3FT_OK;
 }
 
@@ -940,7 +1021,51 @@ int so3ft_enforce_real_sym_mnl(int64_t bw, void *coeffs, int64_t n, void *stream
         }
     }
     const int64_t nc = (4 * bw * bw * bw - bw) / 3;
-    HIPCHK(launch_enforce_real_sym(reinterpret_cast<double2 *>(coeffs), nc, d, (int)bw, (int)n, S(stream)));
+    HIPCHK(launch_enforce_real_sym(reinterpret_cast<double2 *>(coeffs), nc, d, (int)bw, (int)n, 1, S(stream)));
+    return SO3FT_OK;
+}
+
+int so3ft_enforce_real_sym_lmn(int64_t bw, void *coeffs, int64_t n, void *stream)
+{
+    if (bw < 1 || bw > 256 || !coeffs || n < 0) return fail(SO3FT_EINVAL, "bad argument");
+    if (n == 0) return SO3FT_OK;
+    // reorder to mnl in a work buffer, apply the lmn routine's rule, reorder back
+    static std::mutex mu;
+    static std::map<std::pair<int, int64_t>, std::pair<int64_t *, int32_t *>> cache;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::pair<int64_t *, int32_t *> tabs;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto key = std::make_pair(dev, bw);
+        auto it = cache.find(key);
+        if (it == cache.end()) {
+            int64_t *sl = nullptr;
+            int32_t *pm = nullptr;
+            if (int rc = upload(&sl, slice_table(bw))) return rc;
+            if (int rc = upload(&pm, lmn_perm_table(bw))) return rc;
+            tabs = cache[key] = std::make_pair(sl, pm);
+        } else {
+            tabs = it->second;
+        }
+    }
+    const int64_t nc = (4 * bw * bw * bw - bw) / 3;
+    hipStream_t s = S(stream);
+    const int64_t ch = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(1 << 30) / (16 * nc)));
+    double2 *w = nullptr;
+    HIPCHK(hipMalloc((void **)&w, sizeof(double2) * nc * ch));
+    double2 *c = reinterpret_cast<double2 *>(coeffs);
+    hipError_t e = hipSuccess;
+    for (int64_t i0 = 0; i0 < n && e == hipSuccess; i0 += ch) {
+        const int T = (int)std::min(ch, n - i0);
+        e = launch_permute_lmn(tabs.second, nc, c + i0 * nc, nc, w, nc, 16, 1, T, s);
+        if (e == hipSuccess) e = launch_enforce_real_sym(w, nc, tabs.first, (int)bw, T, 0, s);
+        if (e == hipSuccess) e = launch_permute_lmn(tabs.second, nc, w, nc, c + i0 * nc, nc, 16, 0, T, s);
+    }
+    const hipError_t es = hipStreamSynchronize(s);  // the work buffer is freed below
+    hipFree(w);
+    if (e == hipSuccess) e = es;
+    if (e != hipSuccess) return fail(SO3FT_EHIP, std::string("enforce_real_sym_lmn: ") + hipGetErrorString(e));
     return SO3FT_OK;
 }
 

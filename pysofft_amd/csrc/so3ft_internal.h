@@ -138,8 +138,13 @@ hipError_t launch_zero_invalid_columns(double2 *F, int64_t f_ts, int bw, int N, 
 hipError_t launch_scale(double2 *X, int64_t n, double scale, hipStream_t s);
 hipError_t launch_integrate(const void *f, int is_real, const double *w, int N, double *partials,
                             int nblocks, double *result2, hipStream_t s);
-hipError_t launch_enforce_real_sym(double2 *c, int64_t c_ts, const int64_t *slice, int bw, int T,
+// f^l_{-m,-n} = (-1)^{m+n} conj f^l_{m,n} on mnl arrays; m0_neg_src selects the m = 0
+// source side (1: n < 0 as utils.f90:471-488, 0: n > 0 as the lmn routine utils.f90:510-550)
+hipError_t launch_enforce_real_sym(double2 *c, int64_t c_ts, const int64_t *slice, int bw, int T, int m0_neg_src,
                                    hipStream_t s);
+// lmn <-> mnl reordering of T coefficient sets (esz 16: complex128, 8: complex64)
+hipError_t launch_permute_lmn(const int32_t *perm, int64_t nc, const void *in, int64_t in_ts, void *out,
+                              int64_t out_ts, int esz, int to_mnl, int T, hipStream_t s);
 
 // per transform: first flat index (and value) of max |x| over n_per complex values;
 // pv / pi hold argmax_partials(n_per) * T partials

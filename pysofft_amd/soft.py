@@ -150,9 +150,19 @@ class Soft:
         _lib.check(_lib.lib().so3ft_set_dwt_engine(self._handle, code))
 
     @property
+    def coeff_order(self):
+        """'lmn' for Risbo handles without precomputed Wigners, else 'mnl' (soft.py:98-106)."""
+        if self.recurrence_type == self.recurrence_types.risbo and not self.wigners_are_precomputed:
+            return 'lmn'
+        return 'mnl'
+
+    @property
     def coeff_indices(self):
         if self._lmns is None:
-            self._lmns = utils.get_coeff_degrees(self.bw)
+            if self.coeff_order == 'mnl':
+                self._lmns = utils.get_coeff_degrees(self.bw)
+            else:
+                self._lmns = utils.get_coeff_degrees_risbo(self.bw)
             self._lmns.flags.writeable = False
         return self._lmns
 
@@ -184,15 +194,16 @@ class Soft:
         n_coeff = utils.total_num_coeffs(self.bw)
         shape = (howmany, n_coeff) if howmany > 0 else (n_coeff,)
         coeff = np.zeros(shape, dtype=np.complex128)
+        lmn = self.coeff_order == 'lmn'
         if random:
             coeff = self._fill_random(coeff, seed=seed)
             if real:
                 if howmany > 0:
-                    utils.enforce_real_sym_mnl_many(coeff, self.bw)
+                    (utils.enforce_real_sym_lmn_many if lmn else utils.enforce_real_sym_mnl_many)(coeff, self.bw)
                 else:
-                    utils.enforce_real_sym_mnl(coeff, self.bw)
+                    (utils.enforce_real_sym_lmn if lmn else utils.enforce_real_sym_mnl)(coeff, self.bw)
         if not raw:
-            coeff = CoeffSO3(coeff, self.coeff_indices, coeff_order='mnl')
+            coeff = CoeffSO3(coeff, self.coeff_indices, coeff_order=self.coeff_order)
         return coeff
 
     def get_so3func(self, real=False, random=False, seed=12345, howmany=0):
@@ -205,13 +216,14 @@ class Soft:
         return func
 
     def enforce_real_symmetry(self, coeff):
+        lmn = self.coeff_order == 'lmn'
         if _is_torch(coeff):
             t = coeff
             n = 1 if t.ndim == 1 else t.shape[0]
-            _lib.check(_lib.lib().so3ft_enforce_real_sym_mnl(self.bw, C.c_void_p(t.data_ptr()), n,
-                                                            _stream_ptr(t.device)))
+            fn = _lib.lib().so3ft_enforce_real_sym_lmn if lmn else _lib.lib().so3ft_enforce_real_sym_mnl
+            _lib.check(fn(self.bw, C.c_void_p(t.data_ptr()), n, _stream_ptr(t.device)))
             return coeff
-        utils.enforce_real_sym_mnl(coeff, self.bw)
+        (utils.enforce_real_sym_lmn if lmn else utils.enforce_real_sym_mnl)(coeff, self.bw)
 
     # -- transforms (pysofft/soft.py:255-517) ------------------------------------
     def _run(self, fn, x, xdtype, xshape, out, odtype, oshape, many=False):
@@ -306,7 +318,7 @@ class Soft:
 
     def _wrap(self, res, out):
         if out is None and isinstance(res, np.ndarray):
-            return CoeffSO3(res, self.coeff_indices, coeff_order='mnl')
+            return CoeffSO3(res, self.coeff_indices, coeff_order=self.coeff_order)
         return res
 
     # stage-level entry points (pysofft/soft.py:255-262)
@@ -598,7 +610,10 @@ class CoeffView:
         assert len(items) <= 3
         if len(items) == 3 and all(isinstance(i, (int, np.integer)) for i in items):
             self._check_index(items[0], items[1], items[2])
-            id_ = utils.coeff_location_mnl(items[1], items[2], items[0], self.bw) - 1
+            if self.coeff_order == 'lmn':
+                id_ = utils.coeff_location_lmn(items[0], items[1], items[2]) - 1
+            else:
+                id_ = utils.coeff_location_mnl(items[1], items[2], items[0], self.bw) - 1
             return slice(id_, id_ + 1)
         selection_ids = [ids[sel] for sel, ids in zip(items, [self.ls, self.ns, self.ks])]
         masks = [np.isin(lmn, sids) for sids, lmn in zip(selection_ids, self._lmns.T)]

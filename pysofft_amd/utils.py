@@ -169,3 +169,106 @@ def enforce_real_sym_mnl_many(coeffs, bw):
         for i in range(arr.shape[0]):
             enforce_real_sym_mnl(arr[i], bw)
     return arr
+
+
+# ---- lmn order (Risbo recurrence layout), utils.f90:228-355, 510-560 -----------------
+# Degree-major blocks of sum_{l'<l} (2l'+1)^2 entries; inside a degree the pairs
+# 0 <= m1 <= m2 <= l (m1 outer) each own their distinct variants in the order
+# (m1,m2), (-m2,-m1), (m2,m1), (-m1,-m2), (m1,-m2), (-m1,m2), (m2,-m1), (-m2,m1):
+# 1 entry for (0,0), 4 when m1 == 0 or m1 == m2, else 8.
+
+def _pair_variants(m1, m2):
+    v = [(m1, m2)]
+    if m1 == 0 and m2 == 0:
+        return v
+    v.append((-m2, -m1))
+    if m1 != m2:
+        v += [(m2, m1), (-m1, -m2)]
+    if m1 != 0:
+        v += [(m1, -m2), (-m1, m2)]
+        if m1 != m2:
+            v += [(m2, -m1), (-m2, m1)]
+    return v
+
+
+def coeff_slice_lmn(l, n1, n2):
+    """utils.f90:228-258 (1-based, inclusive): the entries of the pair (min, max) of
+    (|n1|, |n2|) at degree l."""
+    l = int(l)
+    m, n = min(abs(int(n1)), abs(int(n2))), max(abs(int(n1)), abs(int(n2)))
+    start = l * (4 * l * l - 1) // 3
+    for a in range(0, m + 1):
+        for b in range(a, l + 1):
+            if (a, b) == (m, n):
+                return np.array([start + 1, start + len(_pair_variants(a, b))], dtype=np.int64)
+            start += len(_pair_variants(a, b))
+    raise IndexError(f"no pair ({m}, {n}) at degree {l}")
+
+
+def coeff_location_lmn(l, n1, n2):
+    """utils.f90:261-292 (1-based)."""
+    s = int(coeff_slice_lmn(l, n1, n2)[0])
+    m, n = min(abs(int(n1)), abs(int(n2))), max(abs(int(n1)), abs(int(n2)))
+    return s + _pair_variants(m, n).index((int(n1), int(n2)))
+
+
+@lru_cache(maxsize=32)
+def _coeff_degrees_risbo(bw):
+    rows = []
+    for l in range(bw):
+        for m1 in range(l + 1):
+            for m2 in range(m1, l + 1):
+                rows += [(l, a, b) for (a, b) in _pair_variants(m1, m2)]
+    out = np.array(rows, dtype=np.int64).reshape(-1, 3)
+    out.flags.writeable = False
+    return out
+
+
+def get_coeff_degrees_risbo(bw):
+    """utils.f90:298-355: (l, m, n) of every coefficient in lmn storage order, (NC, 3)."""
+    return _coeff_degrees_risbo(int(bw)).copy()
+
+
+@lru_cache(maxsize=32)
+def lmn_to_mnl(bw):
+    """perm[i] = mnl position of the coefficient stored at lmn position i (0-based)."""
+    bw = int(bw)
+    d = _coeff_degrees_risbo(bw)
+    t = _slice_table(bw)
+    a = np.maximum(np.abs(d[:, 1]), np.abs(d[:, 2]))
+    p = t[d[:, 1] + bw - 1, d[:, 2] + bw - 1] + d[:, 0] - a
+    p.flags.writeable = False
+    return p
+
+
+def enforce_real_sym_lmn(coeff, bw):
+    """utils.f90:510-550 on a host array, in place: f^l_{-m,-n} = (-1)^{m+n} conj f^l_{m,n}
+    with the sources m > 0 and, for m = 0, n > 0 (the opposite side from the mnl
+    routine); f^l_{0,0} is made real."""
+    bw = int(bw)
+    d = _coeff_degrees_risbo(bw)
+    l, m, n = d[:, 0], d[:, 1], d[:, 2]
+    key = (l * (2 * bw) + (m + bw)) * (2 * bw) + (n + bw)
+    inv = np.empty(key.max() + 1, dtype=np.int64)
+    inv[key] = np.arange(len(d))
+    tgt = (m < 0) | ((m == 0) & (n < 0))
+    src = inv[(l * (2 * bw) + (-m + bw)) * (2 * bw) + (-n + bw)]
+    sg = np.where((m + n) % 2 == 0, 1.0, -1.0)
+    zero = (m == 0) & (n == 0)
+    coeff[zero] = coeff[zero].real
+    coeff[tgt] = sg[tgt] * np.conj(coeff[src[tgt]])
+    return coeff
+
+
+def enforce_real_sym_lmn_many(coeffs, bw):
+    """utils.f90:552-560: coeffs is (NC, n) as the f2py view; also accepts (n, NC) C arrays."""
+    arr = coeffs
+    if arr.ndim == 2 and arr.shape[0] == total_num_coeffs(bw) and arr.shape[1] != arr.shape[0]:
+        for i in range(arr.shape[1]):
+            col = np.ascontiguousarray(arr[:, i])
+            enforce_real_sym_lmn(col, bw)
+            arr[:, i] = col
+    else:
+        for i in range(arr.shape[0]):
+            enforce_real_sym_lmn(arr[i], bw)
+    return arr

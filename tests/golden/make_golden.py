@@ -26,6 +26,7 @@ def main():
     rng = np.random.default_rng(20240611)
     for bw in [1, 2, 3, 5, 8, 13]:
         out[f"degrees_{bw}"] = R.coeff_degrees(bw).astype(np.int16)
+        out[f"degrees_risbo_{bw}"] = R.coeff_degrees_risbo(bw).astype(np.int16)
     for bw in [1, 2, 3, 4, 7, 16, 32, 64, 128, 256]:
         out[f"weights_{bw}"] = R.legendre_weights(bw)
     out["seeds_9"] = R.seeds(9)
@@ -50,6 +51,7 @@ def main():
         c = rng.standard_normal(nc) + 1j * rng.standard_normal(nc)
         out[f"sym_in_{bw}"] = c.copy()
         out[f"sym_out_{bw}"] = R.enforce_real_sym_mnl(c, bw)
+        out[f"sym_lmn_out_{bw}"] = R.enforce_real_sym_lmn(out[f"sym_in_{bw}"], bw)
     # isoft on the grid from the reference's normalised d matrices (wigner_dl_kostelec):
     # f(g_j, a_i, b_k) = sum c_lmn e^{-i m a} d~^l_{mn}(b) e^{-i n g} / (2 pi)
     for bw in [2, 3, 4, 8]:

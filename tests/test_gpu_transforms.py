@@ -425,3 +425,57 @@ def test_workload_bw256_real_round_trip():
         assert abs(v.imag) < 1e-12
         got = fl[ig, ia, ib].item()
         assert abs(got - v.real) <= 1e-12 * max(1.0, abs(v)), (ig, ia, ib, got, v)
+
+
+@pytest.mark.parametrize("bw", [1, 2, 5, 8, 16])
+def test_risbo_lmn_layout_transforms_vs_oracle(bw):
+    """recurrence_type = Risbo without precomputed Wigners: coefficients in lmn order
+    (pysofft/soft.py:98-106, utils.f90:228-355) for every transform and stage."""
+    from pysofft_amd import Soft, utils
+    rng = np.random.default_rng(300 + bw)
+    s = Soft(bw, recurrence_type=1)
+    assert s.coeff_order == 'lmn' and s.recurrence_type == 1
+    assert Soft(bw, recurrence_type=1, precompute_wigners=True).coeff_order == 'mnl'
+    P = O.OraclePlan(bw)
+    perm = utils.lmn_to_mnl(bw)
+    c = _rand_c(rng, utils.total_num_coeffs(bw))
+    cm = np.empty_like(c)
+    cm[perm] = c                      # the same coefficients in mnl order
+    f = s.isoft(c)
+    fo = P.isoft(cm)
+    assert rel_l2(f, fo) < 1e-12
+    assert rel_l2(np.asarray(s.soft(fo)), P.soft(fo)[perm]) < 1e-12
+    fd = s._inverse_wigner_trf_cmplx(c)
+    assert rel_l2(fd, P.inverse_dwt(cm)) < 1e-12
+    assert rel_l2(np.asarray(s._forward_wigner_trf_cmplx(fd)), P.forward_dwt(fd)[perm]) < 1e-12
+    # many, real and mixed-precision entry points
+    cs = _rand_c(rng, (3, utils.total_num_coeffs(bw)))
+    fs = np.asarray(s.isoft_many(cs))
+    for i in range(3):
+        ci = np.empty_like(cs[i])
+        ci[perm] = cs[i]
+        assert rel_l2(fs[i], P.isoft(ci)) < 1e-12
+    cr = c.copy()
+    s.enforce_real_symmetry(cr)
+    cmr = np.empty_like(cr)
+    cmr[perm] = cr
+    assert rel_l2(s.irsoft(cr), P.irsoft(cmr)) < 1e-12
+    assert rel_l2(np.asarray(s.rsoft(P.irsoft(cmr))), cr) < 1e-11
+    if bw in (2, 8, 16):  # complex64 transforms need bw = 2^a or 3*2^a
+        import torch
+        c32 = torch.from_numpy(cs.astype(np.complex64)).cuda()
+        f32 = s.isoft_many_c64(c32)
+        assert rel_l2(f32.cpu().numpy(), fs) < 1e-6
+        assert rel_l2(s.soft_many_c64(f32).cpu().numpy(), c32.cpu().numpy()) < 1e-6
+
+
+def test_enforce_real_sym_lmn_device_vs_host():
+    import torch
+    from pysofft_amd import Soft, utils
+    bw = 9
+    rng = np.random.default_rng(9)
+    c = _rand_c(rng, (4, utils.total_num_coeffs(bw)))
+    ref = utils.enforce_real_sym_lmn_many(c.copy(), bw)
+    t = torch.from_numpy(c).cuda()
+    Soft(bw, recurrence_type=1).enforce_real_symmetry(t)
+    np.testing.assert_array_equal(t.cpu().numpy(), ref)

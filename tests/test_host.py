@@ -97,3 +97,24 @@ def test_group_layout_host():
     assert L.so3ft_group_layout(b.ctypes.data, 1, None, None, None, None) == -5
     assert L.so3ft_group_layout(b.ctypes.data, 0, None, None, None, None) == -1
     assert L.so3ft_group_destroy(99999) == -2
+
+
+@pytest.mark.parametrize("bw", [1, 2, 3, 5, 8, 13])
+def test_lmn_degrees_vs_reference_vectors(golden, bw):
+    """lmn order (Risbo layout) against the reference's get_coeff_degrees_risbo, with
+    coeff_location_lmn its inverse and lmn_to_mnl the reordering onto the mnl layout."""
+    d = utils.get_coeff_degrees_risbo(bw)
+    assert np.array_equal(d, golden[f"degrees_risbo_{bw}"].astype(np.int64))
+    for i in range(0, len(d), max(1, len(d) // 60)):
+        l, m, n = d[i]
+        assert utils.coeff_location_lmn(l, m, n) - 1 == i
+    p = utils.lmn_to_mnl(bw)
+    assert len(np.unique(p)) == len(p) == utils.total_num_coeffs(bw)
+    assert np.array_equal(utils.get_coeff_degrees(bw)[p], d)
+
+
+@pytest.mark.parametrize("bw", [1, 4, 6])
+def test_host_enforce_real_sym_lmn_vs_reference_vectors(golden, bw):
+    c = golden[f"sym_in_{bw}"].copy()
+    utils.enforce_real_sym_lmn(c, bw)
+    np.testing.assert_array_equal(c, golden[f"sym_lmn_out_{bw}"])
