@@ -132,6 +132,11 @@ int so3ft_forward_wigner_trf_cmplx(so3ft_handle h, const void *fft_array, void *
  * beta slice (FFTW_FORWARD / FFTW_BACKWARD sign), scaled by 1/(2bw) ---------- */
 int so3ft_fft(so3ft_handle h, const void *f1, void *f2, void *stream);
 int so3ft_ifft(so3ft_handle h, const void *f1, void *f2, void *stream);
+/* ---- py_wrap.f90:437-456 py_rfft / py_irfft: r2c of a float64 (2bw)^3 array into the
+ * half-complex complex128 (2bw, bw+1, 2bw) (C-order (gamma, alpha, beta), alpha halved)
+ * and c2r back (FFTW_BACKWARD), both scaled by 1/(2bw) */
+int so3ft_rfft(so3ft_handle h, const void *f1, void *f2, void *stream);
+int so3ft_irfft(so3ft_handle h, const void *f1, void *f2, void *stream);
 
 /* ---- py_wrap.f90:325-344 integrate_over_so3 (synchronises the stream) ----- */
 int so3ft_integrate_over_so3_cmplx(so3ft_handle h, const void *f, double *re, double *im, void *stream);

@@ -45,6 +45,8 @@ SIGNATURES = {
     "so3ft_forward_wigner_trf_cmplx": (_int, [_i64, _vp, _vp, _vp]),
     "so3ft_fft": (_int, [_i64, _vp, _vp, _vp]),
     "so3ft_ifft": (_int, [_i64, _vp, _vp, _vp]),
+    "so3ft_rfft": (_int, [_i64, _vp, _vp, _vp]),
+    "so3ft_irfft": (_int, [_i64, _vp, _vp, _vp]),
     "so3ft_integrate_over_so3_cmplx": (_int, [_i64, _vp, _P(_dbl), _P(_dbl), _vp]),
     "so3ft_integrate_over_so3_real": (_int, [_i64, _vp, _P(_dbl), _vp]),
     "so3ft_cross_correlation_ylm_cmplx": (_int, [_i64, _vp, _vp, _vp, _vp]),

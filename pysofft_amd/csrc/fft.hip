@@ -336,4 +336,39 @@ hipError_t launch_permute_lmn(const int32_t *perm, int64_t nc, const void *in, i
     return hipSuccess;
 }
 
+// half-complex <-> full complex over the alpha (m) axis, C-order arrays:
+// half (N, B+1, N) = (i_n, i_m <= B, beta), full (N, N, N) = (i_n, i_m, beta).
+// to_full: entries i_m > B take conj(half(-n, -m)) (the c2r input's Hermitian completion)
+__global__ void halfcomplex_kernel(const double2 *__restrict__ in, double2 *__restrict__ out, int N, int to_full)
+{
+    const int B = N / 2;
+    const int64_t n3 = (int64_t)N * N * N;
+    const int64_t nh = (int64_t)N * (B + 1) * N;
+    const int64_t total = to_full ? n3 : nh;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int beta = (int)(i % N);
+        if (to_full) {
+            const int im = (int)((i / N) % N), in_ = (int)(i / ((int64_t)N * N));
+            if (im <= B) {
+                out[i] = in[((int64_t)in_ * (B + 1) + im) * N + beta];
+            } else {
+                const double2 v = in[((int64_t)((N - in_) % N) * (B + 1) + (N - im)) * N + beta];
+                out[i] = make_double2(v.x, -v.y);
+            }
+        } else {
+            const int im = (int)((i / N) % (B + 1)), in_ = (int)(i / ((int64_t)N * (B + 1)));
+            out[i] = in[((int64_t)in_ * N + im) * N + beta];
+        }
+    }
+}
+
+hipError_t launch_halfcomplex(const double2 *in, double2 *out, int N, int to_full, hipStream_t s)
+{
+    const int64_t total = to_full ? (int64_t)N * N * N : (int64_t)N * (N / 2 + 1) * N;
+    int nb = (int)((total + 255) / 256);
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(halfcomplex_kernel, dim3(nb), dim3(256), 0, s, in, out, N, to_full);
+    return hipGetLastError();
+}
+
 }  // namespace so3

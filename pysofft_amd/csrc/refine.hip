@@ -10,11 +10,11 @@
 // d~^q_{p,q}(b) = sqrt((2q+1)/2) sqrt((2q)!/((q+p)!(q-p)!)) cos(b/2)^(q+p) sin(b/2)^(q-p)
 // (the seeds of wigner.f90:130-187 for 0 < b < pi), and the 8-variant symmetry table.
 //
-// One block (768 threads) per pair.  Work items (p, q, variant pair) run the recurrence for the 5 betas
-// (or pi - beta for the reversed variants), accumulate F(beta, m, n) over l in registers
-// and fold e^{-i n gamma_c} into LDS sums G[beta][m][c]; the alpha sums and the argmax
-// follow.  No counterpart in the reference (its callers evaluate the whole grid).
+// Four blocks per pair (quarters of the symmetric variants; see refine_kernel), then a
+// finalize kernel for |C| and the argmax.  No counterpart in the reference (its callers
+// evaluate the whole grid).
 #include <cmath>
+#include <algorithm>
 #include <cstdint>
 #include <map>
 #include <mutex>
@@ -27,239 +27,274 @@ namespace so3 {
 
 constexpr int RL_MAX = 64;   // largest degree limit Lc
 
-// recurrence constants of the step l -> l+1 for (p, q), l = q .. RL_MAX-2, at
-// off[q(q+1)/2 + p] + (l - q)
-__device__ __host__ inline int64_t rl_pair(int p, int q) { return (int64_t)q * (q + 1) / 2 + p; }
-
 __device__ __forceinline__ double2 cmulc(double2 a, double2 b)  // a * conj(b)
 {
     return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 
-// dynamic LDS: G [W beta][NM m][W gamma] | eg [W gamma][NM n] | ea [W alpha][NM m] (double2)
-__global__ __launch_bounds__(768) void refine_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g,
-                                                     int64_t nb, int n, const double *__restrict__ ein,
-                                                     double *__restrict__ eout, double *__restrict__ score, int Bg,
-                                                     int Lc, const double *__restrict__ rc,
-                                                     const int64_t *__restrict__ rcoff)
+constexpr int RW = 5;        // window points per axis
+
+// window of pair s at grid bandwidth Bg: alpha / gamma grid indices and the beta samples
+// (2k+1) pi/(4Bg), centred on the nearest grid sample to the estimate (beta kept inside
+// (0, pi))
+__device__ inline void refine_window(const double *ein, int64_t s, int Bg, int (&win)[3][RW], double (&bet)[RW])
 {
-    const int s = blockIdx.x;
-    if (s >= n) return;
-    constexpr int W = 5;
-    const int NM = 2 * Lc - 1;
-    extern __shared__ double sm[];
-    double2 *G = reinterpret_cast<double2 *>(sm);    // [W beta][NM m][W gamma]
-    double2 *eg = G + W * NM * W;                     // e^{-i n gamma_c}, [c][n + Lc - 1]
-    double2 *ea = eg + W * NM;                        // e^{-i m alpha_a}, [a][m + Lc - 1]
-    __shared__ double Cabs[W * W * W];
-    __shared__ double wl[RL_MAX];                     // 2 pi sqrt(2/(2l+1))
-    __shared__ double lfac[2 * RL_MAX + 1];           // log k!
-    __shared__ double lq[RL_MAX];                     // log sqrt((2q+1)/2)
-    __shared__ double lcb[2 * W], lsb[2 * W], cbv[2 * W];  // log cos(b/2), log sin(b/2), cos b; b, pi - b
-    __shared__ double bet[W];
-    __shared__ int win[3][W];                         // grid indices of the window
-    const int tid = threadIdx.x;
     const double pi = 3.14159265358979323846;
     const int N2 = 2 * Bg;
     const double h = pi / Bg;
+    const double a0 = ein[3 * s], b0 = ein[3 * s + 1], g0 = ein[3 * s + 2];
+    const int ja = (int)llround(a0 / h), jg = (int)llround(g0 / h);
+    int kb = (int)llround(b0 / (pi / N2) - 0.5);
+    kb = min(max(kb, 2), N2 - 3);
+    for (int i = 0; i < RW; ++i) {
+        win[0][i] = ((ja + i - 2) % N2 + N2) % N2;
+        win[1][i] = kb + i - 2;
+        win[2][i] = ((jg + i - 2) % N2 + N2) % N2;
+        bet[i] = (2.0 * win[1][i] + 1.0) * pi / (2.0 * N2);
+    }
+}
+
+// Four blocks per pair, one per quarter of the 8 symmetric variants of every order pair
+// 0 <= p <= q < Lc, chosen so that every variant of a quarter has its "row" index +p or
+// -p:
+//   quarter 0 (X+): v0 (p, q) at beta,  v4 (p, -q) at pi - beta;  rows m = +p
+//   quarter 1 (X-): v3 (-p, -q) at beta, v5 (-p, q) at pi - beta; rows m = -p
+//   quarter 2 (Y+): v2 (q, p) at beta,  v7 (-q, p) at pi - beta;  rows n = +p
+//   quarter 3 (Y-): v1 (-q, -p) at beta, v6 (q, -p) at pi - beta; rows n = -p
+// X quarters fold F(beta, m, n) over n with e^{-i n gamma_c} into R[beta][p][c], Y
+// quarters over m with e^{-i m alpha_a} into R[beta][p][a]; the remaining sum over the
+// row index follows at the end.  Work items are the order pairs in q-major order: the
+// lanes of a wave run recurrences of equal length (Lc - q) and write distinct rows p.
+// Each item runs the recurrence at the 5 window betas and at pi - beta (10 chains) and
+// accumulates F over l in registers.  The quarter stages in LDS (FP64) the coefficient
+// set indexed by +-p on its side only (f for X, g for Y) and the other set in full.
+// dynamic LDS: R [RW b][Lc p][RW k] | ep [RW k][2Lc-1] fold phases | hs [Lc(Lc+1)/2]
+// (the +-p side set) | fs [Lc^2] (the other set); hs / fs hold at least RW Lc entries:
+// the row phases of the last phase reuse them
+__global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g,
+                                                     int64_t nb, int n, const double *__restrict__ ein, int Bg, int Lc,
+                                                     double2 *__restrict__ part)
+{
+    const int s = blockIdx.x >> 2, quarter = blockIdx.x & 3;
+    if (s >= n) return;
+    const int ytype = quarter >> 1, neg = quarter & 1;
+    constexpr int W = RW;
+    const int NM = 2 * Lc - 1;
+    extern __shared__ double sm[];
+    double2 *R = reinterpret_cast<double2 *>(sm);     // [b][p][k]
+    double2 *ep = R + W * Lc * W;                      // [k][j + Lc - 1]: e^{-i j gamma_k} (X) / e^{-i j alpha_k} (Y)
+    double2 *hs = ep + W * NM;                         // side set at +-p: [l(l+1)/2 + p]
+    double2 *fs = hs + Lc * (Lc + 1) / 2;              // other set: [l(l+1) + j]
+    __shared__ double wl[RL_MAX];                      // 2 pi sqrt(2/(2l+1))
+    // recurrence constants of the step l -> l+1 (wigner.f90:243-256) factored as
+    // a = A_l / sqrt(P_{l+1}), b = B_l sqrt(P_l) / sqrt(P_{l+1}), x = p q X_l with
+    // P_l = (l^2 - p^2)(l^2 - q^2): A_l = (l+1) sqrt((2l+1)(2l+3)),
+    // B_l = (l+1) sqrt(2l+3) / (l sqrt(2l-1)), X_l = 1 / (l (l+1))
+    __shared__ double TA[RL_MAX], TB[RL_MAX], TX[RL_MAX];
+    __shared__ double lfac[2 * RL_MAX + 1];            // log k!
+    __shared__ double lq[RL_MAX];                      // log sqrt((2q+1)/2)
+    __shared__ double lcb[2 * W], lsb[2 * W], cbv[2 * W];  // log cos(b/2), log sin(b/2), cos b; b, pi - b
+    __shared__ double bet[W];
+    __shared__ int win[3][W];
+    const int tid = threadIdx.x;
+    const double pi = 3.14159265358979323846;
+    const double h = pi / Bg;
     if (tid == 0) {
-        // nearest level-grid sample to the estimate; beta samples are (2k+1) pi/(4Bg)
-        const double a0 = ein[3 * (int64_t)s], b0 = ein[3 * (int64_t)s + 1], g0 = ein[3 * (int64_t)s + 2];
-        const int ja = (int)llround(a0 / h), jg = (int)llround(g0 / h);
-        int kb = (int)llround(b0 / (pi / N2) - 0.5);
-        kb = min(max(kb, 2), N2 - 3);   // keep the window inside (0, pi)
+        int wi[3][W];
+        double be[W];
+        refine_window(ein, s, Bg, wi, be);
         for (int i = 0; i < W; ++i) {
-            win[0][i] = ((ja + i - 2) % N2 + N2) % N2;
-            win[1][i] = kb + i - 2;
-            win[2][i] = ((jg + i - 2) % N2 + N2) % N2;
-            bet[i] = (2.0 * win[1][i] + 1.0) * pi / (2.0 * N2);
+            win[0][i] = wi[0][i]; win[1][i] = wi[1][i]; win[2][i] = wi[2][i];
+            bet[i] = be[i];
         }
     }
-    __syncthreads();
-    for (int i = tid; i < W * NM * W; i += blockDim.x) G[i] = make_double2(0.0, 0.0);
+    const double2 *side = (ytype ? g : f) + (int64_t)s * nb, *other = (ytype ? f : g) + (int64_t)s * nb;
+    for (int i = tid; i < W * Lc * W; i += blockDim.x) R[i] = make_double2(0.0, 0.0);
+    for (int i = tid; i < Lc * Lc; i += blockDim.x) fs[i] = other[i];
+    for (int i = tid; i < Lc * (Lc + 1) / 2; i += blockDim.x) {
+        int l = (int)((sqrt(8.0 * i + 1.0) - 1.0) * 0.5);
+        while (l * (l + 1) / 2 > i) --l;
+        while ((l + 1) * (l + 2) / 2 <= i) ++l;
+        const int mp = i - l * (l + 1) / 2;
+        hs[i] = side[l * (l + 1) + (neg ? -mp : mp)];
+    }
     for (int l = tid; l < Lc; l += blockDim.x) {
         wl[l] = 2.0 * pi * sqrt(2.0 / (2.0 * l + 1.0));
         lq[l] = 0.5 * log((2.0 * l + 1.0) * 0.5);
+        const double dl = (double)l;
+        TA[l] = (dl + 1.0) * sqrt((2.0 * dl + 1.0) * (2.0 * dl + 3.0));
+        TB[l] = l == 0 ? 0.0 : (dl + 1.0) * sqrt(2.0 * dl + 3.0) / (dl * sqrt(2.0 * dl - 1.0));
+        TX[l] = l == 0 ? 0.0 : 1.0 / (dl * (dl + 1.0));
     }
     for (int k = tid; k <= 2 * Lc; k += blockDim.x) lfac[k] = lgamma((double)k + 1.0);
+    __syncthreads();
     if (tid < 2 * W) {
-        const double be = tid < W ? bet[tid] : pi - bet[tid - W];
-        lcb[tid] = log(cos(0.5 * be));
-        lsb[tid] = log(sin(0.5 * be));
-        cbv[tid] = cos(be);
+        const double b = tid < W ? bet[tid] : pi - bet[tid - W];
+        lcb[tid] = log(cos(0.5 * b));
+        lsb[tid] = log(sin(0.5 * b));
+        cbv[tid] = cos(b);
     }
     for (int i = tid; i < W * NM; i += blockDim.x) {
-        const int c = i / NM, m = i % NM - (Lc - 1);
-        const double tg = (double)m * win[2][c] * h, ta = (double)m * win[0][c] * h;
-        eg[i] = make_double2(cos(tg), -sin(tg));
-        ea[i] = make_double2(cos(ta), -sin(ta));
+        const int k = i / NM, j = i % NM - (Lc - 1);
+        const double t = (double)j * (ytype ? win[0][k] : win[2][k]) * h;
+        ep[i] = make_double2(cos(t), -sin(t));
     }
     __syncthreads();
 
-    const double2 *fs = f + (int64_t)s * nb, *gs = g + (int64_t)s * nb;
-    // items: (p <= q < Lc) x 4 variant pairs {0,1} {2,3} {4,5} {6,7}
+    // variant at beta (va) and at pi - beta (vb) of this quarter
+    const int va = ytype ? (neg ? 1 : 2) : (neg ? 3 : 0);
+    const int vb = ytype ? (neg ? 6 : 7) : (neg ? 5 : 4);
+    const double cb[2 * W] = {cbv[0], cbv[1], cbv[2], cbv[3], cbv[4], cbv[5], cbv[6], cbv[7], cbv[8], cbv[9]};
     const int npq = Lc * (Lc + 1) / 2;
-    for (int it = tid; it < 4 * npq; it += blockDim.x) {
-        const int vg = it & 3, P = it >> 2;
+    for (int P = tid; P < npq; P += blockDim.x) {
         int q = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
         while (q * (q + 1) / 2 > P) --q;
         while ((q + 1) * (q + 2) / 2 <= P) ++q;
         const int p = P - q * (q + 1) / 2;
-        const Variant v0 = variant(2 * vg, p, q), v1 = variant(2 * vg + 1, p, q);
+        const Variant v0 = variant(va, p, q), v1 = variant(vb, p, q);
         if (!v0.valid && !v1.valid) continue;
-        const int ro = vg >= 2 ? W : 0;   // reversed variants use pi - beta
-        // seeds d~^q_{p,q} at the 5 betas (closed form, log domain)
+        // seeds d~^q_{p,q} at beta and pi - beta (closed form, log domain)
         const double lc = 0.5 * (lfac[2 * q] - lfac[q + p] - lfac[q - p]) + lq[q];
-        double d[W], dp[W];
+        double d[2 * W], dp[2 * W];
 #pragma unroll
-        for (int b = 0; b < W; ++b) {
-            d[b] = exp(lc + (double)(q + p) * lcb[ro + b] + (double)(q - p) * lsb[ro + b]);
-            dp[b] = 0.0;
+        for (int bb = 0; bb < 2 * W; ++bb) {
+            d[bb] = exp(lc + (double)(q + p) * lcb[bb] + (double)(q - p) * lsb[bb]);
+            dp[bb] = 0.0;
         }
         double2 F0[W], F1[W];
 #pragma unroll
-        for (int b = 0; b < W; ++b) { F0[b] = make_double2(0.0, 0.0); F1[b] = make_double2(0.0, 0.0); }
-        const double *cst = rc + rcoff[rl_pair(p, q)];
-        const double s0 = (((v0.m + v0.n) & 1) ? -1.0 : 1.0) * v0.sign, s1 = (((v1.m + v1.n) & 1) ? -1.0 : 1.0) * v1.sign;
-        // operands of degree l (coefficients of both variants, recurrence constants of the
-        // step l -> l+1), loaded one degree ahead of their use
-        const int m0 = v0.valid ? v0.m : 0, n0 = v0.valid ? v0.n : 0;
-        const int m1 = v1.valid ? v1.m : 0, n1 = v1.valid ? v1.n : 0;
+        for (int bb = 0; bb < W; ++bb) { F0[bb] = make_double2(0.0, 0.0); F1[bb] = make_double2(0.0, 0.0); }
+        const double s0 = v0.valid ? (((v0.m + v0.n) & 1) ? -1.0 : 1.0) * v0.sign : 0.0;
+        const double s1 = v1.valid ? (((v1.m + v1.n) & 1) ? -1.0 : 1.0) * v1.sign : 0.0;
+        // the other set's index of each variant (n for X, m for Y; +-q); the side index is +-p
+        const int j0 = v0.valid ? (ytype ? v0.m : v0.n) : 0, j1 = v1.valid ? (ytype ? v1.m : v1.n) : 0;
+        const double pp = (double)p * p, qq = (double)q * q, pq = (double)p * q;
+        double sP = 0.0;   // sqrt(P_l); P_q = 0
         struct Ops {
-            double2 f0, g0, f1, g1;
-            double ca, cx, cbb;
+            double2 hv, o0, o1;
         };
         auto load = [&](Ops &o, int l) {
-            const int r = l * (l + 1);
-            o.f0 = fs[r + m0]; o.g0 = gs[r + n0];
-            o.f1 = fs[r + m1]; o.g1 = gs[r + n1];
-            const int k = max(0, min(l, Lc - 2) - q);   // the last degree has no step: reread entry 0
-            o.ca = cst[3 * k]; o.cx = cst[3 * k + 1]; o.cbb = cst[3 * k + 2];
+            const int rg = l * (l + 1);
+            o.hv = hs[rg / 2 + p];
+            o.o0 = fs[rg + j0];
+            o.o1 = fs[rg + j1];
         };
-        Ops nx;
-        load(nx, q);
-        for (int l = q; l < Lc; ++l) {
-            const Ops cu = nx;
-            if (l + 1 < Lc) load(nx, l + 1);
+        // one degree: F += cc^l d^l, then d^{l+1}
+        auto step = [&](const Ops &cu, int l) {
             const double wn = wl[l];
-            const double sl0 = v0.valid ? wn * s0 * ((v0.lsign && (l & 1)) ? -1.0 : 1.0) : 0.0;
-            const double sl1 = v1.valid ? wn * s1 * ((v1.lsign && (l & 1)) ? -1.0 : 1.0) : 0.0;
-            const double2 z0 = cmulc(cu.f0, cu.g0), z1 = cmulc(cu.f1, cu.g1);
+            const double sl0 = wn * s0 * ((v0.lsign && (l & 1)) ? -1.0 : 1.0);
+            const double sl1 = wn * s1 * ((v1.lsign && (l & 1)) ? -1.0 : 1.0);
+            // f^l_m conj(g^l_n): X (side f) hv conj(o), Y (side g) o conj(hv)
+            const double2 z0 = ytype ? cmulc(cu.o0, cu.hv) : cmulc(cu.hv, cu.o0);
+            const double2 z1 = ytype ? cmulc(cu.o1, cu.hv) : cmulc(cu.hv, cu.o1);
             const double2 c0 = make_double2(sl0 * z0.x, sl0 * z0.y), c1 = make_double2(sl1 * z1.x, sl1 * z1.y);
 #pragma unroll
-            for (int b = 0; b < W; ++b) {
-                F0[b].x = fma(c0.x, d[b], F0[b].x); F0[b].y = fma(c0.y, d[b], F0[b].y);
-                F1[b].x = fma(c1.x, d[b], F1[b].x); F1[b].y = fma(c1.y, d[b], F1[b].y);
+            for (int bb = 0; bb < W; ++bb) {
+                F0[bb].x = fma(c0.x, d[bb], F0[bb].x); F0[bb].y = fma(c0.y, d[bb], F0[bb].y);
+                F1[bb].x = fma(c1.x, d[W + bb], F1[bb].x); F1[bb].y = fma(c1.y, d[W + bb], F1[bb].y);
             }
             if (l < Lc - 1) {
+                const double L1 = (double)(l + 1) * (l + 1);
+                const double P1 = (L1 - pp) * (L1 - qq);
+                const double r = rsqrt(P1);
+                const double ca = TA[l] * r, cbb = TB[l] * sP * r, cacx = ca * (pq * TX[l]);
+                sP = P1 * r;
 #pragma unroll
-                for (int b = 0; b < W; ++b) {
-                    const double dn = (cu.ca * (cbv[ro + b] - cu.cx)) * d[b] - cu.cbb * dp[b];
-                    dp[b] = d[b];
-                    d[b] = dn;
+                for (int bb = 0; bb < 2 * W; ++bb) {
+                    const double k = fma(ca, cb[bb], -cacx);
+                    const double dn = fma(k, d[bb], -cbb * dp[bb]);
+                    dp[bb] = d[bb];
+                    d[bb] = dn;
                 }
             }
+        };
+        // two operand sets in turn: the loads of degree l+1 are in flight during degree l
+        Ops A, B;
+        load(A, q);
+        int l = q;
+        for (; l + 1 < Lc; l += 2) {
+            load(B, l + 1);
+            step(A, l);
+            if (l + 2 < Lc) load(A, l + 2);
+            step(B, l + 1);
         }
-        // fold e^{-i n gamma_c} into G[b][m][c]
-        auto fold = [&](const Variant &vv, const double2 (&Fk)[W]) {
+        if (l < Lc) step(A, l);
+        // fold the other index's phase into R[b][p][k] (distinct p across the lanes)
+        auto fold = [&](int jj, const double2 (&Fk)[W]) {
 #pragma unroll 1
-            for (int c = 0; c < W; ++c) {
-                const double2 e = eg[c * NM + vv.n + Lc - 1];
+            for (int k = 0; k < W; ++k) {
+                const double2 e = ep[k * NM + jj + Lc - 1];
 #pragma unroll
-                for (int b = 0; b < W; ++b) {
-                    const double2 z = Fk[b];
-                    double2 *dst = &G[(b * NM + (vv.m + Lc - 1)) * W + c];
+                for (int bb = 0; bb < W; ++bb) {
+                    const double2 z = Fk[bb];
+                    double2 *dst = &R[(bb * Lc + p) * W + k];
                     atomicAdd(&dst->x, z.x * e.x - z.y * e.y);
                     atomicAdd(&dst->y, z.x * e.y + z.y * e.x);
                 }
             }
         };
-        if (v0.valid) fold(v0, F0);
-        if (v1.valid) fold(v1, F1);
+        if (v0.valid) fold(j0, F0);
+        if (v1.valid) fold(j1, F1);
     }
     __syncthreads();
-    // alpha sums: C(a, b, c) = (1/2pi) sum_m G[b][m][c] e^{-i m alpha_a}
+    // the row sums: X quarters sum_p R[b][p][c] e^{-i m alpha_a} (m = +-p), Y quarters
+    // sum_p R[b][p][a] e^{-i n gamma_c} (n = +-p); the row phases reuse the hs space
+    double2 *er = hs;   // [k'][p]
+    const double sgn = neg ? 1.0 : -1.0;
+    for (int i = tid; i < W * Lc; i += blockDim.x) {
+        const int k = i / Lc, pr = i % Lc;
+        double sn, cs;
+        sincos((double)pr * (ytype ? win[2][k] : win[0][k]) * h, &sn, &cs);
+        er[i] = make_double2(cs, sgn * sn);
+    }
+    __syncthreads();
     for (int o = tid; o < W * W * W; o += blockDim.x) {
         const int a = o / (W * W), b = (o / W) % W, c = o % W;
+        const int kf = ytype ? a : c, kr = ytype ? c : a;   // fold index in R, row-phase index
         double re = 0.0, im = 0.0;
-        for (int mi = 0; mi < NM; ++mi) {
-            const double2 z = G[(b * NM + mi) * W + c], e = ea[a * NM + mi];
+        for (int pr = 0; pr < Lc; ++pr) {
+            const double2 z = R[(b * Lc + pr) * W + kf], e = er[kr * Lc + pr];
             re += z.x * e.x - z.y * e.y;
             im += z.x * e.y + z.y * e.x;
+        }
+        part[((int64_t)s * 4 + quarter) * (W * W * W) + o] = make_double2(re, im);
+    }
+}
+
+// |C| on the window from the four quarters, argmax (ties: the window centre, then the first
+// C-order index), euler_out / score
+__global__ void refine_finalize(const double2 *__restrict__ part, int n, const double *__restrict__ ein, int Bg,
+                                double *__restrict__ eout, double *__restrict__ score)
+{
+    constexpr int W = RW;
+    const int s = blockIdx.x;
+    if (s >= n) return;
+    __shared__ double Cabs[W * W * W];
+    const double pi = 3.14159265358979323846;
+    for (int o = threadIdx.x; o < W * W * W; o += blockDim.x) {
+        double re = 0.0, im = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            const double2 u = part[((int64_t)s * 4 + k) * (W * W * W) + o];
+            re += u.x;
+            im += u.y;
         }
         Cabs[o] = sqrt(re * re + im * im) / (2.0 * pi);
     }
     __syncthreads();
-    if (tid == 0) {
+    if (threadIdx.x == 0) {
+        int win[3][W];
+        double bet[W];
+        refine_window(ein, s, Bg, win, bet);
         int best = 62;  // window centre
         for (int o = 0; o < W * W * W; ++o)
             if (Cabs[o] > Cabs[best]) best = o;
         const int a = best / (W * W), b = (best / W) % W, c = best % W;
+        const double h = pi / Bg;
         eout[3 * (int64_t)s] = win[0][a] * h;
         eout[3 * (int64_t)s + 1] = bet[b];
         eout[3 * (int64_t)s + 2] = win[2][c] * h;
         if (score) score[s] = Cabs[best];
     }
-}
-
-__global__ void rl_consts_kernel(double *rc, const int64_t *off, int Lmax)
-{
-    const int P = blockIdx.x;  // pair index q(q+1)/2 + p
-    int q = 0;
-    while ((q + 1) * (q + 2) / 2 <= P) ++q;
-    const int p = P - q * (q + 1) / 2;
-    for (int l = q + threadIdx.x; l < Lmax - 1; l += blockDim.x) {
-        const int64_t lp = l + 1;
-        const int64_t den = (lp * lp - (int64_t)p * p) * (lp * lp - (int64_t)q * q);
-        double cc = (double)lp * sqrt(1.0 / (double)den);
-        cc = cc * sqrt((double)(2 * l + 3));
-        const double a = cc * sqrt((double)(2 * l + 1));
-        const double x = (l == 0) ? 0.0 : (double)((int64_t)p * q) / (double)((int64_t)l * (l + 1));
-        double b = 0.0;
-        if (l != q) {
-            double c1 = sqrt((double)(((int64_t)l * l - (int64_t)p * p) * ((int64_t)l * l - (int64_t)q * q))) /
-                        (double)l;
-            b = cc * c1 / sqrt((double)(2 * l - 1));
-        }
-        double *e = rc + off[P] + 3 * (l - q);
-        e[0] = a; e[1] = x; e[2] = b;
-    }
-}
-
-struct RlTables {
-    double *rc = nullptr;
-    int64_t *off = nullptr;
-};
-std::mutex g_rl_mu;
-std::map<int, RlTables> g_rl;   // per device
-
-static int rl_tables(RlTables &t)
-{
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return SO3FT_EHIP;
-    std::lock_guard<std::mutex> lk(g_rl_mu);
-    auto it = g_rl.find(dev);
-    if (it != g_rl.end()) { t = it->second; return SO3FT_OK; }
-    const int npair = RL_MAX * (RL_MAX + 1) / 2;
-    std::vector<int64_t> off(npair);
-    int64_t tot = 0;
-    for (int q = 0; q < RL_MAX; ++q)
-        for (int p = 0; p <= q; ++p) {
-            off[rl_pair(p, q)] = tot;
-            tot += 3 * (RL_MAX - q);
-        }
-    RlTables nt;
-    if (hipMalloc((void **)&nt.rc, sizeof(double) * tot) != hipSuccess ||
-        hipMalloc((void **)&nt.off, sizeof(int64_t) * npair) != hipSuccess) {
-        hipGetLastError();
-        return SO3FT_ENOMEM;
-    }
-    if (hipMemcpy(nt.off, off.data(), sizeof(int64_t) * npair, hipMemcpyHostToDevice) != hipSuccess) return SO3FT_EHIP;
-    hipLaunchKernelGGL(rl_consts_kernel, dim3(npair), dim3(64), 0, 0, nt.rc, nt.off, RL_MAX);
-    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return SO3FT_EHIP;
-    g_rl[dev] = nt;
-    t = nt;
-    return SO3FT_OK;
 }
 
 }  // namespace so3
@@ -274,20 +309,52 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
     if (bw < 2 || !f_lms || !g_lms || !euler_in || !euler_out || n < 0) return SO3FT_EINVAL;
     if (grid_bw < 4 || grid_bw > 4096 || lmax < 1 || lmax > RL_MAX || lmax > bw) return SO3FT_EINVAL;
     if (n == 0) return SO3FT_OK;
-    RlTables t;
-    if (int rc = rl_tables(t)) return rc;
-    const int NM = 2 * (int)lmax - 1;
-    const size_t bytes = sizeof(double2) * 35 * NM;   // G (25 NM) + eg, ea (5 NM each)
+    if (n > (1 << 29)) return SO3FT_EINVAL;
+    const int Lc = (int)lmax, NM = 2 * Lc - 1;
+    // the alpha table [RW][Lc] reuses the f / g space after the main phase
+    const size_t fg = std::max<size_t>((size_t)Lc * (Lc + 1) / 2 + (size_t)Lc * Lc, (size_t)RW * Lc);
+    const size_t bytes = sizeof(double2) * ((size_t)RW * Lc * RW + RW * NM + fg);
     static bool attr = false;
     if (!attr) {
-        if (hipFuncSetAttribute((const void *)refine_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024) !=
+        if (hipFuncSetAttribute((const void *)refine_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024) !=
             hipSuccess)
             return SO3FT_EHIP;
         attr = true;
     }
-    hipLaunchKernelGGL(refine_kernel, dim3(n), dim3(768), bytes, reinterpret_cast<hipStream_t>(stream),
-                       reinterpret_cast<const double2 *>(f_lms), reinterpret_cast<const double2 *>(g_lms), bw * bw,
-                       (int)n, euler_in, euler_out, score, (int)grid_bw, (int)lmax, t.rc, t.off);
+    if (bytes > 150 * 1024) return SO3FT_EINVAL;
+    // the four quarters' window sums, per device (grown on demand; calls on one device share it)
+    double2 *part = nullptr;
+    {
+        static std::mutex mu;
+        static std::map<int, std::pair<double2 *, int64_t>> cache;
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return SO3FT_EHIP;
+        std::lock_guard<std::mutex> lk(mu);
+        auto &e = cache[dev];
+        if (e.second < n) {
+            if (e.first) {
+                if (hipDeviceSynchronize() != hipSuccess) return SO3FT_EHIP;
+                hipFree(e.first);
+            }
+            e.first = nullptr;
+            e.second = 0;
+            if (hipMalloc((void **)&e.first, sizeof(double2) * 4 * RW * RW * RW * (size_t)n) != hipSuccess) {
+                hipGetLastError();
+                return SO3FT_ENOMEM;
+            }
+            e.second = n;
+        }
+        part = e.first;
+    }
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    // one thread per order pair up to 512 (small degree limits leave room for more blocks per CU)
+    const int npq = Lc * (Lc + 1) / 2;
+    const int threads = std::min(512, std::max(64, (npq + 63) / 64 * 64));
+    hipLaunchKernelGGL(refine_kernel, dim3(4 * n), dim3(threads), bytes, st, reinterpret_cast<const double2 *>(f_lms),
+                       reinterpret_cast<const double2 *>(g_lms), bw * bw, (int)n, euler_in, (int)grid_bw, Lc, part);
+    if (hipGetLastError() != hipSuccess) return SO3FT_EHIP;
+    hipLaunchKernelGGL(refine_finalize, dim3(n), dim3(128), 0, st, part, (int)n, euler_in, (int)grid_bw, euler_out,
+                       score);
     return hipGetLastError() == hipSuccess ? SO3FT_OK : SO3FT_EHIP;
 }
 

@@ -777,6 +777,48 @@ int so3ft_ifft(so3ft_handle h, const void *f1, void *f2, void *stream)
     return fft2_scaled(P, f1, f2, +1, S(stream));
 }
 
+// soft.f08:3270-3276 rfft: r2c over (alpha, gamma) of every beta slice, the alpha axis
+// halved to bw+1 (f2: C-order (2bw, bw+1, 2bw)), scaled by 1/(2bw)
+int so3ft_rfft(so3ft_handle h, const void *f1, void *f2, void *stream)
+{
+    GETP(h);
+    if (int rc = check_ptrs(f1, f2)) return rc;
+    if (int rc = ensure_scratch(P, 1)) return rc;
+    hipStream_t s = S(stream);
+    const int64_t N = P->N, N2 = N * N, N3 = N2 * N;
+    FftAxis b{};
+    b.in_t = N3; b.in_o = N; b.in_r = N2; b.out_t = N3; b.out_o = N; b.out_r = N2;
+    b.in_real = 1; b.sign = -1; b.scale = 1.0;
+    HIPCHK(launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, f1, P->scratch, b, 1, s));
+    FftAxis a{};
+    a.in_t = N3; a.in_o = N2; a.in_r = N; a.out_t = N3; a.out_o = N2; a.out_r = N;
+    a.sign = -1; a.scale = 1.0 / (double)N;
+    HIPCHK(launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, P->scratch, P->scratch, a, 1, s));
+    HIPCHK(launch_halfcomplex(P->scratch, reinterpret_cast<double2 *>(f2), (int)N, 0, s));
+    return SO3FT_OK;
+}
+
+// soft.f08:3277-3283 irfft: c2r (FFTW_BACKWARD) of the half-complex f1 into the real f2,
+// scaled by 1/(2bw); the redundant half is the Hermitian completion, as FFTW's c2r reads it
+int so3ft_irfft(so3ft_handle h, const void *f1, void *f2, void *stream)
+{
+    GETP(h);
+    if (int rc = check_ptrs(f1, f2)) return rc;
+    if (int rc = ensure_scratch(P, 1)) return rc;
+    hipStream_t s = S(stream);
+    const int64_t N = P->N, N2 = N * N, N3 = N2 * N;
+    HIPCHK(launch_halfcomplex(reinterpret_cast<const double2 *>(f1), P->scratch, (int)N, 1, s));
+    FftAxis a{};
+    a.in_t = N3; a.in_o = N2; a.in_r = N; a.out_t = N3; a.out_o = N2; a.out_r = N;
+    a.sign = +1; a.scale = 1.0;
+    HIPCHK(launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, P->scratch, P->scratch, a, 1, s));
+    FftAxis b{};
+    b.in_t = N3; b.in_o = N; b.in_r = N2; b.out_t = N3; b.out_o = N; b.out_r = N2;
+    b.out_real = 1; b.sign = +1; b.scale = 1.0 / (double)N;
+    HIPCHK(launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, P->scratch, f2, b, 1, s));
+    return SO3FT_OK;
+}
+
 static int integrate(Plan *P, const void *f, int is_real, double *re, double *im, hipStream_t s)
 {
     const int nb = 1024;

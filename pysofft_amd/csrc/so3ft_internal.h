@@ -142,6 +142,8 @@ hipError_t launch_integrate(const void *f, int is_real, const double *w, int N, 
 // source side (1: n < 0 as utils.f90:471-488, 0: n > 0 as the lmn routine utils.f90:510-550)
 hipError_t launch_enforce_real_sym(double2 *c, int64_t c_ts, const int64_t *slice, int bw, int T, int m0_neg_src,
                                    hipStream_t s);
+// half-complex (N, N/2+1, N) <-> full (N, N, N) complex arrays over the alpha axis
+hipError_t launch_halfcomplex(const double2 *in, double2 *out, int N, int to_full, hipStream_t s);
 // lmn <-> mnl reordering of T coefficient sets (esz 16: complex128, 8: complex64)
 hipError_t launch_permute_lmn(const int32_t *perm, int64_t nc, const void *in, int64_t in_ts, void *out,
                               int64_t out_ts, int esz, int to_mnl, int T, hipStream_t s);

@@ -97,12 +97,14 @@ class Soft:
         self._lmns = None
 
     def __del__(self):
-        if self._handle is not None and multiprocessing.current_process().name == self._init_process_name:
-            try:
+        # freed only by the creating process (soft.py:69-72); at interpreter shutdown the
+        # modules this needs may already be gone
+        try:
+            if self._handle is not None and multiprocessing.current_process().name == self._init_process_name:
                 _lib.lib().so3ft_destroy(self._handle)
-            except Exception:
-                pass
-            self._handle = None
+        except Exception:
+            pass
+        self._handle = None
 
     # -- properties (pysofft/soft.py:74-112) -------------------------------------
     def _get(self, fn, ctype=C.c_int64):
@@ -341,6 +343,18 @@ class Soft:
         torch = _torch()
         return self._run(_lib.lib().so3ft_ifft, f1, torch.complex128, self.grid_shape, f2, torch.complex128,
                          self.grid_shape)
+
+    def _rfft(self, f1, f2=None):
+        """soft.py:708-709: real (2bw)^3 -> half-complex (2bw, bw+1, 2bw), /(2bw)."""
+        torch = _torch()
+        half = (2 * self.bw, self.bw + 1, 2 * self.bw)
+        return self._run(_lib.lib().so3ft_rfft, f1, torch.float64, self.grid_shape, f2, torch.complex128, half)
+
+    def _irfft(self, f1, f2=None):
+        """soft.py:710-711: half-complex (2bw, bw+1, 2bw) -> real (2bw)^3, /(2bw)."""
+        torch = _torch()
+        half = (2 * self.bw, self.bw + 1, 2 * self.bw)
+        return self._run(_lib.lib().so3ft_irfft, f1, torch.complex128, half, f2, torch.float64, self.grid_shape)
 
     # -- derived (pysofft/soft.py:519-703) ---------------------------------------
     def integrate_over_so3(self, f):

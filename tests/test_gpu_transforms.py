@@ -479,3 +479,21 @@ def test_enforce_real_sym_lmn_device_vs_host():
     t = torch.from_numpy(c).cuda()
     Soft(bw, recurrence_type=1).enforce_real_symmetry(t)
     np.testing.assert_array_equal(t.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("bw", [1, 3, 8])
+def test_rfft_irfft_vs_numpy(bw):
+    """py_rfft / py_irfft (soft.f08:3270-3283): r2c over (alpha, gamma) with the alpha axis
+    halved, c2r back (FFTW_BACKWARD, reading the stored half as FFTW does), both /(2bw);
+    numpy's rfftn / irfftn over axes (gamma, alpha) are the library routines they equal."""
+    N = 2 * bw
+    rng = np.random.default_rng(bw)
+    s = _soft(bw)
+    f = rng.standard_normal((N, N, N))
+    got = s._rfft(f)
+    assert got.shape == (N, bw + 1, N)
+    assert rel_l2(got, np.fft.rfftn(f, axes=(0, 1)) / N) < 1e-13
+    X = _rand_c(rng, (N, bw + 1, N))
+    back = s._irfft(X)
+    assert rel_l2(back, np.fft.irfftn(X, s=(N, N), axes=(0, 1)) * N) < 1e-13
+    assert rel_l2(s._irfft(got), f) < 1e-13
