@@ -14,10 +14,10 @@
 //   recurrence; every 4 degrees the wave writes d (hi, lo) for its 64 betas to a small
 //   LDS tile and reads it back as the A operands of four 16-beta tiles (M = beta,
 //   K = degree); B = the coefficients (K = degree, N = variant x re/im).
-// forward: NW waves per (pair, 64-degree chunk), wave w owning TB transforms.  For each
-//   64-beta tile the lanes (as betas, 64/NW per wave) run the recurrence and store w(beta) d (hi, lo) in LDS as
-//   [beta][degree]; A = that tile (M = degree, K = beta), B = the fft-array samples
-//   (K = beta, N = variant x re/im).
+// forward: one wave per (pair, TB transforms), lane = beta within 64-beta chunks; for
+//   each 16-degree tile and chunk the lanes advance their recurrence and store
+//   w(beta) d (hi, lo) in LDS as [degree][beta]; A = that tile (M = degree, K = beta),
+//   B = the fft-array samples (K = beta, N = variant x re/im).
 #include "so3ft_internal.h"
 
 namespace so3 {
@@ -239,30 +239,30 @@ __global__ __launch_bounds__(64) void dwt_inverse_mix_grouped(const float2 *__re
 }
 
 // ---------------------------------------------------------------------------
-// forward
+// forward, one wave per (pair, TB transforms): lane = beta within a 64-beta chunk; the
+// lane carries one recurrence chain per chunk (beta = 64 c + lane), its state kept in
+// LDS between tiles.  Degrees go in 16-degree tiles; for each (tile, chunk) the lanes
+// advance their chain by 16 degrees, writing w(beta) d (hi, lo) to an LDS tile
+// [degree][beta], and the wave runs 16 K-steps of v_mfma_f32_16x16x4f32 with A = that
+// tile (M = degree, K = beta) and B = the fft-array samples (K = beta, N = variant x
+// re/im), in four independent accumulators per transform.  FP32 accumulation over 16
+// betas, FP64 across them and across chunks: a tile's coefficients are complete after
+// its last chunk, so there is no cross-wave reduction and no degree is recomputed.
 // ---------------------------------------------------------------------------
-constexpr int FWD_DS = 66;  // padded row of the forward tile (float2 entries)
+constexpr int FT_DS = 66;   // padded row of the forward tile (float2 entries)
 
-// one forward work item: order pair (p, q), degrees 64 chunk .. 64 chunk + 63; wave wv
-// owns transforms tb0 + wv TB .. tb0 + wv TB + TB - 1 (< T)
-template <int TB, int NW>
-__device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64_t f_ts,
-                                             float2 *__restrict__ coeff, int64_t c_ts, int T,
-                                             const double *__restrict__ w, const double *__restrict__ cosb,
-                                             const double *__restrict__ seeds, const int64_t *__restrict__ slice,
-                                             int bw, int N, int p, int q, int tb0, int chunk, double *cst,
-                                             float2 (*Dt)[FWD_DS])
+template <int TB>
+__device__ __forceinline__ void fwd_mix2_item(const float2 *__restrict__ F, int64_t f_ts,
+                                              float2 *__restrict__ coeff, int64_t c_ts, int T,
+                                              const double *__restrict__ w, const double *__restrict__ cosb,
+                                              const double *__restrict__ seeds, const int64_t *__restrict__ slice,
+                                              int bw, int N, int p, int q, int t0, double *cst,
+                                              float2 (*Dt)[FT_DS], double2 (*rs)[64])
 {
-    constexpr int BPW = 64 / NW;           // betas of a tile whose recurrence one wave runs
     const int lane = threadIdx.x & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index, uniform
     const int L = bw - q;
-    const int lc0 = chunk * 64;
-    if (lc0 >= L) return;
-    const int lend = min(L, lc0 + 64);
-    const int t0 = tb0 + wv * TB;
-    const int nt = min(TB, T - t0);        // <= 0 for spare waves of the last group
-
+    const int nc = (N + 63) >> 6;
+    const int nt = min(TB, T - t0);
     const int j = lane & 15, kk = lane >> 4;
     const int vj = j >> 1, partj = j & 1;
     const Variant vr = variant(vj, p, q);
@@ -272,168 +272,152 @@ __device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64
     for (int t = 0; t < TB; ++t)
         src[t] = reinterpret_cast<const float *>(F + (int64_t)max(0, min(t0 + t, T - 1)) * f_ts + colj) + partj;
 
-    for (int i = threadIdx.x; i < lend - 1; i += 64 * NW) {
+    for (int i = lane; i < L - 1; i += 64) {
         double a, x, b;
         rec_consts_m(q + i, p, q, a, x, b);
         cst[3 * i] = a; cst[3 * i + 1] = x; cst[3 * i + 2] = b;
     }
     const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
-
-    f4_t acc[TB][4];
-    double accd[TB][4][4];
-#pragma unroll
-    for (int t = 0; t < TB; ++t)
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            acc[t][g] = f4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int r = 0; r < 4; ++r) accd[t][g][r] = 0.0;
-        }
-    const int nlt = (lend - lc0 + 15) >> 4;   // 16-degree tiles with work
-    // B operand of beta step bs (betas 4 bs .. 4 bs + 3): the fft-array sample of column j
-    // at beta 4 bs + kk (reversed for v >= 4); ring three steps deep
-    const int nbs = (N + 3) >> 2;
-    auto loadb = [&](float (&dst)[TB], int bs) {
-        const int bt = min(4 * bs + kk, N - 1);   // clamped rows are multiplied by zero d rows
-#pragma unroll
-        for (int t = 0; t < TB; ++t) dst[t] = src[t][2 * (int64_t)(vr.rev ? N - 1 - bt : bt)];
-    };
-    float bq0[TB], bq1[TB], bq2[TB];
-    if (nt > 0) {
-        loadb(bq0, 0);
-        loadb(bq1, 1);
-        loadb(bq2, 2);
+    // recurrence state (d^l, d^{l-1}) of this lane's chain in each chunk
+    for (int c = 0; c < nc; ++c) {
+        const int beta = 64 * c + lane;
+        rs[c][lane] = make_double2(beta < N ? seeds[tri * N + beta] : 0.0, 0.0);
     }
+    __syncthreads();   // cst, rs (one wave: orders the LDS writes before the reads)
 
-    for (int b0 = 0; b0 < N; b0 += 64) {
-        __syncthreads();  // cst ready / previous tile consumed
-        if (lane < BPW) {
-            const int bl = wv * BPW + lane;
-            const int beta = b0 + bl;
+#pragma unroll 1
+    for (int l0 = 0; l0 < L; l0 += 16) {
+        double accd[TB][4];
+#pragma unroll
+        for (int t = 0; t < TB; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) accd[t][r] = 0.0;
+#pragma unroll 1
+        for (int c = 0; c < nc; ++c) {
+            // B operands of this chunk: the sample of column j at beta 64 c + 4 s + kk
+            float bv[16][TB];
+#pragma unroll
+            for (int s4 = 0; s4 < 16; ++s4) {
+                const int bt = min(64 * c + 4 * s4 + kk, N - 1);   // rows past N meet zero A columns
+                const int64_t o = 2 * (int64_t)(vr.rev ? N - 1 - bt : bt);
+#pragma unroll
+                for (int t = 0; t < TB; ++t) bv[s4][t] = src[t][o];
+            }
+            // w d (hi, lo) for degrees l0 .. l0 + 15 at this lane's beta of chunk c
+            const int beta = 64 * c + lane;
             const bool ok = beta < N;
-            double dc = ok ? seeds[tri * N + beta] : 0.0, dp = 0.0;
-            const double cb = ok ? cosb[beta] : 0.0;
-            const double wb = ok ? w[beta] : 0.0;
-            // unrolled so the constants' LDS reads run ahead of the chain through d
-            int li = 0;
-#pragma unroll 8
-            for (; li < lc0; ++li) {
-                const double ca = cst[3 * li], cx = cst[3 * li + 1], cbb = cst[3 * li + 2];
-                const double dn = (ca * (cb - cx)) * dc - cbb * dp;
-                dp = dc;
-                dc = dn;
-            }
-#pragma unroll 8
-            for (; li < lc0 + 64; ++li) {
+            const double cb = ok ? cosb[beta] : 0.0, wt = ok ? w[beta] : 0.0;
+            asm volatile("" ::: "memory");
+            double2 st = rs[c][lane];
+            double dcur = st.x, dprev = st.y;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int li = l0 + k;
                 float hi, lo;
-                split(li < lend ? wb * dc : 0.0, hi, lo);
-                Dt[bl][li - lc0] = make_float2(hi, lo);
-                if (li < lend - 1) {
+                split(li < L ? wt * dcur : 0.0, hi, lo);
+                Dt[k][lane] = make_float2(hi, lo);
+                if (li < L - 1) {
                     const double ca = cst[3 * li], cx = cst[3 * li + 1], cbb = cst[3 * li + 2];
-                    const double dn = (ca * (cb - cx)) * dc - cbb * dp;
-                    dp = dc;
-                    dc = dn;
+                    const double dn = (ca * (cb - cx)) * dcur - cbb * dprev;
+                    dprev = dcur;
+                    dcur = dn;
                 }
             }
+            rs[c][lane] = make_double2(dcur, dprev);
+            asm volatile("" ::: "memory");
+            // four independent accumulators per transform (every fourth K-step), 16 betas
+            // each: no back-to-back dependent MFMAs
+            f4_t acc[TB][4];
+#pragma unroll
+            for (int t = 0; t < TB; ++t)
+#pragma unroll
+                for (int g = 0; g < 4; ++g) acc[t][g] = f4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s4 = 0; s4 < 16; ++s4) {
+                const float2 a = Dt[j][4 * s4 + kk];
+#pragma unroll
+                for (int t = 0; t < TB; ++t) acc[t][s4 & 3] = mma32(a.x, bv[s4][t], acc[t][s4 & 3]);
+#pragma unroll
+                for (int t = 0; t < TB; ++t) acc[t][s4 & 3] = mma32(a.y, bv[s4][t], acc[t][s4 & 3]);
+            }
+#pragma unroll
+            for (int t = 0; t < TB; ++t)
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) accd[t][r] += (double)acc[t][g][r];
         }
-        __syncthreads();
-        if (nt <= 0) continue;
-        const int nb = min(64, N - b0);
-        for (int bb = 0; bb < nb; bb += 4) {
-            const int bs = (b0 + bb) >> 2;
-            float bv[TB];
+        // item r of lane l: degree row 4(l>>4) + r of the tile, column j = l & 15
+        if (vr.valid) {
+            const int64_t o = slice_m(slice, vr.m, vr.n, bw);
 #pragma unroll
-            for (int t = 0; t < TB; ++t) { bv[t] = bq0[t]; bq0[t] = bq1[t]; bq1[t] = bq2[t]; }
-            loadb(bq2, min(bs + 3, nbs - 1));
-            float2 a[4];
+            for (int t = 0; t < TB; ++t) {
+                if (t >= nt) continue;
+                float *ct = reinterpret_cast<float *>(coeff + (int64_t)(t0 + t) * c_ts + o) + partj;
 #pragma unroll
-            for (int g = 0; g < 4; ++g) a[g] = Dt[bb + kk][16 * g + j];
-            // all hi products, then all lo products: no back-to-back dependent MFMAs
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                if (g >= nlt) break;
-#pragma unroll
-                for (int t = 0; t < TB; ++t) acc[t][g] = mma32(a[g].x, bv[t], acc[t][g]);
-            }
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                if (g >= nlt) break;
-#pragma unroll
-                for (int t = 0; t < TB; ++t) acc[t][g] = mma32(a[g].y, bv[t], acc[t][g]);
-            }
-            if (((bb >> 2) & 3) == 3 || bb + 4 >= nb) {  // flush every 16 betas
-#pragma unroll
-                for (int t = 0; t < TB; ++t)
-#pragma unroll
-                    for (int g = 0; g < 4; ++g) {
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) accd[t][g][r] += (double)acc[t][g][r];
-                        acc[t][g] = f4_t{0.f, 0.f, 0.f, 0.f};
+                for (int r = 0; r < 4; ++r) {
+                    const int li = l0 + 4 * kk + r;
+                    if (li < L) {
+                        const int l = q + li;
+                        const double sg = vr.sign * ((vr.lsign && (l & 1)) ? -1.0 : 1.0);
+                        ct[2 * (int64_t)li] = (float)(sg * accd[t][r]);
                     }
-            }
-        }
-    }
-    // item r of lane l: degree row 4(l>>4) + r of tile g, column j = l & 15
-    if (!vr.valid || nt <= 0) return;
-    const int64_t o = slice_m(slice, vr.m, vr.n, bw);
-#pragma unroll
-    for (int t = 0; t < TB; ++t) {
-        if (t >= nt) continue;
-        float *ct = reinterpret_cast<float *>(coeff + (int64_t)(t0 + t) * c_ts + o) + partj;
-#pragma unroll
-        for (int g = 0; g < 4; ++g)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int li = lc0 + 16 * g + 4 * kk + r;
-                if (li < L) {
-                    const int l = q + li;
-                    const double s = vr.sign * ((vr.lsign && (l & 1)) ? -1.0 : 1.0);
-                    ct[2 * (int64_t)li] = (float)(s * accd[t][g][r]);
                 }
             }
+        }
     }
 }
 
-template <int TB, int NW>
-__global__ __launch_bounds__(64 * NW) void dwt_forward_mix(const float2 *__restrict__ F, int64_t f_ts,
-                                                           float2 *__restrict__ coeff, int64_t c_ts, int T,
-                                                           const double *__restrict__ w,
-                                                           const double *__restrict__ cosb,
-                                                           const double *__restrict__ seeds,
-                                                           const int64_t *__restrict__ slice,
-                                                           const int32_t *__restrict__ pairs, int bw, int N, int npair,
-                                                           int nch, int nunit)
+template <int TB>
+__global__ __launch_bounds__(64) void dwt_forward_mix2(const float2 *__restrict__ F, int64_t f_ts,
+                                                       float2 *__restrict__ coeff, int64_t c_ts, int T,
+                                                       const double *__restrict__ w, const double *__restrict__ cosb,
+                                                       const double *__restrict__ seeds,
+                                                       const int64_t *__restrict__ slice,
+                                                       const int32_t *__restrict__ pairs, int bw, int N, int npair,
+                                                       int nunit)
 {
-    __shared__ double cst[3 * 258];
-    __shared__ float2 Dt[64][FWD_DS];      // [beta in tile][degree in chunk] = w d (hi, lo)
-    const int id = blockIdx.x;
-    const int chunk = (id >> 3) % nch;
-    const int unit = ((id >> 3) / nch) * 8 + (id & 7);
+    // dynamic LDS: Dt [16][FT_DS] float2 | rs [nc][64] double2 | cst [3 (bw - 1)] double
+    extern __shared__ double smx[];
+    float2 (*Dt)[FT_DS] = reinterpret_cast<float2 (*)[FT_DS]>(smx);
+    double2 (*rs)[64] = reinterpret_cast<double2 (*)[64]>(smx + 16 * FT_DS);
+    double *cst = smx + 16 * FT_DS + 2 * 64 * ((N + 63) >> 6);
+    const int unit = blockIdx.x;
     if (unit >= nunit) return;
     const int pq = pairs[unit % npair];
-    fwd_mix_item<TB, NW>(F, f_ts, coeff, c_ts, T, w, cosb, seeds, slice, bw, N, pq & 0xffff, pq >> 16,
-                         (unit / npair) * NW * TB, chunk, cst, Dt);
+    fwd_mix2_item<TB>(F, f_ts, coeff, c_ts, T, w, cosb, seeds, slice, bw, N, pq & 0xffff, pq >> 16,
+                      (unit / npair) * TB, cst, Dt, rs);
 }
 
-template <int TB, int NW>
-__global__ __launch_bounds__(64 * NW) void dwt_forward_mix_grouped(const float2 *__restrict__ F,
-                                                                   float2 *__restrict__ coeff,
-                                                                   const MixSegment *__restrict__ segs,
-                                                                   const int4 *__restrict__ items, int nitem)
+// grouped forward over transforms of several bandwidths (see dwt_inverse_mix_grouped);
+// dynamic LDS sized for the largest bandwidth of the group
+template <int TB>
+__global__ __launch_bounds__(64) void dwt_forward_mix2_grouped(const float2 *__restrict__ F,
+                                                               float2 *__restrict__ coeff,
+                                                               const MixSegment *__restrict__ segs,
+                                                               const int4 *__restrict__ items, int nitem)
 {
-    __shared__ double cst[3 * 258];
-    __shared__ float2 Dt[64][FWD_DS];
+    extern __shared__ double smx[];
     for (int it = blockIdx.x; it < nitem; it += gridDim.x) {
         const int4 wk = items[it];
         if (wk.x < 0) continue;
         const MixSegment sg = segs[wk.x];
-        __syncthreads();  // every wave is done with the previous item's LDS
-        fwd_mix_item<TB, NW>(F + sg.goff, (int64_t)sg.N * sg.N * sg.N, coeff + sg.coff, sg.nc, sg.T, sg.w, sg.cosb,
-                             sg.seeds, sg.slice, sg.bw, sg.N, wk.y & 0xffff, wk.y >> 16, wk.z, wk.w, cst, Dt);
+        __syncthreads();  // the previous item's LDS reads are done
+        float2 (*Dt)[FT_DS] = reinterpret_cast<float2 (*)[FT_DS]>(smx);
+        double2 (*rs)[64] = reinterpret_cast<double2 (*)[64]>(smx + 16 * FT_DS);
+        double *cst = smx + 16 * FT_DS + 2 * 64 * ((sg.N + 63) >> 6);
+        fwd_mix2_item<TB>(F + sg.goff, (int64_t)sg.N * sg.N * sg.N, coeff + sg.coff, sg.nc, sg.T, sg.w, sg.cosb,
+                          sg.seeds, sg.slice, sg.bw, sg.N, wk.y & 0xffff, wk.y >> 16, wk.z, cst, Dt, rs);
     }
 }
 
 static int grid_m(int nunit, int nch) { return (nunit + 7) / 8 * 8 * nch; }
+
+// dynamic LDS bytes of dwt_forward_mix2 (A tile, chunk states, recurrence constants)
+static size_t fwd2_lds(int bw, int N)
+{
+    return sizeof(double) * (16 * FT_DS + 2 * 64 * ((N + 63) / 64) + 3 * (bw > 1 ? bw - 1 : 1));
+}
 
 hipError_t launch_dwt_inverse_mixed(const DeviceTables &tb, int bw, int N, int npair, int mode, const float2 *coeff,
                                     int64_t c_ts, const float2 *aux, float2 *F, int64_t f_ts, int T, hipStream_t s)
@@ -458,22 +442,22 @@ hipError_t launch_dwt_forward_mixed(const DeviceTables &tb, int bw, int N, int n
 {
     if (mode != DWT_COMPLEX) return hipErrorInvalidValue;
     if (npair <= 0 || T <= 0) return hipSuccess;
-    const int nch = (bw + 63) / 64;
-#define SO3_MFWD(TBV, NWV)                                                                                     \
+#define SO3_MFWD2(TBV)                                                                                         \
     do {                                                                                                       \
-        const int nunit = npair * ((T + TBV * NWV - 1) / (TBV * NWV));                                         \
-        hipLaunchKernelGGL((dwt_forward_mix<TBV, NWV>), dim3(grid_m(nunit, nch)), dim3(64 * NWV), 0, s, F, f_ts, \
-                           coeff, c_ts, T, tb.w, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, npair, nch, nunit); \
+        const int nunit = npair * ((T + TBV - 1) / TBV);                                                       \
+        hipLaunchKernelGGL((dwt_forward_mix2<TBV>), dim3(nunit), dim3(64), fwd2_lds(bw, N), s, F, f_ts, coeff,   \
+                           c_ts, T, tb.w,                                                                      \
+                           tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, npair, nunit);                       \
     } while (0)
-    if (T >= 8) SO3_MFWD(2, 4);
-    else if (T >= 4) SO3_MFWD(1, 4);
-    else if (T >= 2) SO3_MFWD(1, 2);
-    else SO3_MFWD(1, 1);
-#undef SO3_MFWD
+    // four transforms per wave from bw = 96 (more MFMA work per recurrence step), else two
+    if (bw >= 96 && T >= 4) SO3_MFWD2(4);
+    else if (T >= 2) SO3_MFWD2(2);
+    else SO3_MFWD2(1);
+#undef SO3_MFWD2
     return hipGetLastError();
 }
 
-// ---- grouped launches: TB (inverse), TB x NW (forward) from the tuned configuration
+// ---- grouped launches: TB (transforms per wave) and grid from the tuned configuration
 hipError_t launch_dwt_inverse_mixed_grouped(int TB, const float2 *coeff, float2 *F, const MixSegment *segs,
                                             const int4 *items, int nitem, int grid, hipStream_t s)
 {
@@ -488,27 +472,25 @@ hipError_t launch_dwt_inverse_mixed_grouped(int TB, const float2 *coeff, float2 
     return hipGetLastError();
 }
 
-hipError_t launch_dwt_forward_mixed_grouped(int TB, int NW, const float2 *F, float2 *coeff, const MixSegment *segs,
-                                            const int4 *items, int nitem, int grid, hipStream_t s)
+hipError_t launch_dwt_forward_mixed_grouped(int TB, const float2 *F, float2 *coeff, const MixSegment *segs,
+                                            const int4 *items, int nitem, int grid, int max_bw, hipStream_t s)
 {
     if (nitem <= 0) return hipSuccess;
     if (grid <= 0) return hipErrorInvalidValue;
-#define SO3_GFWD(TBV, NWV)                                                                                     \
-    if (TB == TBV && NW == NWV) {                                                                              \
-        hipLaunchKernelGGL((dwt_forward_mix_grouped<TBV, NWV>), dim3(grid), dim3(64 * NWV), 0, s, F, coeff, segs, \
-                           items, nitem);                                                                      \
-        return hipGetLastError();                                                                              \
-    }
-    SO3_GFWD(2, 4)
-    SO3_GFWD(1, 4)
-    SO3_GFWD(1, 2)
-    SO3_GFWD(1, 1)
-#undef SO3_GFWD
-    return hipErrorInvalidValue;
+    const size_t lds = fwd2_lds(max_bw, 2 * max_bw);
+    if (TB == 4)
+        hipLaunchKernelGGL((dwt_forward_mix2_grouped<4>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
+    else if (TB == 2)
+        hipLaunchKernelGGL((dwt_forward_mix2_grouped<2>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
+    else if (TB == 1)
+        hipLaunchKernelGGL((dwt_forward_mix2_grouped<1>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
-// resident blocks per CU of a grouped kernel
-int mixed_grouped_occupancy(int inverse, int TB, int NW)
+// resident blocks per CU of a grouped kernel (forward: LDS for bandwidth max_bw)
+int mixed_grouped_occupancy(int inverse, int TB, int max_bw)
 {
     int nb = 0;
     hipError_t e = hipErrorInvalidValue;
@@ -516,10 +498,10 @@ int mixed_grouped_occupancy(int inverse, int TB, int NW)
         if (TB == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_inverse_mix_grouped<2>, 64, 0);
         else if (TB == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_inverse_mix_grouped<1>, 64, 0);
     } else {
-        if (TB == 2 && NW == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<2, 4>, 256, 0);
-        else if (TB == 1 && NW == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<1, 4>, 256, 0);
-        else if (TB == 1 && NW == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<1, 2>, 128, 0);
-        else if (TB == 1 && NW == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<1, 1>, 64, 0);
+        const size_t lds = fwd2_lds(max_bw, 2 * max_bw);
+        if (TB == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix2_grouped<4>, 64, lds);
+        else if (TB == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix2_grouped<2>, 64, lds);
+        else if (TB == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix2_grouped<1>, 64, lds);
     }
     if (e != hipSuccess) {
         (void)hipGetLastError();

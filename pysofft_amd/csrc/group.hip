@@ -38,7 +38,7 @@ struct Seg {
 
 struct Config {
     int inv_tb = 2, inv_grid = 0;          // grid 0: one block per work item
-    int fwd_tb = 2, fwd_nw = 4, fwd_grid = 0;
+    int fwd_tb = 4, fwd_nw = 1, fwd_grid = 0;
 };
 
 struct Items {
@@ -48,7 +48,7 @@ struct Items {
 
 struct Group {
     int device = 0;
-    int64_t n = 0, ctot = 0, gtot = 0;
+    int64_t n = 0, ctot = 0, gtot = 0, max_bw = 0;
     std::vector<int64_t> coff, goff;
     std::vector<Seg> segs;
     MixSegment *dsegs = nullptr;
@@ -149,9 +149,9 @@ int build_items(const Group &G, bool inverse, int TB, int NW, std::vector<int4> 
             if (inverse) {
                 b.nch = (N + 63) / 64;
                 b.cost = (double)(L + 8);
-            } else {
-                b.nch = (L + 63) / 64;
-                b.cost = (double)N * (std::min(L, 64) + 24);
+            } else {   // one item per unit: every beta and degree
+                b.nch = 1;
+                b.cost = (double)N * (L + 8);
             }
             bundles.push_back(std::move(b));
         }
@@ -161,7 +161,7 @@ int build_items(const Group &G, bool inverse, int TB, int NW, std::vector<int4> 
     for (const Bundle &b : bundles)
         for (int c = 0; c < b.nch; ++c)
             for (int k = 0; k < 8; ++k) {
-                if (k < (int)b.u.size() && (inverse || 64 * c < b.u[k].L))
+                if (k < (int)b.u.size())
                     out.push_back(make_int4(b.seg, b.u[k].pq, b.u[k].t0, c));
                 else
                     out.push_back(make_int4(-1, 0, 0, 0));
@@ -174,7 +174,7 @@ int build_items(const Group &G, bool inverse, int TB, int NW, std::vector<int4> 
         const Seg &sg = G.segs[w.x];
         const int p = w.y & 0xffff, q = w.y >> 16;
         const bool ok = p <= q && q < sg.bw && w.z >= 0 && w.z < sg.T && w.w >= 0 &&
-                        (inverse ? 64 * w.w < sg.N : 64 * w.w < sg.bw - q);
+                        (inverse ? 64 * w.w < sg.N : w.w == 0);
         if (!ok) return set_error(SO3FT_EINVAL, "internal: grouped work item out of range");
     }
     return SO3FT_OK;
@@ -235,8 +235,8 @@ int launch_inv_dwt(Group &G, const Config &c, const Items &it, const float2 *coe
 
 int launch_fwd_dwt(Group &G, const Config &c, const Items &it, float2 *coeff, hipStream_t s)
 {
-    GCHK(launch_dwt_forward_mixed_grouped(c.fwd_tb, c.fwd_nw, G.F, coeff, G.dsegs, it.d, it.n,
-                                          grid_of(c.fwd_grid, it.n), s));
+    GCHK(launch_dwt_forward_mixed_grouped(c.fwd_tb, G.F, coeff, G.dsegs, it.d, it.n, grid_of(c.fwd_grid, it.n),
+                                          (int)G.max_bw, s));
     return SO3FT_OK;
 }
 
@@ -383,7 +383,7 @@ int tune(Group &G, hipStream_t s)
         std::vector<int4> v;
         Items it;
         if ((rc = build_items(G, true, TB, 1, v)) || (rc = upload_items(v, it))) break;
-        for (int g : grid_candidates(mixed_grouped_occupancy(1, TB, 1))) {
+        for (int g : grid_candidates(mixed_grouped_occupancy(1, TB, 0))) {
             Config c = G.cfg;
             c.inv_tb = TB;
             c.inv_grid = g;
@@ -394,20 +394,19 @@ int tune(Group &G, hipStream_t s)
         hipFree(it.d);
     }
     tbest = 1e30f;
-    const int shapes[4][2] = {{2, 4}, {1, 4}, {1, 2}, {1, 1}};
-    for (auto &sh : shapes) {
+    for (int TB : {4, 2, 1}) {
         if (rc) break;
         std::vector<int4> v;
         Items it;
-        if ((rc = build_items(G, false, sh[0], sh[1], v)) || (rc = upload_items(v, it))) break;
-        for (int g : grid_candidates(mixed_grouped_occupancy(0, sh[0], sh[1]))) {
+        if ((rc = build_items(G, false, TB, 1, v)) || (rc = upload_items(v, it))) break;
+        for (int g : grid_candidates(mixed_grouped_occupancy(0, TB, (int)G.max_bw))) {
             Config c = G.cfg;
-            c.fwd_tb = sh[0];
-            c.fwd_nw = sh[1];
+            c.fwd_tb = TB;
+            c.fwd_nw = 1;
             c.fwd_grid = g;
             const float ms = time_launches(s, e0, e1, [&] { return launch_fwd_dwt(G, c, it, ctmp, s); }, &rc);
             if (rc) break;
-            if (ms < tbest) { tbest = ms; best.fwd_tb = sh[0]; best.fwd_nw = sh[1]; best.fwd_grid = g; }
+            if (ms < tbest) { tbest = ms; best.fwd_tb = TB; best.fwd_nw = 1; best.fwd_grid = g; }
         }
         hipFree(it.d);
     }
@@ -481,6 +480,7 @@ int so3ft_group_create(const int64_t *bws, int64_t n, int64_t *out)
     Group *G = new Group();
     G->n = n;
     layout(bws, n, G->segs, G->coff, G->goff, G->ctot, G->gtot);
+    G->max_bw = G->segs.front().bw;   // segments are in decreasing bandwidth
     int rc = SO3FT_OK;
     if (hipGetDevice(&G->device) != hipSuccess) rc = set_error(SO3FT_EHIP, "hipGetDevice");
     std::vector<MixSegment> ms;
@@ -574,9 +574,9 @@ int so3ft_group_set_config(int64_t h, const int32_t *cfg)
     Config c;
     c.inv_tb = cfg[0]; c.inv_grid = cfg[1]; c.fwd_tb = cfg[2]; c.fwd_nw = cfg[3]; c.fwd_grid = cfg[4];
     const bool inv_ok = c.inv_tb == 1 || c.inv_tb == 2;
-    const bool fwd_ok = (c.fwd_tb == 2 && c.fwd_nw == 4) || (c.fwd_tb == 1 && (c.fwd_nw == 1 || c.fwd_nw == 2 || c.fwd_nw == 4));
+    const bool fwd_ok = (c.fwd_tb == 1 || c.fwd_tb == 2 || c.fwd_tb == 4) && c.fwd_nw == 1;
     if (!inv_ok || !fwd_ok || c.inv_grid < 0 || c.fwd_grid < 0 || (c.inv_grid % 8) || (c.fwd_grid % 8))
-        return set_error(SO3FT_EINVAL, "config: inverse TB in {1,2}; forward (TB,NW) in {(2,4),(1,4),(1,2),(1,1)}; "
+        return set_error(SO3FT_EINVAL, "config: inverse TB in {1,2}; forward TB in {1,2,4} with NW = 1; "
                                        "grids 0 (one block per item) or a positive multiple of 8");
     G->cfg = c;
     if (int rc = set_items(*G)) return rc;

@@ -63,18 +63,20 @@ def test_grouped_every_launch_shape_same_result():
     item or a persistent grid looping over the work list) gives identical results."""
     from pysofft_amd.grouped import SoftGroup
     g = SoftGroup(BWS_SMALL)
-    g.config = dict(inv_tb=2, inv_grid=0, fwd_tb=2, fwd_nw=4, fwd_grid=0)
+    g.config = dict(inv_tb=2, inv_grid=0, fwd_tb=4, fwd_nw=1, fwd_grid=0)
     c = _coeffs(g, 2)
     f0 = g.isoft(c).clone()
     c0 = g.soft(f0).clone()
     for inv_tb in (1, 2):
-        for (tb, nw) in ((2, 4), (1, 4), (1, 2), (1, 1)):
+        for tb in (4, 2, 1):
             for grid in (0, 64):
-                g.config = dict(inv_tb=inv_tb, inv_grid=grid, fwd_tb=tb, fwd_nw=nw, fwd_grid=grid)
+                g.config = dict(inv_tb=inv_tb, inv_grid=grid, fwd_tb=tb, fwd_nw=1, fwd_grid=grid)
                 assert torch.equal(g.isoft(c), f0), (inv_tb, grid)
-                assert torch.equal(g.soft(f0), c0), (tb, nw, grid)
+                assert torch.equal(g.soft(f0), c0), (tb, grid)
+    with pytest.raises(Exception):
+        g.config = dict(inv_tb=2, inv_grid=0, fwd_tb=2, fwd_nw=4, fwd_grid=0)
     cfg = g.autotune()
-    assert cfg["tuned"] == 1 and cfg["inv_tb"] in (1, 2) and cfg["fwd_nw"] in (1, 2, 4)
+    assert cfg["tuned"] == 1 and cfg["inv_tb"] in (1, 2) and cfg["fwd_tb"] in (1, 2, 4) and cfg["fwd_nw"] == 1
     assert torch.equal(g.isoft(c), f0) and torch.equal(g.soft(f0), c0)
 
 
