@@ -126,19 +126,29 @@ __global__ __launch_bounds__(256) void fft_reg_kernel(const void *__restrict__ i
         if (k.sign > 0) w.y = -w.y;
         tw[i] = w;
     }
-    // step-1 rows of tile jt for thread (n2 = grp, c): rows R2*n1 + n2, masked rows zero
+    // step-1 rows of tile jt for thread (n2 = grp, c): rows R2*n1 + n2, masked rows zero.
+    // Row n1 sits at element n1 * (R2 in_r) past the thread's first row: 32-bit offsets
+    // from one per-tile pointer (in_r N < 2^31 for N <= 512), the row mask computed once.
+    uint32_t rowmask = 0;
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) {
+        const int r = R2 * n1 + grp;
+        if (grp < R2 && (!k.mask_rows || order_ok2(r, N, k.bw, k.lmax))) rowmask |= 1u << n1;
+    }
+    const uint32_t rstep = (uint32_t)(R2 * k.in_r);
     auto load = [&](TI (&dst)[R1], int jt) {
         const int cb = k.col0 + jt * C;
         const bool colok = jt < ntile && (cb + c) < (k.col0 + k.ncols);
-        const int64_t ib = t * k.in_t + o * k.in_o + cb + c;
+        const int64_t ib = t * k.in_t + o * k.in_o + cb + c + (int64_t)grp * k.in_r;
+        const TI *pb = reinterpret_cast<const TI *>(in) + ib;
+        const typename Scalar<TI>::type *ps = reinterpret_cast<const typename Scalar<TI>::type *>(in) + ib;
 #pragma unroll
         for (int n1 = 0; n1 < R1; ++n1) {
-            const int r = R2 * n1 + grp;
             TI x = zero_of<TI>();
-            if (grp < R2 && colok && (!k.mask_rows || order_ok2(r, N, k.bw, k.lmax))) {
-                const int64_t off = ib + (int64_t)r * k.in_r;
-                if (k.in_real) x.x = reinterpret_cast<const typename Scalar<TI>::type *>(in)[off];
-                else x = reinterpret_cast<const TI *>(in)[off];
+            if (colok && ((rowmask >> n1) & 1u)) {
+                const uint32_t off = (uint32_t)n1 * rstep;
+                if (k.in_real) x.x = ps[off];
+                else x = pb[off];
             }
             dst[n1] = x;
         }
@@ -171,14 +181,17 @@ __global__ __launch_bounds__(256) void fft_reg_kernel(const void *__restrict__ i
             for (int n2 = 0; n2 < R2; ++n2) v[n2] = Y[(k1 * R2 + n2) * C + c];
             Dft<R2>::run(v, tw, N / R2, k.sign);
             if (colok) {
-                const int64_t ob = t * k.out_t + o * k.out_o + cb + c;
+                // output row k1 + R1 k2: 32-bit offsets from the thread's first row
+                const int64_t ob = t * k.out_t + o * k.out_o + cb + c + (int64_t)k1 * k.out_r;
+                const uint32_t ostep = (uint32_t)(R1 * k.out_r);
+                TO *po = reinterpret_cast<TO *>(out) + ob;
+                typename Scalar<TO>::type *pr = reinterpret_cast<typename Scalar<TO>::type *>(out) + ob;
 #pragma unroll
                 for (int k2 = 0; k2 < R2; ++k2) {
-                    const int64_t off = ob + (int64_t)(k1 + R1 * k2) * k.out_r;
+                    const uint32_t off = (uint32_t)k2 * ostep;
                     const double2 x = make_double2(v[k2].x * k.scale, v[k2].y * k.scale);
-                    if (k.out_real)
-                        reinterpret_cast<typename Scalar<TO>::type *>(out)[off] = (typename Scalar<TO>::type)x.x;
-                    else st<TO>(reinterpret_cast<TO *>(out), off, x);
+                    if (k.out_real) pr[off] = (typename Scalar<TO>::type)x.x;
+                    else st<TO>(po, off, x);
                 }
             }
         }
