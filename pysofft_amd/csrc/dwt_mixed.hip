@@ -251,7 +251,7 @@ __global__ __launch_bounds__(64) void dwt_inverse_mix_grouped(const float2 *__re
 // ---------------------------------------------------------------------------
 constexpr int FT_DS = 66;   // padded row of the forward tile (float2 entries)
 
-template <int TB>
+template <int TB, int MT>
 __device__ __forceinline__ void fwd_mix2_item(const float2 *__restrict__ F, int64_t f_ts,
                                               float2 *__restrict__ coeff, int64_t c_ts, int T,
                                               const double *__restrict__ w, const double *__restrict__ cosb,
@@ -259,6 +259,8 @@ __device__ __forceinline__ void fwd_mix2_item(const float2 *__restrict__ F, int6
                                               int bw, int N, int p, int q, int t0, double *cst,
                                               float2 (*Dt)[FT_DS], double2 (*rs)[64])
 {
+    constexpr int TD = 16 * MT;   // degrees per tile: MT row tiles of 16 share each B operand
+    constexpr int KS = MT == 1 ? 4 : 2;   // independent accumulators per (row tile, transform)
     const int lane = threadIdx.x & 63;
     const int L = bw - q;
     const int nc = (N + 63) >> 6;
@@ -286,12 +288,14 @@ __device__ __forceinline__ void fwd_mix2_item(const float2 *__restrict__ F, int6
     __syncthreads();   // cst, rs (one wave: orders the LDS writes before the reads)
 
 #pragma unroll 1
-    for (int l0 = 0; l0 < L; l0 += 16) {
-        double accd[TB][4];
+    for (int l0 = 0; l0 < L; l0 += TD) {
+        double accd[MT][TB][4];
 #pragma unroll
-        for (int t = 0; t < TB; ++t)
+        for (int m = 0; m < MT; ++m)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) accd[t][r] = 0.0;
+            for (int t = 0; t < TB; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) accd[m][t][r] = 0.0;
 #pragma unroll 1
         for (int c = 0; c < nc; ++c) {
             // B operands of this chunk: the sample of column j at beta 64 c + 4 s + kk
@@ -303,7 +307,7 @@ __device__ __forceinline__ void fwd_mix2_item(const float2 *__restrict__ F, int6
 #pragma unroll
                 for (int t = 0; t < TB; ++t) bv[s4][t] = src[t][o];
             }
-            // w d (hi, lo) for degrees l0 .. l0 + 15 at this lane's beta of chunk c
+            // w d (hi, lo) for degrees l0 .. l0 + TD - 1 at this lane's beta of chunk c
             const int beta = 64 * c + lane;
             const bool ok = beta < N;
             const double cb = ok ? cosb[beta] : 0.0, wt = ok ? w[beta] : 0.0;
@@ -311,7 +315,7 @@ __device__ __forceinline__ void fwd_mix2_item(const float2 *__restrict__ F, int6
             double2 st = rs[c][lane];
             double dcur = st.x, dprev = st.y;
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
+            for (int k = 0; k < TD; ++k) {
                 const int li = l0 + k;
                 float hi, lo;
                 split(li < L ? wt * dcur : 0.0, hi, lo);
@@ -325,29 +329,54 @@ __device__ __forceinline__ void fwd_mix2_item(const float2 *__restrict__ F, int6
             }
             rs[c][lane] = make_double2(dcur, dprev);
             asm volatile("" ::: "memory");
-            // four independent accumulators per transform (every fourth K-step), 16 betas
-            // each: no back-to-back dependent MFMAs
-            f4_t acc[TB][4];
+            // independent accumulators per (row tile, transform): K-steps split KS ways,
+            // 16 betas each at most: no back-to-back dependent MFMAs
+            f4_t acc[MT][TB][KS];
 #pragma unroll
-            for (int t = 0; t < TB; ++t)
+            for (int m = 0; m < MT; ++m)
 #pragma unroll
-                for (int g = 0; g < 4; ++g) acc[t][g] = f4_t{0.f, 0.f, 0.f, 0.f};
+                for (int t = 0; t < TB; ++t)
+#pragma unroll
+                    for (int g = 0; g < KS; ++g) acc[m][t][g] = f4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int s4 = 0; s4 < 16; ++s4) {
-                const float2 a = Dt[j][4 * s4 + kk];
+                float2 a[MT];
 #pragma unroll
-                for (int t = 0; t < TB; ++t) acc[t][s4 & 3] = mma32(a.x, bv[s4][t], acc[t][s4 & 3]);
+                for (int m = 0; m < MT; ++m) a[m] = Dt[16 * m + j][4 * s4 + kk];
+                const int g = s4 % KS;
 #pragma unroll
-                for (int t = 0; t < TB; ++t) acc[t][s4 & 3] = mma32(a.y, bv[s4][t], acc[t][s4 & 3]);
+                for (int m = 0; m < MT; ++m)
+#pragma unroll
+                    for (int t = 0; t < TB; ++t) acc[m][t][g] = mma32(a[m].x, bv[s4][t], acc[m][t][g]);
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+#pragma unroll
+                    for (int t = 0; t < TB; ++t) acc[m][t][g] = mma32(a[m].y, bv[s4][t], acc[m][t][g]);
+                if (KS == 2 && (s4 & 7) == 7) {   // KS = 2: flush every 8 K-steps (16 betas each)
+#pragma unroll
+                    for (int m = 0; m < MT; ++m)
+#pragma unroll
+                        for (int t = 0; t < TB; ++t)
+#pragma unroll
+                            for (int gg = 0; gg < KS; ++gg) {
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) accd[m][t][r] += (double)acc[m][t][gg][r];
+                                acc[m][t][gg] = f4_t{0.f, 0.f, 0.f, 0.f};
+                            }
+                }
             }
+            if (KS == 4) {
 #pragma unroll
-            for (int t = 0; t < TB; ++t)
+                for (int m = 0; m < MT; ++m)
 #pragma unroll
-                for (int g = 0; g < 4; ++g)
+                    for (int t = 0; t < TB; ++t)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) accd[t][r] += (double)acc[t][g][r];
+                        for (int gg = 0; gg < KS; ++gg)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) accd[m][t][r] += (double)acc[m][t][gg][r];
+            }
         }
-        // item r of lane l: degree row 4(l>>4) + r of the tile, column j = l & 15
+        // item r of lane l: degree row 4(l>>4) + r of row tile m, column j = l & 15
         if (vr.valid) {
             const int64_t o = slice_m(slice, vr.m, vr.n, bw);
 #pragma unroll
@@ -355,20 +384,22 @@ __device__ __forceinline__ void fwd_mix2_item(const float2 *__restrict__ F, int6
                 if (t >= nt) continue;
                 float *ct = reinterpret_cast<float *>(coeff + (int64_t)(t0 + t) * c_ts + o) + partj;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int li = l0 + 4 * kk + r;
-                    if (li < L) {
-                        const int l = q + li;
-                        const double sg = vr.sign * ((vr.lsign && (l & 1)) ? -1.0 : 1.0);
-                        ct[2 * (int64_t)li] = (float)(sg * accd[t][r]);
+                for (int m = 0; m < MT; ++m)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int li = l0 + 16 * m + 4 * kk + r;
+                        if (li < L) {
+                            const int l = q + li;
+                            const double sg = vr.sign * ((vr.lsign && (l & 1)) ? -1.0 : 1.0);
+                            ct[2 * (int64_t)li] = (float)(sg * accd[m][t][r]);
+                        }
                     }
-                }
             }
         }
     }
 }
 
-template <int TB>
+template <int TB, int MT>
 __global__ __launch_bounds__(64) void dwt_forward_mix2(const float2 *__restrict__ F, int64_t f_ts,
                                                        float2 *__restrict__ coeff, int64_t c_ts, int T,
                                                        const double *__restrict__ w, const double *__restrict__ cosb,
@@ -377,21 +408,21 @@ __global__ __launch_bounds__(64) void dwt_forward_mix2(const float2 *__restrict_
                                                        const int32_t *__restrict__ pairs, int bw, int N, int npair,
                                                        int nunit)
 {
-    // dynamic LDS: Dt [16][FT_DS] float2 | rs [nc][64] double2 | cst [3 (bw - 1)] double
+    // dynamic LDS: Dt [16 MT][FT_DS] float2 | rs [nc][64] double2 | cst [3 (bw - 1)] double
     extern __shared__ double smx[];
     float2 (*Dt)[FT_DS] = reinterpret_cast<float2 (*)[FT_DS]>(smx);
-    double2 (*rs)[64] = reinterpret_cast<double2 (*)[64]>(smx + 16 * FT_DS);
-    double *cst = smx + 16 * FT_DS + 2 * 64 * ((N + 63) >> 6);
+    double2 (*rs)[64] = reinterpret_cast<double2 (*)[64]>(smx + 16 * MT * FT_DS);
+    double *cst = smx + 16 * MT * FT_DS + 2 * 64 * ((N + 63) >> 6);
     const int unit = blockIdx.x;
     if (unit >= nunit) return;
     const int pq = pairs[unit % npair];
-    fwd_mix2_item<TB>(F, f_ts, coeff, c_ts, T, w, cosb, seeds, slice, bw, N, pq & 0xffff, pq >> 16,
-                      (unit / npair) * TB, cst, Dt, rs);
+    fwd_mix2_item<TB, MT>(F, f_ts, coeff, c_ts, T, w, cosb, seeds, slice, bw, N, pq & 0xffff, pq >> 16,
+                          (unit / npair) * TB, cst, Dt, rs);
 }
 
 // grouped forward over transforms of several bandwidths (see dwt_inverse_mix_grouped);
 // dynamic LDS sized for the largest bandwidth of the group
-template <int TB>
+template <int TB, int MT>
 __global__ __launch_bounds__(64) void dwt_forward_mix2_grouped(const float2 *__restrict__ F,
                                                                float2 *__restrict__ coeff,
                                                                const MixSegment *__restrict__ segs,
@@ -404,19 +435,20 @@ __global__ __launch_bounds__(64) void dwt_forward_mix2_grouped(const float2 *__r
         const MixSegment sg = segs[wk.x];
         __syncthreads();  // the previous item's LDS reads are done
         float2 (*Dt)[FT_DS] = reinterpret_cast<float2 (*)[FT_DS]>(smx);
-        double2 (*rs)[64] = reinterpret_cast<double2 (*)[64]>(smx + 16 * FT_DS);
-        double *cst = smx + 16 * FT_DS + 2 * 64 * ((sg.N + 63) >> 6);
-        fwd_mix2_item<TB>(F + sg.goff, (int64_t)sg.N * sg.N * sg.N, coeff + sg.coff, sg.nc, sg.T, sg.w, sg.cosb,
-                          sg.seeds, sg.slice, sg.bw, sg.N, wk.y & 0xffff, wk.y >> 16, wk.z, cst, Dt, rs);
+        double2 (*rs)[64] = reinterpret_cast<double2 (*)[64]>(smx + 16 * MT * FT_DS);
+        double *cst = smx + 16 * MT * FT_DS + 2 * 64 * ((sg.N + 63) >> 6);
+        fwd_mix2_item<TB, MT>(F + sg.goff, (int64_t)sg.N * sg.N * sg.N, coeff + sg.coff, sg.nc, sg.T, sg.w, sg.cosb,
+                              sg.seeds, sg.slice, sg.bw, sg.N, wk.y & 0xffff, wk.y >> 16, wk.z, cst, Dt, rs);
     }
 }
 
 static int grid_m(int nunit, int nch) { return (nunit + 7) / 8 * 8 * nch; }
 
-// dynamic LDS bytes of dwt_forward_mix2 (A tile, chunk states, recurrence constants)
-static size_t fwd2_lds(int bw, int N)
+// dynamic LDS bytes of dwt_forward_mix2 (A tile of 16 MT degrees, chunk states,
+// recurrence constants)
+static size_t fwd2_lds(int bw, int N, int MT)
 {
-    return sizeof(double) * (16 * FT_DS + 2 * 64 * ((N + 63) / 64) + 3 * (bw > 1 ? bw - 1 : 1));
+    return sizeof(double) * (16 * MT * FT_DS + 2 * 64 * ((N + 63) / 64) + 3 * (bw > 1 ? bw - 1 : 1));
 }
 
 hipError_t launch_dwt_inverse_mixed(const DeviceTables &tb, int bw, int N, int npair, int mode, const float2 *coeff,
@@ -442,17 +474,17 @@ hipError_t launch_dwt_forward_mixed(const DeviceTables &tb, int bw, int N, int n
 {
     if (mode != DWT_COMPLEX) return hipErrorInvalidValue;
     if (npair <= 0 || T <= 0) return hipSuccess;
-#define SO3_MFWD2(TBV)                                                                                         \
+#define SO3_MFWD2(TBV, MTV)                                                                                    \
     do {                                                                                                       \
         const int nunit = npair * ((T + TBV - 1) / TBV);                                                       \
-        hipLaunchKernelGGL((dwt_forward_mix2<TBV>), dim3(nunit), dim3(64), fwd2_lds(bw, N), s, F, f_ts, coeff,   \
-                           c_ts, T, tb.w,                                                                      \
-                           tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, npair, nunit);                       \
+        hipLaunchKernelGGL((dwt_forward_mix2<TBV, MTV>), dim3(nunit), dim3(64), fwd2_lds(bw, N, MTV), s, F, f_ts, \
+                           coeff, c_ts, T, tb.w, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, npair, nunit);   \
     } while (0)
-    // four transforms per wave from bw = 96 (more MFMA work per recurrence step), else two
-    if (bw >= 96 && T >= 4) SO3_MFWD2(4);
-    else if (T >= 2) SO3_MFWD2(2);
-    else SO3_MFWD2(1);
+    // four transforms per wave from bw = 96 (more MFMA work per recurrence step), else two;
+    // one 16-degree row tile per B operand load (measured faster than two)
+    if (bw >= 96 && T >= 4) SO3_MFWD2(4, 1);
+    else if (T >= 2) SO3_MFWD2(2, 1);
+    else SO3_MFWD2(1, 1);
 #undef SO3_MFWD2
     return hipGetLastError();
 }
@@ -477,13 +509,13 @@ hipError_t launch_dwt_forward_mixed_grouped(int TB, const float2 *F, float2 *coe
 {
     if (nitem <= 0) return hipSuccess;
     if (grid <= 0) return hipErrorInvalidValue;
-    const size_t lds = fwd2_lds(max_bw, 2 * max_bw);
+    const size_t lds = fwd2_lds(max_bw, 2 * max_bw, 1);
     if (TB == 4)
-        hipLaunchKernelGGL((dwt_forward_mix2_grouped<4>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
+        hipLaunchKernelGGL((dwt_forward_mix2_grouped<4, 1>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
     else if (TB == 2)
-        hipLaunchKernelGGL((dwt_forward_mix2_grouped<2>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
+        hipLaunchKernelGGL((dwt_forward_mix2_grouped<2, 1>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
     else if (TB == 1)
-        hipLaunchKernelGGL((dwt_forward_mix2_grouped<1>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
+        hipLaunchKernelGGL((dwt_forward_mix2_grouped<1, 1>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -498,10 +530,10 @@ int mixed_grouped_occupancy(int inverse, int TB, int max_bw)
         if (TB == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_inverse_mix_grouped<2>, 64, 0);
         else if (TB == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_inverse_mix_grouped<1>, 64, 0);
     } else {
-        const size_t lds = fwd2_lds(max_bw, 2 * max_bw);
-        if (TB == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix2_grouped<4>, 64, lds);
-        else if (TB == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix2_grouped<2>, 64, lds);
-        else if (TB == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix2_grouped<1>, 64, lds);
+        const size_t lds = fwd2_lds(max_bw, 2 * max_bw, 1);
+        if (TB == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix2_grouped<4, 1>, 64, lds);
+        else if (TB == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix2_grouped<2, 1>, 64, lds);
+        else if (TB == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix2_grouped<1, 1>, 64, lds);
     }
     if (e != hipSuccess) {
         (void)hipGetLastError();

@@ -112,14 +112,27 @@ __global__ void gelu_bwd_kernel(const float *__restrict__ dA, const float *__res
     if (dZf) dZf[i] = g;
 }
 
-// bias gradient: db[n] = sum_m dZ[m][n]
-__global__ void colsum_kernel(const float *__restrict__ dZ, int M, int N, float *__restrict__ db)
+// bias gradient: db[n] = sum_m dZ[m][n].  A 1024-thread block per 64 columns: wave w
+// sums rows w, w + 16, ... of its 64 columns (256-byte rows), then the 16 partial sums
+// are added in a fixed order (deterministic).
+__global__ __launch_bounds__(1024) void colsum_kernel(const float *__restrict__ dZ, int M, int N,
+                                                       float *__restrict__ db)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+    __shared__ float part[16][64];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + c;
     float s = 0.f;
-    for (int m = 0; m < M; ++m) s += dZ[(int64_t)m * N + n];
-    db[n] = s;
+    if (n < N) {
+#pragma unroll 4
+        for (int m = g; m < M; m += 16) s += dZ[(int64_t)m * N + n];
+    }
+    part[g][c] = s;
+    __syncthreads();
+    if (g == 0 && n < N) {
+        float t = 0.f;
+        for (int k = 0; k < 16; ++k) t += part[k][c];
+        db[n] = t;
+    }
 }
 
 __global__ void f32_to_bf16_kernel(const float *__restrict__ x, uint16_t *__restrict__ y, int64_t n)
@@ -534,7 +547,7 @@ int so3ft_mlp_train_step(int64_t h, const float *x, const float *T9, int64_t n, 
         const int K = P->dims[l], N = P->dims[l + 1];
         // dW (N x K) = dZ^T A: A(m = out, k = sample) = dZ[k N + m], B(k = sample, n = in) = Ab[k K + n]
         MHIP(gemm(P->dZb, 1, N, P->Ab[l], K, 1, N, K, (int)n, nullptr, P->gW[l], nullptr, 0, 0, s));
-        hipLaunchKernelGGL(colsum_kernel, dim3((N + 255) / 256), dim3(256), 0, s, dZf, (int)n, N, P->gb[l]);
+        hipLaunchKernelGGL(colsum_kernel, dim3((N + 63) / 64), dim3(1024), 0, s, dZf, (int)n, N, P->gb[l]);
         MHIP(hipGetLastError());
         if (l == 0) break;
         // dA (n x K) = dZ W: A(m, k) = dZ[m N + k], B(k = out, n = in) = Wb[k K + n]
