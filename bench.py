@@ -171,7 +171,7 @@ def main():
     ap.add_argument("--bw", type=int, default=128)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=0, help="transforms in the CPU sample (0 = 2 x threads)")
+    ap.add_argument("--cpu-sample", type=int, default=0, help="transforms in the CPU sample (0 = 8 x threads)")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing section")
     ap.add_argument("--no-align", action="store_true", help="skip the alignment workload")
     ap.add_argument("--align-pairs", type=int, default=1024)
@@ -344,7 +344,7 @@ def main():
     if not args.no_cpu:
         import oracle as O
         nthr = O.num_threads()
-        ns = args.cpu_sample or 2 * nthr
+        ns = args.cpu_sample or 8 * nthr
         P = O.OraclePlan(bw)
         rng = np.random.default_rng(5)
         cc = rng.standard_normal((ns, P.nc)) + 1j * rng.standard_normal((ns, P.nc))
