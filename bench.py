@@ -152,6 +152,7 @@ def ragged_batch(args, dev, timed):
         c64 = plans[b].soft(fv[i].to(torch.complex128))
         e[1] = max(e[1], ((rv[i].to(torch.complex128) - c64).norm() / c64.norm()).item())
         e[2] = max(e[2], ((rv[i] - cv[i]).norm() / cv[i].norm()).item())
+    cfg = G.config   # with the schedules chosen on the first steps
     print(f"# ragged: grouped {ms:.3f} ms/step, bandwidth by bandwidth {ms1:.3f} ms/step", file=sys.stderr,
           flush=True)
     return {"workload": f"{len(bws)} complex64 transforms, bw uniform over {{32,48,64,96,128}}, coefficients iid "

@@ -107,10 +107,13 @@ int so3ft_soft_many_c64(so3ft_handle h, const void *so3funcs, void *coeffs, int6
  * the group's shapes, synchronising the stream; the choice is kept per (device,
  * bandwidth multiset).  Precision and semantics as so3ft_isoft_many_c64 /
  * so3ft_soft_many_c64.  A group may be used from one stream at a time.
- * config (6 int32): inverse TB, inverse grid, forward TB, forward NW (waves per block:
+ * config (8 int32): inverse TB, inverse grid, forward TB, forward NW (waves per block:
  * 1), forward grid (grid 0 = one block per work item, else a multiple of 8 blocks
- * looping over the work list), tuned flag (get only); TB = transforms per wave, inverse
- * {1, 2}, forward {1, 2, 4}. */
+ * looping over the work list), tuned flag (get only), inverse schedule, forward schedule;
+ * TB = transforms per wave, inverse {1, 2}, forward {1, 2, 4}; schedule 0 = one grouped DWT
+ * launch then the bandwidths' FFT passes as parallel graph branches, 1 = one graph branch
+ * per bandwidth holding its DWT and FFT passes (its forward DWT with 4 transforms per wave
+ * from bw = 96, else 2), -1 = both timed on first use (in a graph), the faster kept. */
 int so3ft_group_layout(const int64_t *bws, int64_t n, int64_t *coeff_off, int64_t *grid_off,
                        int64_t *coeff_total, int64_t *grid_total);
 int so3ft_group_create(const int64_t *bws, int64_t n, int64_t *group);

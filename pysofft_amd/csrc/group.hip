@@ -55,7 +55,14 @@ struct Group {
     float2 *F = nullptr;                   // DWT <-> FFT intermediate, gtot entries
     Config cfg;
     bool tuned = false, graphs = true;
-    Items inv, fwd;
+    Items inv, fwd;                        // work lists over every segment
+    std::vector<Items> inv_seg, fwd_seg;   // one list per segment
+    std::vector<int> fwd_seg_tb;           // forward transforms per wave of each segment's list
+    // schedule per direction ([0] forward, [1] inverse): 0 = one grouped DWT launch and the
+    // segments' FFT passes as parallel branches; 1 = one branch per segment holding its DWT
+    // and FFT passes (a segment's FFT overlaps another's DWT); -1 = choose by timing
+    int sched[2] = {-1, -1};
+    float sched_ms[2][2] = {{0.f, 0.f}, {0.f, 0.f}};   // measured graph time per schedule
     hipStream_t cap = nullptr;
     std::vector<hipStream_t> side;
     hipEvent_t ev_fork = nullptr;
@@ -127,13 +134,14 @@ void layout(const int64_t *bws, int64_t n, std::vector<Seg> &segs, std::vector<i
 // chunks), so the chunks of one unit sit at list positions that agree mod 8 and, with a
 // grid that is a multiple of 8, run on one XCD (shared L2 for the unit's operands).
 // Bundles of all segments are ordered by decreasing cost (largest first for balance).
-int build_items(const Group &G, bool inverse, int TB, int NW, std::vector<int4> &out)
+int build_items(const Group &G, bool inverse, int TB, int NW, std::vector<int4> &out, int only_seg = -1)
 {
     struct Unit { int pq, L, t0; };
     struct Bundle { double cost; int seg; std::vector<Unit> u; int nch; };
     std::vector<Bundle> bundles;
     const int per = inverse ? TB : TB * NW;
     for (int s = 0; s < (int)G.segs.size(); ++s) {
+        if (only_seg >= 0 && s != only_seg) continue;
         const Seg &sg = G.segs[s];
         const int bw = (int)sg.bw, N = (int)sg.N;
         std::vector<Unit> units;
@@ -198,7 +206,21 @@ int set_items(Group &G)
     if (int rc = build_items(G, true, G.cfg.inv_tb, 1, v)) return rc;
     if (int rc = upload_items(v, G.inv)) return rc;
     if (int rc = build_items(G, false, G.cfg.fwd_tb, G.cfg.fwd_nw, v)) return rc;
-    return upload_items(v, G.fwd);
+    if (int rc = upload_items(v, G.fwd)) return rc;
+    G.inv_seg.resize(G.segs.size());
+    G.fwd_seg.resize(G.segs.size());
+    G.fwd_seg_tb.assign(G.segs.size(), G.cfg.fwd_tb);
+    for (int k = 0; k < (int)G.segs.size(); ++k) {
+        if (int rc = build_items(G, true, G.cfg.inv_tb, 1, v, k)) return rc;
+        if (int rc = upload_items(v, G.inv_seg[k])) return rc;
+        // a segment's own forward launch takes four transforms per wave from bw = 96 (as the
+        // single-bandwidth path), else two
+        const int tb = G.segs[k].bw >= 96 ? 4 : 2;
+        G.fwd_seg_tb[k] = tb;
+        if (int rc = build_items(G, false, tb, 1, v, k)) return rc;
+        if (int rc = upload_items(v, G.fwd_seg[k])) return rc;
+    }
+    return SO3FT_OK;
 }
 
 void drop_graphs(Group &G)
@@ -218,6 +240,8 @@ void free_group(Group *G)
     hipFree(G->F);
     hipFree(G->inv.d);
     hipFree(G->fwd.d);
+    for (auto &it : G->inv_seg) hipFree(it.d);
+    for (auto &it : G->fwd_seg) hipFree(it.d);
     for (auto s : G->side) hipStreamDestroy(s);
     if (G->cap) hipStreamDestroy(G->cap);
     if (G->ev_fork) hipEventDestroy(G->ev_fork);
@@ -306,12 +330,38 @@ int all_fft(Group &G, bool inverse, const float2 *in, float2 *out, hipStream_t s
 
 int enqueue(Group &G, bool inverse, const void *in, void *out, hipStream_t s, bool fork)
 {
-    if (inverse) {
-        if (int rc = launch_inv_dwt(G, G.cfg, G.inv, reinterpret_cast<const float2 *>(in), s)) return rc;
-        return all_fft(G, true, nullptr, reinterpret_cast<float2 *>(out), s, fork);
+    const float2 *fin = reinterpret_cast<const float2 *>(in);
+    float2 *fout = reinterpret_cast<float2 *>(out);
+    if (G.sched[inverse ? 1 : 0] == 1) {
+        // one branch per segment: DWT then FFT (inverse) or FFT then DWT (forward)
+        if (fork) {
+            GCHK(hipEventRecord(G.ev_fork, s));
+            for (size_t i = 1; i < G.segs.size(); ++i) GCHK(hipStreamWaitEvent(G.side[i - 1], G.ev_fork, 0));
+        }
+        for (size_t i = 0; i < G.segs.size(); ++i) {
+            hipStream_t st = (fork && i > 0) ? G.side[i - 1] : s;
+            if (inverse) {
+                if (int rc = launch_inv_dwt(G, G.cfg, G.inv_seg[i], fin, st)) return rc;
+                if (int rc = seg_fft(G, G.segs[i], true, nullptr, fout, st)) return rc;
+            } else {
+                if (int rc = seg_fft(G, G.segs[i], false, fin, nullptr, st)) return rc;
+                Config cs = G.cfg;
+                cs.fwd_tb = G.fwd_seg_tb[i];
+                if (int rc = launch_fwd_dwt(G, cs, G.fwd_seg[i], fout, st)) return rc;
+            }
+            if (fork && i > 0) {
+                GCHK(hipEventRecord(G.ev_join[i - 1], st));
+                GCHK(hipStreamWaitEvent(s, G.ev_join[i - 1], 0));
+            }
+        }
+        return SO3FT_OK;
     }
-    if (int rc = all_fft(G, false, reinterpret_cast<const float2 *>(in), nullptr, s, fork)) return rc;
-    return launch_fwd_dwt(G, G.cfg, G.fwd, reinterpret_cast<float2 *>(out), s);
+    if (inverse) {
+        if (int rc = launch_inv_dwt(G, G.cfg, G.inv, fin, s)) return rc;
+        return all_fft(G, true, nullptr, fout, s, fork);
+    }
+    if (int rc = all_fft(G, false, fin, nullptr, s, fork)) return rc;
+    return launch_fwd_dwt(G, G.cfg, G.fwd, fout, s);
 }
 
 int capture(Group &G, bool inverse, const void *in, void *out, hipGraphExec_t *exec)
@@ -434,10 +484,17 @@ int run(int64_t h, bool inverse, const void *in, void *out, void *stream)
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     GCHK(hipStreamIsCapturing(s, &cs));
-    if (cs != hipStreamCaptureStatusNone) return enqueue(*G, inverse, in, out, s, false);  // caller's own capture
+    if (cs != hipStreamCaptureStatusNone) {  // caller's own capture: launches go into the caller's graph
+        if (G->sched[inverse ? 1 : 0] < 0) G->sched[inverse ? 1 : 0] = 0;
+        return enqueue(*G, inverse, in, out, s, false);
+    }
     if (!G->tuned)
         if (int rc = tune(*G, s)) return rc;
-    if (!G->graphs) return enqueue(*G, inverse, in, out, s, false);
+    if (!G->graphs) {
+        const int keep = G->sched[inverse ? 1 : 0];
+        if (keep < 0) G->sched[inverse ? 1 : 0] = 0;
+        return enqueue(*G, inverse, in, out, s, false);
+    }
     auto &cache = inverse ? G->ginv : G->gfwd;
     const auto key = std::make_pair(in, out);
     auto itg = cache.find(key);
@@ -447,7 +504,52 @@ int run(int64_t h, bool inverse, const void *in, void *out, void *stream)
             cache.erase(cache.begin());
         }
         hipGraphExec_t ex = nullptr;
-        if (int rc = capture(*G, inverse, in, out, &ex)) return rc;
+        int &sc = G->sched[inverse ? 1 : 0];
+        if (sc < 0) {
+            // time both schedules' graphs on the caller's buffers (the output is rewritten by
+            // the launch below) and keep the faster
+            hipEvent_t e0, e1;
+            GCHK(hipEventCreate(&e0));
+            GCHK(hipEventCreate(&e1));
+            float best = 1e30f;
+            int rc = SO3FT_OK;
+            for (int cand = 0; cand < 2 && !rc; ++cand) {
+                sc = cand;
+                hipGraphExec_t cx = nullptr;
+                if ((rc = capture(*G, inverse, in, out, &cx))) break;
+                float ms = 0.f;
+                hipError_t e = hipGraphLaunch(cx, s);
+                if (e == hipSuccess) e = hipEventRecord(e0, s);
+                for (int r = 0; r < 2 && e == hipSuccess; ++r) e = hipGraphLaunch(cx, s);
+                if (e == hipSuccess) e = hipEventRecord(e1, s);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                if (e != hipSuccess) {
+                    hipGraphExecDestroy(cx);
+                    rc = set_error(SO3FT_EHIP, std::string("schedule timing: ") + hipGetErrorString(e));
+                    break;
+                }
+                if (ms < best) {
+                    best = ms;
+                    if (ex) hipGraphExecDestroy(ex);
+                    ex = cx;
+                    G->sched_ms[inverse ? 1 : 0][cand] = ms / 2;
+                } else {
+                    G->sched_ms[inverse ? 1 : 0][cand] = ms / 2;
+                    hipGraphExecDestroy(cx);
+                }
+            }
+            hipEventDestroy(e0);
+            hipEventDestroy(e1);
+            if (rc) {
+                if (ex) hipGraphExecDestroy(ex);
+                sc = -1;
+                return rc;
+            }
+            sc = G->sched_ms[inverse ? 1 : 0][1] < G->sched_ms[inverse ? 1 : 0][0] ? 1 : 0;
+        } else if (int rc = capture(*G, inverse, in, out, &ex)) {
+            return rc;
+        }
         itg = cache.emplace(key, ex).first;
     }
     GCHK(hipGraphLaunch(itg->second, s));
@@ -561,8 +663,9 @@ int so3ft_group_get_config(int64_t h, int32_t *cfg)
     if (!G) return set_error(SO3FT_EHANDLE, "unknown group handle");
     if (!cfg) return set_error(SO3FT_EINVAL, "null config array");
     const Config &c = G->cfg;
-    const int32_t v[6] = {c.inv_tb, c.inv_grid, c.fwd_tb, c.fwd_nw, c.fwd_grid, G->tuned ? 1 : 0};
-    std::copy(v, v + 6, cfg);
+    const int32_t v[8] = {c.inv_tb, c.inv_grid, c.fwd_tb, c.fwd_nw, c.fwd_grid, G->tuned ? 1 : 0, G->sched[1],
+                          G->sched[0]};
+    std::copy(v, v + 8, cfg);
     return SO3FT_OK;
 }
 
@@ -578,10 +681,14 @@ int so3ft_group_set_config(int64_t h, const int32_t *cfg)
     if (!inv_ok || !fwd_ok || c.inv_grid < 0 || c.fwd_grid < 0 || (c.inv_grid % 8) || (c.fwd_grid % 8))
         return set_error(SO3FT_EINVAL, "config: inverse TB in {1,2}; forward TB in {1,2,4} with NW = 1; "
                                        "grids 0 (one block per item) or a positive multiple of 8");
+    if (cfg[6] < -1 || cfg[6] > 1 || cfg[7] < -1 || cfg[7] > 1)
+        return set_error(SO3FT_EINVAL, "config: schedules -1 (by timing), 0 (grouped launch) or 1 (segment branches)");
     G->cfg = c;
     if (int rc = set_items(*G)) return rc;
     drop_graphs(*G);
     G->tuned = true;
+    G->sched[1] = cfg[6];
+    G->sched[0] = cfg[7];
     return SO3FT_OK;
 }
 

@@ -2,8 +2,10 @@
 
 ``SoftGroup(bws)`` fixes one bandwidth per transform.  ``isoft`` / ``soft`` run the
 whole batch with complex64 storage (FP64 recurrence and FFT arithmetic, FP32 MFMA
-contraction with d split into hi + lo parts) as one persistent grouped DWT launch plus
-per-bandwidth FFT passes, captured into a HIP graph on first use (so3ft_group_* in
+contraction with d split into hi + lo parts) as grouped DWT launches over work lists and
+per-bandwidth FFT passes on parallel branches of a HIP graph captured on first use; the
+launch shapes and the schedule (one grouped DWT launch, or one branch per bandwidth so that
+FFT passes overlap other bandwidths' DWTs) are chosen by timing (so3ft_group_* in
 include/so3ft.h).  The reference has no counterpart: its ``*_many`` calls
 (pysofft/soft.py:255-330) take one bandwidth per ``Soft`` instance.
 
@@ -124,14 +126,16 @@ class SoftGroup:
 
     @property
     def config(self):
-        """dict(inv_tb, inv_grid, fwd_tb, fwd_nw, fwd_grid, tuned)."""
-        c = np.zeros(6, np.int32)
+        """dict(inv_tb, inv_grid, fwd_tb, fwd_nw, fwd_grid, tuned, inv_sched, fwd_sched)."""
+        c = np.zeros(8, np.int32)
         _lib.check(_lib.lib().so3ft_group_get_config(self._handle, c.ctypes.data))
-        return dict(zip(("inv_tb", "inv_grid", "fwd_tb", "fwd_nw", "fwd_grid", "tuned"), map(int, c)))
+        return dict(zip(("inv_tb", "inv_grid", "fwd_tb", "fwd_nw", "fwd_grid", "tuned", "inv_sched", "fwd_sched"),
+                        map(int, c)))
 
     @config.setter
     def config(self, cfg):
-        c = np.array([cfg["inv_tb"], cfg["inv_grid"], cfg["fwd_tb"], cfg["fwd_nw"], cfg["fwd_grid"], 0], np.int32)
+        c = np.array([cfg["inv_tb"], cfg["inv_grid"], cfg["fwd_tb"], cfg["fwd_nw"], cfg["fwd_grid"], 0,
+                      cfg.get("inv_sched", -1), cfg.get("fwd_sched", -1)], np.int32)
         _lib.check(_lib.lib().so3ft_group_set_config(self._handle, c.ctypes.data))
 
     def set_graphs(self, enable):
