@@ -17,7 +17,8 @@
 // forward: one wave per (pair, TB transforms), lane = beta within 64-beta chunks; for
 //   each 16-degree tile and chunk the lanes advance their recurrence and store
 //   w(beta) d (hi, lo) in LDS as [degree][beta]; A = that tile (M = degree, K = beta),
-//   B = the fft-array samples (K = beta, N = variant x re/im).
+//   B = the fft-array samples (K = beta, N = 8 variants x 2 transforms; real and
+//   imaginary parts in separate MFMAs, so a lane loads whole complex samples).
 #include "so3ft_internal.h"
 
 namespace so3 {
@@ -43,6 +44,30 @@ __device__ inline void rec_consts_m(int l, int p, int q, double &a, double &x, d
         c1 = c1 / sqrt((double)(2 * l - 1));
         b = cc * c1;
     }
+}
+
+// recurrence constants of pair (p, q) in LDS: entry i = (a, a x, b) of the step from
+// degree q + i to q + i + 1, d^{l+1} = (a cos(beta) - a x) d^l - b d^{l-1}, for
+// 0 <= i < L - 1 (L = bw - q); entries L - 1 .. L - 2 + pad are zero, so a chain run past
+// the last degree stays at zero and needs no per-degree test.  One wave fills it.
+constexpr int CST_PAD = 16;
+constexpr int CST_CAP = 256 + CST_PAD;   // entries for bw <= 256
+__device__ __forceinline__ void fill_rec_consts(double *cst, int L, int p, int q)
+{
+    for (int i = threadIdx.x & 63; i < L - 1 + CST_PAD; i += 64) {
+        double a = 0.0, x = 0.0, b = 0.0;
+        if (i < L - 1) rec_consts_m(q + i, p, q, a, x, b);
+        cst[3 * i] = a; cst[3 * i + 1] = a * x; cst[3 * i + 2] = b;
+    }
+}
+
+// one step of the chain: (dcur, dprev) <- (d^{l+1}, d^l) with entry i's constants
+__device__ __forceinline__ void rec_step(const double *cst, int i, double cb, double &dcur, double &dprev)
+{
+    const double t = fma(cst[3 * i], cb, -cst[3 * i + 1]);
+    const double dn = fma(t, dcur, -cst[3 * i + 2] * dprev);
+    dprev = dcur;
+    dcur = dn;
 }
 
 __device__ __forceinline__ int64_t slice_m(const int64_t *slice, int m, int n, int bw)
@@ -87,18 +112,15 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
     const int vj = j >> 1, partj = j & 1;
     const Variant vr = variant(vj, p, q);
     const int64_t offj = vr.valid ? slice_m(slice, vr.m, vr.n, bw) : slice_m(slice, p, q, bw);
-    const float sgn = vr.valid ? (float)vr.sign : 0.0f;
-    const bool lsg = vr.lsign;
+    // the lane's degrees are l = q + l0 + kk with l0 a multiple of 4: the variant's
+    // (-1)^l factor is the same at every step
+    const float sgn = vr.valid ? (float)vr.sign * ((vr.lsign && ((q + kk) & 1)) ? -1.0f : 1.0f) : 0.0f;
     const float *src[TB];
 #pragma unroll
     for (int t = 0; t < TB; ++t)
         src[t] = reinterpret_cast<const float *>(coeff + (int64_t)(t0 + min(t, nt - 1)) * c_ts + offj) + partj;
 
-    for (int i = lane; i < L - 1; i += 64) {
-        double a, x, b;
-        rec_consts_m(q + i, p, q, a, x, b);
-        cst[3 * i] = a; cst[3 * i + 1] = x; cst[3 * i + 2] = b;
-    }
+    fill_rec_consts(cst, L, p, q);
     const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
     double dcur = (beta < N) ? seeds[tri * N + beta] : 0.0;
     const double cb = (beta < N) ? cosb[beta] : 0.0;
@@ -116,49 +138,37 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
         }
     __syncthreads();
 
-    // B operand of degree step l0: the signed coefficient of column j at degree l0 + kk;
-    // kept in a ring three steps deep so the loads are in flight while earlier steps run
+    // raw coefficient of column j at degree l0 + kk (clamped into the column; past-the-end
+    // degrees enter with a zero sign)
     auto loadb = [&](float (&dst)[TB], int l0) {
-        const int lk = l0 + kk;
-        const float s = (lsg && ((q + lk) & 1)) ? -sgn : sgn;
-        const int lc = min(lk, L - 1);   // clamped: past-the-end degrees enter with a zero sign
-        const float sz = (lk < L) ? s : 0.0f;
+        const int lc = min(l0 + kk, L - 1);
 #pragma unroll
-        for (int t = 0; t < TB; ++t) dst[t] = sz * src[t][2 * (int64_t)lc];
+        for (int t = 0; t < TB; ++t) dst[t] = src[t][2 * (int64_t)lc];
     };
-    float bq0[TB], bq1[TB], bq2[TB];
-    loadb(bq0, 0);
-    loadb(bq1, 4);
-    loadb(bq2, 8);
-
-    // d for degrees l0..l0+3 of this lane's beta -> tile buf (then advanced past them)
+    // d for degrees l0..l0+3 of this lane's beta -> tile buf (then advanced past them);
+    // zero past the last degree
     auto dtile = [&](int buf, int l0) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const int li = l0 + k;
             float hi, lo;
-            split(li < L ? dcur : 0.0, hi, lo);
+            split(dcur, hi, lo);
             Dl[buf][k][lane] = make_float2(hi, lo);
-            if (li < L - 1) {
-                const double ca = cst[3 * li], cx = cst[3 * li + 1], cbb = cst[3 * li + 2];
-                const double dn = (ca * (cb - cx)) * dcur - cbb * dprev;
-                dprev = dcur;
-                dcur = dn;
-            }
+            rec_step(cst, l0 + k, cb, dcur, dprev);
         }
     };
-    dtile(0, 0);
-    for (int l0 = 0, buf = 0; l0 < L; l0 += 4, buf ^= 1) {
-        // LDS operations of one wave execute in order; the empty asm keeps the compiler
-        // from moving tile reads above the writes that produced them
-        asm volatile("" ::: "memory");
+    // one 4-degree step: MFMAs on tile (l0 >> 2) & 1 with the ring slot r (loaded three
+    // steps earlier, reloaded here for step l0 + 12), interleaved with the recurrence
+    // that fills the other tile for the next step
+    auto step = [&](int l0, float (&r)[TB]) {
+        const int buf = (l0 >> 2) & 1;
         float2 a[4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) a[g] = Dl[buf][kk][16 * g + j];
+        const float sz = (l0 + kk < L) ? sgn : 0.0f;
         float bv[TB];
 #pragma unroll
-        for (int t = 0; t < TB; ++t) { bv[t] = bq0[t]; bq0[t] = bq1[t]; bq1[t] = bq2[t]; }
-        loadb(bq2, l0 + 12);
+        for (int t = 0; t < TB; ++t) bv[t] = sz * r[t];
+        loadb(r, l0 + 12);
         // all hi products, then all lo products: no back-to-back dependent MFMAs
 #pragma unroll
         for (int g = 0; g < 4; ++g)
@@ -168,19 +178,39 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
         for (int g = 0; g < 4; ++g)
 #pragma unroll
             for (int t = 0; t < TB; ++t) acc[t][g] = mma32(a[g].y, bv[t], acc[t][g]);
-        // the next step's recurrence runs on the VALU while the MFMAs execute
-        asm volatile("" ::: "memory");
-        if (l0 + 4 < L) dtile(buf ^ 1, l0 + 4);
+        dtile(buf ^ 1, l0 + 4);
+        // issue order: each MFMA followed by a few VALU instructions of the recurrence,
+        // which run while the matrix core works
+#pragma unroll
+        for (int i = 0; i < 8 * TB; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 8 / TB, 0);
+        }
         if (((l0 >> 2) & 3) == 3 || l0 + 4 >= L) {  // flush every 16 degrees
 #pragma unroll
             for (int t = 0; t < TB; ++t)
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) accd[t][g][r] += (double)acc[t][g][r];
+                    for (int rr = 0; rr < 4; ++rr) accd[t][g][rr] += (double)acc[t][g][rr];
                     acc[t][g] = f4_t{0.f, 0.f, 0.f, 0.f};
                 }
         }
+    };
+    float r0[TB], r1[TB], r2[TB];
+    loadb(r0, 0);
+    loadb(r1, 4);
+    loadb(r2, 8);
+    dtile(0, 0);
+    // LDS operations of one wave execute in order, so a tile written by one step is read
+    // by the next with no barrier; the three ring slots rotate without register copies
+    for (int l0 = 0;;) {
+        step(l0, r0);
+        if ((l0 += 4) >= L) break;
+        step(l0, r1);
+        if ((l0 += 4) >= L) break;
+        step(l0, r2);
+        if ((l0 += 4) >= L) break;
     }
     // item r of lane l: beta row 4(l>>4) + r of tile g, column j = l & 15
     if (!vr.valid) return;
@@ -200,14 +230,14 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
 }
 
 template <int TB>
-__global__ __launch_bounds__(64) void dwt_inverse_mix(const float2 *__restrict__ coeff, int64_t c_ts,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dwt_inverse_mix(const float2 *__restrict__ coeff, int64_t c_ts,
                                                       float2 *__restrict__ F, int64_t f_ts, int T,
                                                       const double *__restrict__ cosb, const double *__restrict__ seeds,
                                                       const int64_t *__restrict__ slice,
                                                       const int32_t *__restrict__ pairs, int bw, int N, int npair,
                                                       int nch, int nunit)
 {
-    __shared__ double cst[3 * 258];
+    __shared__ double cst[3 * CST_CAP];
     __shared__ float2 Dl[2][4][INV_DS];    // [step parity][degree in step][beta] = (hi, lo)
     const int id = blockIdx.x;
     const int chunk = (id >> 3) % nch;
@@ -221,12 +251,12 @@ __global__ __launch_bounds__(64) void dwt_inverse_mix(const float2 *__restrict__
 // grouped launch over transforms of several bandwidths: a persistent grid walks the
 // work list (item = segment, p | q << 16, first transform, chunk; segment -1 = padding)
 template <int TB>
-__global__ __launch_bounds__(64) void dwt_inverse_mix_grouped(const float2 *__restrict__ coeff,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dwt_inverse_mix_grouped(const float2 *__restrict__ coeff,
                                                               float2 *__restrict__ F,
                                                               const MixSegment *__restrict__ segs,
                                                               const int4 *__restrict__ items, int nitem)
 {
-    __shared__ double cst[3 * 258];
+    __shared__ double cst[3 * CST_CAP];
     __shared__ float2 Dl[2][4][INV_DS];
     for (int it = blockIdx.x; it < nitem; it += gridDim.x) {
         const int4 w = items[it];
@@ -244,189 +274,206 @@ __global__ __launch_bounds__(64) void dwt_inverse_mix_grouped(const float2 *__re
 // LDS between tiles.  Degrees go in 16-degree tiles; for each (tile, chunk) the lanes
 // advance their chain by 16 degrees, writing w(beta) d (hi, lo) to an LDS tile
 // [degree][beta], and the wave runs 16 K-steps of v_mfma_f32_16x16x4f32 with A = that
-// tile (M = degree, K = beta) and B = the fft-array samples (K = beta, N = variant x
-// re/im), in four independent accumulators per transform.  FP32 accumulation over 16
-// betas, FP64 across them and across chunks: a tile's coefficients are complete after
-// its last chunk, so there is no cross-wave reduction and no degree is recomputed.
+// tile (M = degree, K = beta) and B = the fft-array samples (K = beta, N = 8 variants x
+// 2 transforms), one MFMA for the real and one for the imaginary parts.  FP32
+// accumulation over 16 betas, FP64 across them and across chunks: a tile's
+// coefficients are complete after its last chunk, so there is no cross-wave reduction
+// and no degree is recomputed.
+//
+// B loads: K-step s of lane (column j, k) takes beta 8 (s >> 1) + 2 k + (s & 1) of the
+// chunk, so one 16-byte load gives a lane both samples (re, im, re, im) of two K-steps
+// and the four k lanes of a column read 64 contiguous bytes: 8 loads per chunk and
+// transform pair.  The columns are sample-contiguous in beta (reversed for the variants
+// at pi - beta), so the fft array must be 16-byte aligned with an even transform stride.
 // ---------------------------------------------------------------------------
 constexpr int FT_DS = 66;   // padded row of the forward tile (float2 entries)
 
-template <int TB, int MT>
-__device__ __forceinline__ void fwd_mix2_item(const float2 *__restrict__ F, int64_t f_ts,
-                                              float2 *__restrict__ coeff, int64_t c_ts, int T,
-                                              const double *__restrict__ w, const double *__restrict__ cosb,
-                                              const double *__restrict__ seeds, const int64_t *__restrict__ slice,
-                                              int bw, int N, int p, int q, int t0, double *cst,
-                                              float2 (*Dt)[FT_DS], double2 (*rs)[64])
+template <int TB>
+__device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64_t f_ts,
+                                             float2 *__restrict__ coeff, int64_t c_ts, int T,
+                                             const double *__restrict__ seeds, const int64_t *__restrict__ slice,
+                                             int bw, int N, int p, int q, int t0, double *cst,
+                                             float2 (*Dt)[FT_DS], double2 (*rs)[64])
 {
-    constexpr int TD = 16 * MT;   // degrees per tile: MT row tiles of 16 share each B operand
-    constexpr int KS = MT == 1 ? 4 : 2;   // independent accumulators per (row tile, transform)
+    constexpr int TG = (TB + 1) / 2;   // transform pairs: column j = variant j & 7 of transform 2 g + (j >> 3)
+    constexpr int KS = 4;              // accumulators per (pair, part): K-step s uses s % KS
     const int lane = threadIdx.x & 63;
     const int L = bw - q;
     const int nc = (N + 63) >> 6;
     const int nt = min(TB, T - t0);
     const int j = lane & 15, kk = lane >> 4;
-    const int vj = j >> 1, partj = j & 1;
+    const int vj = j & 7, tt = j >> 3;
     const Variant vr = variant(vj, p, q);
     const int64_t colj = ((int64_t)wrap_m(vr.n, N) * N + wrap_m(vr.m, N)) * N;
-    const float *src[TB];
+    const float2 *src[TG];
 #pragma unroll
-    for (int t = 0; t < TB; ++t)
-        src[t] = reinterpret_cast<const float *>(F + (int64_t)max(0, min(t0 + t, T - 1)) * f_ts + colj) + partj;
+    for (int g = 0; g < TG; ++g)
+        src[g] = F + (int64_t)max(0, min(t0 + 2 * g + tt, T - 1)) * f_ts + colj;
 
-    for (int i = lane; i < L - 1; i += 64) {
-        double a, x, b;
-        rec_consts_m(q + i, p, q, a, x, b);
-        cst[3 * i] = a; cst[3 * i + 1] = x; cst[3 * i + 2] = b;
-    }
+    fill_rec_consts(cst, L, p, q);
+    // chain states (d^l, d^{l-1}) per chunk; rows nc .. 2 nc - 1 of rs hold (cos beta, w),
+    // staged by the caller, so that the only global loads in the loop are the B operands
+    // (vector loads complete in order)
     const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
-    // recurrence state (d^l, d^{l-1}) of this lane's chain in each chunk
     for (int c = 0; c < nc; ++c) {
         const int beta = 64 * c + lane;
         rs[c][lane] = make_double2(beta < N ? seeds[tri * N + beta] : 0.0, 0.0);
     }
     __syncthreads();   // cst, rs (one wave: orders the LDS writes before the reads)
 
+    // B operands of K-steps 2h, 2h+1 of chunk c for transform pair g: the samples at
+    // betas b0 and b0 + 1, b0 = 64 c + 8 h + 2 k (clamped into the column; rows past N
+    // meet zero A rows), as loaded: (re, im) of b0 then b0 + 1, or the reverse order for
+    // the variants at pi - beta.  They are picked apart where the K-steps use them, so
+    // that nothing waits for a load before then.
+    float4 raw[8][TG];
+    auto load_b = [&](int c, int h) {
+        const int b0 = min(64 * c + 8 * h + 2 * kk, N - 2);
+        const int m0 = vr.rev ? N - 2 - b0 : b0;
+#pragma unroll
+        for (int g = 0; g < TG; ++g) raw[h][g] = *reinterpret_cast<const float4 *>(src[g] + m0);
+    };
+#pragma unroll
+    for (int h = 0; h < 8; ++h) load_b(0, h);
+
 #pragma unroll 1
-    for (int l0 = 0; l0 < L; l0 += TD) {
-        double accd[MT][TB][4];
+    for (int l0 = 0; l0 < L; l0 += 16) {
+        double accr[TG][4], acci[TG][4];
 #pragma unroll
-        for (int m = 0; m < MT; ++m)
+        for (int g = 0; g < TG; ++g)
 #pragma unroll
-            for (int t = 0; t < TB; ++t)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) accd[m][t][r] = 0.0;
+            for (int r = 0; r < 4; ++r) accr[g][r] = acci[g][r] = 0.0;
 #pragma unroll 1
         for (int c = 0; c < nc; ++c) {
-            // B operands of this chunk: the sample of column j at beta 64 c + 4 s + kk
-            float bv[16][TB];
-#pragma unroll
-            for (int s4 = 0; s4 < 16; ++s4) {
-                const int bt = min(64 * c + 4 * s4 + kk, N - 1);   // rows past N meet zero A columns
-                const int64_t o = 2 * (int64_t)(vr.rev ? N - 1 - bt : bt);
-#pragma unroll
-                for (int t = 0; t < TB; ++t) bv[s4][t] = src[t][o];
-            }
-            // w d (hi, lo) for degrees l0 .. l0 + TD - 1 at this lane's beta of chunk c
-            const int beta = 64 * c + lane;
-            const bool ok = beta < N;
-            const double cb = ok ? cosb[beta] : 0.0, wt = ok ? w[beta] : 0.0;
+            // raw holds this chunk's B operands, loaded during the previous chunk's K-steps
+            const int cn = c + 1 == nc ? 0 : c + 1;   // chunk whose operands the K-steps reload
+            // w d (hi, lo) for degrees l0 .. l0 + 15 at this lane's beta of chunk c
+            const double2 cw = rs[nc + c][lane];
+            const double cb = cw.x, wt = cw.y;
             asm volatile("" ::: "memory");
             double2 st = rs[c][lane];
             double dcur = st.x, dprev = st.y;
 #pragma unroll
-            for (int k = 0; k < TD; ++k) {
-                const int li = l0 + k;
+            for (int k = 0; k < 16; ++k) {   // the chain is zero past the last degree
                 float hi, lo;
-                split(li < L ? wt * dcur : 0.0, hi, lo);
+                split(wt * dcur, hi, lo);
                 Dt[k][lane] = make_float2(hi, lo);
-                if (li < L - 1) {
-                    const double ca = cst[3 * li], cx = cst[3 * li + 1], cbb = cst[3 * li + 2];
-                    const double dn = (ca * (cb - cx)) * dcur - cbb * dprev;
-                    dprev = dcur;
-                    dcur = dn;
-                }
+                rec_step(cst, l0 + k, cb, dcur, dprev);
             }
             rs[c][lane] = make_double2(dcur, dprev);
             asm volatile("" ::: "memory");
-            // independent accumulators per (row tile, transform): K-steps split KS ways,
-            // 16 betas each at most: no back-to-back dependent MFMAs
-            f4_t acc[MT][TB][KS];
+            f4_t ar[TG][KS], ai[TG][KS];
 #pragma unroll
-            for (int m = 0; m < MT; ++m)
+            for (int g = 0; g < TG; ++g)
 #pragma unroll
-                for (int t = 0; t < TB; ++t)
-#pragma unroll
-                    for (int g = 0; g < KS; ++g) acc[m][t][g] = f4_t{0.f, 0.f, 0.f, 0.f};
+                for (int u = 0; u < KS; ++u) ar[g][u] = ai[g][u] = f4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int s4 = 0; s4 < 16; ++s4) {
-                float2 a[MT];
+                const float2 a = Dt[j][8 * (s4 >> 1) + 2 * kk + (s4 & 1)];
+                const int u = s4 % KS;
+                const bool second = (s4 & 1) != vr.rev;   // this K-step's sample is the load's second
+                float bre[TG], bim[TG];
 #pragma unroll
-                for (int m = 0; m < MT; ++m) a[m] = Dt[16 * m + j][4 * s4 + kk];
-                const int g = s4 % KS;
-#pragma unroll
-                for (int m = 0; m < MT; ++m)
-#pragma unroll
-                    for (int t = 0; t < TB; ++t) acc[m][t][g] = mma32(a[m].x, bv[s4][t], acc[m][t][g]);
-#pragma unroll
-                for (int m = 0; m < MT; ++m)
-#pragma unroll
-                    for (int t = 0; t < TB; ++t) acc[m][t][g] = mma32(a[m].y, bv[s4][t], acc[m][t][g]);
-                if (KS == 2 && (s4 & 7) == 7) {   // KS = 2: flush every 8 K-steps (16 betas each)
-#pragma unroll
-                    for (int m = 0; m < MT; ++m)
-#pragma unroll
-                        for (int t = 0; t < TB; ++t)
-#pragma unroll
-                            for (int gg = 0; gg < KS; ++gg) {
-#pragma unroll
-                                for (int r = 0; r < 4; ++r) accd[m][t][r] += (double)acc[m][t][gg][r];
-                                acc[m][t][gg] = f4_t{0.f, 0.f, 0.f, 0.f};
-                            }
+                for (int g = 0; g < TG; ++g) {
+                    const float4 v = raw[s4 >> 1][g];
+                    bre[g] = second ? v.z : v.x;
+                    bim[g] = second ? v.w : v.y;
                 }
+#pragma unroll
+                for (int g = 0; g < TG; ++g) {
+                    ar[g][u] = mma32(a.x, bre[g], ar[g][u]);
+                    ai[g][u] = mma32(a.x, bim[g], ai[g][u]);
+                }
+#pragma unroll
+                for (int g = 0; g < TG; ++g) {
+                    ar[g][u] = mma32(a.y, bre[g], ar[g][u]);
+                    ai[g][u] = mma32(a.y, bim[g], ai[g][u]);
+                }
+                // K-steps 2h, 2h+1 consumed: reload them for chunk cn (chunk 0 of the next
+                // tile after the last chunk; once more, unused, after the last tile), so
+                // that the loads have a whole chunk of work to arrive in
+                if (s4 & 1) load_b(cn, s4 >> 1);
             }
-            if (KS == 4) {
 #pragma unroll
-                for (int m = 0; m < MT; ++m)
+            for (int g = 0; g < TG; ++g)
 #pragma unroll
-                    for (int t = 0; t < TB; ++t)
+                for (int u = 0; u < KS; ++u)
 #pragma unroll
-                        for (int gg = 0; gg < KS; ++gg)
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) accd[m][t][r] += (double)acc[m][t][gg][r];
-            }
+                    for (int r = 0; r < 4; ++r) {
+                        accr[g][r] += (double)ar[g][u][r];
+                        acci[g][r] += (double)ai[g][u][r];
+                    }
         }
-        // item r of lane l: degree row 4(l>>4) + r of row tile m, column j = l & 15
+        // item r of lane l: degree row 4(l>>4) + r, column j = l & 15
         if (vr.valid) {
             const int64_t o = slice_m(slice, vr.m, vr.n, bw);
 #pragma unroll
-            for (int t = 0; t < TB; ++t) {
+            for (int g = 0; g < TG; ++g) {
+                const int t = 2 * g + tt;
                 if (t >= nt) continue;
-                float *ct = reinterpret_cast<float *>(coeff + (int64_t)(t0 + t) * c_ts + o) + partj;
+                float2 *ct = coeff + (int64_t)(t0 + t) * c_ts + o;
 #pragma unroll
-                for (int m = 0; m < MT; ++m)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int li = l0 + 16 * m + 4 * kk + r;
-                        if (li < L) {
-                            const int l = q + li;
-                            const double sg = vr.sign * ((vr.lsign && (l & 1)) ? -1.0 : 1.0);
-                            ct[2 * (int64_t)li] = (float)(sg * accd[m][t][r]);
-                        }
+                for (int r = 0; r < 4; ++r) {
+                    const int li = l0 + 4 * kk + r;
+                    if (li < L) {
+                        const double sg = vr.sign * ((vr.lsign && ((q + li) & 1)) ? -1.0 : 1.0);
+                        ct[li] = make_float2((float)(sg * accr[g][r]), (float)(sg * acci[g][r]));
                     }
+                }
             }
         }
     }
 }
 
-template <int TB, int MT>
-__global__ __launch_bounds__(64) void dwt_forward_mix2(const float2 *__restrict__ F, int64_t f_ts,
-                                                       float2 *__restrict__ coeff, int64_t c_ts, int T,
-                                                       const double *__restrict__ w, const double *__restrict__ cosb,
-                                                       const double *__restrict__ seeds,
-                                                       const int64_t *__restrict__ slice,
-                                                       const int32_t *__restrict__ pairs, int bw, int N, int npair,
-                                                       int nunit)
+// LDS of the forward item: Dt [16][FT_DS] float2 | rs [2 nc][64] double2 (chain states,
+// then (cos beta, w)) | cst [3 (bw - 1 + CST_PAD)] double
+__device__ __forceinline__ void fwd_lds_carve(double *smx, int N, float2 (*&Dt)[FT_DS], double2 (*&rs)[64],
+                                              double *&cst)
 {
-    // dynamic LDS: Dt [16 MT][FT_DS] float2 | rs [nc][64] double2 | cst [3 (bw - 1)] double
+    Dt = reinterpret_cast<float2 (*)[FT_DS]>(smx);
+    rs = reinterpret_cast<double2 (*)[64]>(smx + 16 * FT_DS);
+    cst = smx + 16 * FT_DS + 4 * 64 * ((N + 63) >> 6);
+}
+
+// (cos beta, w(beta)) rows of the chain state table
+__device__ __forceinline__ void fwd_stage_sampling(double2 (*rs)[64], const double *cosb, const double *w, int N)
+{
+    const int lane = threadIdx.x & 63, nc = (N + 63) >> 6;
+    for (int c = 0; c < nc; ++c) {
+        const int beta = 64 * c + lane;
+        rs[nc + c][lane] = beta < N ? make_double2(cosb[beta], w[beta]) : make_double2(0.0, 0.0);
+    }
+}
+
+template <int TB>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void dwt_forward_mix(const float2 *__restrict__ F, int64_t f_ts,
+                                                      float2 *__restrict__ coeff, int64_t c_ts, int T,
+                                                      const double *__restrict__ w, const double *__restrict__ cosb,
+                                                      const double *__restrict__ seeds,
+                                                      const int64_t *__restrict__ slice,
+                                                      const int32_t *__restrict__ pairs, int bw, int N, int npair,
+                                                      int nunit)
+{
     extern __shared__ double smx[];
-    float2 (*Dt)[FT_DS] = reinterpret_cast<float2 (*)[FT_DS]>(smx);
-    double2 (*rs)[64] = reinterpret_cast<double2 (*)[64]>(smx + 16 * MT * FT_DS);
-    double *cst = smx + 16 * MT * FT_DS + 2 * 64 * ((N + 63) >> 6);
+    float2 (*Dt)[FT_DS];
+    double2 (*rs)[64];
+    double *cst;
+    fwd_lds_carve(smx, N, Dt, rs, cst);
     const int unit = blockIdx.x;
     if (unit >= nunit) return;
     const int pq = pairs[unit % npair];
-    fwd_mix2_item<TB, MT>(F, f_ts, coeff, c_ts, T, w, cosb, seeds, slice, bw, N, pq & 0xffff, pq >> 16,
-                          (unit / npair) * TB, cst, Dt, rs);
+    fwd_stage_sampling(rs, cosb, w, N);
+    fwd_mix_item<TB>(F, f_ts, coeff, c_ts, T, seeds, slice, bw, N, pq & 0xffff, pq >> 16, (unit / npair) * TB, cst,
+                     Dt, rs);
 }
 
 // grouped forward over transforms of several bandwidths (see dwt_inverse_mix_grouped);
 // dynamic LDS sized for the largest bandwidth of the group
-template <int TB, int MT>
-__global__ __launch_bounds__(64) void dwt_forward_mix2_grouped(const float2 *__restrict__ F,
-                                                               float2 *__restrict__ coeff,
-                                                               const MixSegment *__restrict__ segs,
-                                                               const int4 *__restrict__ items, int nitem)
+template <int TB>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void dwt_forward_mix_grouped(const float2 *__restrict__ F,
+                                                              float2 *__restrict__ coeff,
+                                                              const MixSegment *__restrict__ segs,
+                                                              const int4 *__restrict__ items, int nitem)
 {
     extern __shared__ double smx[];
     for (int it = blockIdx.x; it < nitem; it += gridDim.x) {
@@ -434,21 +481,29 @@ __global__ __launch_bounds__(64) void dwt_forward_mix2_grouped(const float2 *__r
         if (wk.x < 0) continue;
         const MixSegment sg = segs[wk.x];
         __syncthreads();  // the previous item's LDS reads are done
-        float2 (*Dt)[FT_DS] = reinterpret_cast<float2 (*)[FT_DS]>(smx);
-        double2 (*rs)[64] = reinterpret_cast<double2 (*)[64]>(smx + 16 * MT * FT_DS);
-        double *cst = smx + 16 * MT * FT_DS + 2 * 64 * ((sg.N + 63) >> 6);
-        fwd_mix2_item<TB, MT>(F + sg.goff, (int64_t)sg.N * sg.N * sg.N, coeff + sg.coff, sg.nc, sg.T, sg.w, sg.cosb,
-                              sg.seeds, sg.slice, sg.bw, sg.N, wk.y & 0xffff, wk.y >> 16, wk.z, cst, Dt, rs);
+        float2 (*Dt)[FT_DS];
+        double2 (*rs)[64];
+        double *cst;
+        fwd_lds_carve(smx, sg.N, Dt, rs, cst);
+        fwd_stage_sampling(rs, sg.cosb, sg.w, sg.N);
+        fwd_mix_item<TB>(F + sg.goff, (int64_t)sg.N * sg.N * sg.N, coeff + sg.coff, sg.nc, sg.T, sg.seeds, sg.slice,
+                         sg.bw, sg.N, wk.y & 0xffff, wk.y >> 16, wk.z, cst, Dt, rs);
     }
 }
 
 static int grid_m(int nunit, int nch) { return (nunit + 7) / 8 * 8 * nch; }
 
-// dynamic LDS bytes of dwt_forward_mix2 (A tile of 16 MT degrees, chunk states,
+// dynamic LDS bytes of dwt_forward_mix (A tile of 16 degrees, chunk states and sampling,
 // recurrence constants)
-static size_t fwd2_lds(int bw, int N, int MT)
+static size_t fwd_lds(int bw, int N)
 {
-    return sizeof(double) * (16 * MT * FT_DS + 2 * 64 * ((N + 63) / 64) + 3 * (bw > 1 ? bw - 1 : 1));
+    return sizeof(double) * (16 * FT_DS + 4 * 64 * ((N + 63) / 64) + 3 * (bw - 1 + CST_PAD));
+}
+
+// the forward's 16-byte B loads
+static bool fwd_aligned(const float2 *F, int64_t f_ts)
+{
+    return (reinterpret_cast<uintptr_t>(F) & 15) == 0 && (f_ts & 1) == 0;
 }
 
 hipError_t launch_dwt_inverse_mixed(const DeviceTables &tb, int bw, int N, int npair, int mode, const float2 *coeff,
@@ -472,20 +527,19 @@ hipError_t launch_dwt_inverse_mixed(const DeviceTables &tb, int bw, int N, int n
 hipError_t launch_dwt_forward_mixed(const DeviceTables &tb, int bw, int N, int npair, int mode, const float2 *F,
                                     int64_t f_ts, float2 *coeff, int64_t c_ts, int T, hipStream_t s)
 {
-    if (mode != DWT_COMPLEX) return hipErrorInvalidValue;
+    if (mode != DWT_COMPLEX || !fwd_aligned(F, f_ts)) return hipErrorInvalidValue;
     if (npair <= 0 || T <= 0) return hipSuccess;
-#define SO3_MFWD2(TBV, MTV)                                                                                    \
+#define SO3_MFWD(TBV)                                                                                          \
     do {                                                                                                       \
         const int nunit = npair * ((T + TBV - 1) / TBV);                                                       \
-        hipLaunchKernelGGL((dwt_forward_mix2<TBV, MTV>), dim3(nunit), dim3(64), fwd2_lds(bw, N, MTV), s, F, f_ts, \
-                           coeff, c_ts, T, tb.w, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, npair, nunit);   \
+        hipLaunchKernelGGL((dwt_forward_mix<TBV>), dim3(nunit), dim3(64), fwd_lds(bw, N), s, F, f_ts, coeff, c_ts, \
+                           T, tb.w, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, npair, nunit);                \
     } while (0)
-    // four transforms per wave from bw = 96 (more MFMA work per recurrence step), else two;
-    // one 16-degree row tile per B operand load (measured faster than two)
-    if (bw >= 96 && T >= 4) SO3_MFWD2(4, 1);
-    else if (T >= 2) SO3_MFWD2(2, 1);
-    else SO3_MFWD2(1, 1);
-#undef SO3_MFWD2
+    // four transforms per wave from bw = 96 (more MFMA work per recurrence step), else two
+    if (bw >= 96 && T >= 4) SO3_MFWD(4);
+    else if (T >= 2) SO3_MFWD(2);
+    else SO3_MFWD(1);
+#undef SO3_MFWD
     return hipGetLastError();
 }
 
@@ -508,14 +562,14 @@ hipError_t launch_dwt_forward_mixed_grouped(int TB, const float2 *F, float2 *coe
                                             const int4 *items, int nitem, int grid, int max_bw, hipStream_t s)
 {
     if (nitem <= 0) return hipSuccess;
-    if (grid <= 0) return hipErrorInvalidValue;
-    const size_t lds = fwd2_lds(max_bw, 2 * max_bw, 1);
+    if (grid <= 0 || !fwd_aligned(F, 0)) return hipErrorInvalidValue;
+    const size_t lds = fwd_lds(max_bw, 2 * max_bw);
     if (TB == 4)
-        hipLaunchKernelGGL((dwt_forward_mix2_grouped<4, 1>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
+        hipLaunchKernelGGL((dwt_forward_mix_grouped<4>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
     else if (TB == 2)
-        hipLaunchKernelGGL((dwt_forward_mix2_grouped<2, 1>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
+        hipLaunchKernelGGL((dwt_forward_mix_grouped<2>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
     else if (TB == 1)
-        hipLaunchKernelGGL((dwt_forward_mix2_grouped<1, 1>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
+        hipLaunchKernelGGL((dwt_forward_mix_grouped<1>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -530,10 +584,10 @@ int mixed_grouped_occupancy(int inverse, int TB, int max_bw)
         if (TB == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_inverse_mix_grouped<2>, 64, 0);
         else if (TB == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_inverse_mix_grouped<1>, 64, 0);
     } else {
-        const size_t lds = fwd2_lds(max_bw, 2 * max_bw, 1);
-        if (TB == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix2_grouped<4, 1>, 64, lds);
-        else if (TB == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix2_grouped<2, 1>, 64, lds);
-        else if (TB == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix2_grouped<1, 1>, 64, lds);
+        const size_t lds = fwd_lds(max_bw, 2 * max_bw);
+        if (TB == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<4>, 64, lds);
+        else if (TB == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<2>, 64, lds);
+        else if (TB == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<1>, 64, lds);
     }
     if (e != hipSuccess) {
         (void)hipGetLastError();
