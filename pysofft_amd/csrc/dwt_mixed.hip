@@ -27,45 +27,14 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int wrap_m(int m, int N) { return m >= 0 ? m : N + m; }
 
-__device__ inline void rec_consts_m(int l, int p, int q, double &a, double &x, double &b)
+// one step of the chain: (dcur, dprev) <- (d^{l+1}, d^l) with entry i of the pair's
+// recurrence constants (zero past the last degree), from global memory (i is the same in
+// every lane: scalar loads) or from LDS
+__device__ __forceinline__ void rec_step(const double *__restrict__ rcp, int i, double cb, double &dcur,
+                                         double &dprev)
 {
-    const int64_t lp = l + 1;
-    const int64_t den = (lp * lp - (int64_t)p * p) * (lp * lp - (int64_t)q * q);
-    double cc = (double)lp * sqrt(1.0 / (double)den);
-    cc = cc * sqrt((double)(2 * l + 3));
-    const double cl = sqrt((double)(2 * l + 1));
-    a = cc * cl;
-    x = (l == 0) ? 0.0 : (double)((int64_t)p * q) / (double)((int64_t)l * (l + 1));
-    if (l == q) {
-        b = 0.0;
-    } else {
-        double c1 = sqrt((double)(((int64_t)l * l - (int64_t)p * p) * ((int64_t)l * l - (int64_t)q * q))) /
-                    (double)l;
-        c1 = c1 / sqrt((double)(2 * l - 1));
-        b = cc * c1;
-    }
-}
-
-// recurrence constants of pair (p, q) in LDS: entry i = (a, a x, b) of the step from
-// degree q + i to q + i + 1, d^{l+1} = (a cos(beta) - a x) d^l - b d^{l-1}, for
-// 0 <= i < L - 1 (L = bw - q); entries L - 1 .. L - 2 + pad are zero, so a chain run past
-// the last degree stays at zero and needs no per-degree test.  One wave fills it.
-constexpr int CST_PAD = 16;
-constexpr int CST_CAP = 256 + CST_PAD;   // entries for bw <= 256
-__device__ __forceinline__ void fill_rec_consts(double *cst, int L, int p, int q)
-{
-    for (int i = threadIdx.x & 63; i < L - 1 + CST_PAD; i += 64) {
-        double a = 0.0, x = 0.0, b = 0.0;
-        if (i < L - 1) rec_consts_m(q + i, p, q, a, x, b);
-        cst[3 * i] = a; cst[3 * i + 1] = a * x; cst[3 * i + 2] = b;
-    }
-}
-
-// one step of the chain: (dcur, dprev) <- (d^{l+1}, d^l) with entry i's constants
-__device__ __forceinline__ void rec_step(const double *cst, int i, double cb, double &dcur, double &dprev)
-{
-    const double t = fma(cst[3 * i], cb, -cst[3 * i + 1]);
-    const double dn = fma(t, dcur, -cst[3 * i + 2] * dprev);
+    const double t = rcp[3 * i] * (cb - rcp[3 * i + 1]);   // one scalar operand per instruction
+    const double dn = fma(t, dcur, -rcp[3 * i + 2] * dprev);
     dprev = dcur;
     dcur = dn;
 }
@@ -89,182 +58,220 @@ __device__ __forceinline__ void split(double d, float &hi, float &lo)
 // ---------------------------------------------------------------------------
 // inverse
 // ---------------------------------------------------------------------------
-constexpr int INV_DS = 80;  // padded row of the inverse d tile (float2 entries)
+constexpr int INV_DS = 72;  // padded row of the inverse d tile (float2 entries)
+constexpr int INV_TD = 8;   // degrees per tile: two K-steps of 4
 
 // one inverse work item: order pair (p, q), transforms t0 .. t0+TB-1 (< T), betas
-// 64 chunk .. 64 chunk + 63; cst / Dl are the block's LDS
+// 64 chunk .. 64 chunk + 63.  MFMA columns j = variant j & 7 of transform t0 + (j >> 3),
+// real and imaginary parts in separate MFMAs; degrees go in tiles of 8, K-step s of a
+// tile taking degree l0 + 2 k + s in lane row k, so that one 16-byte load gives a lane
+// its coefficients of both K-steps.  rc / rc_off: recurrence constants; cst / Dl: the
+// block's LDS.
 template <int TB>
 __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, int64_t c_ts,
                                              float2 *__restrict__ F, int64_t f_ts, int T,
                                              const double *__restrict__ cosb, const double *__restrict__ seeds,
+                                             const double *__restrict__ rc, const int64_t *__restrict__ rc_off,
                                              const int64_t *__restrict__ slice, int bw, int N, int p, int q, int t0,
-                                             int chunk, double *cst, float2 (*Dl)[4][INV_DS])
+                                             int chunk, double *cst, float2 (*Dl)[INV_TD][INV_DS])
 {
+    static_assert(TB == 1 || TB == 2, "one transform pair per wave");
     const int lane = threadIdx.x;
     const int L = bw - q;
     const int b0 = chunk * 64;
     const int beta = b0 + lane;
     const int nt = min(TB, T - t0);
 
-    // B operand of this lane: column j = lane & 15 (variant j >> 1, part j & 1), degree
-    // offset k = lane >> 4 within each 4-degree step
+    // B operands of this lane: column j = lane & 15, degree row k = lane >> 4
     const int j = lane & 15, kk = lane >> 4;
-    const int vj = j >> 1, partj = j & 1;
+    const int vj = j & 7, tt = j >> 3;
     const Variant vr = variant(vj, p, q);
     const int64_t offj = vr.valid ? slice_m(slice, vr.m, vr.n, bw) : slice_m(slice, p, q, bw);
-    // the lane's degrees are l = q + l0 + kk with l0 a multiple of 4: the variant's
-    // (-1)^l factor is the same at every step
-    const float sgn = vr.valid ? (float)vr.sign * ((vr.lsign && ((q + kk) & 1)) ? -1.0f : 1.0f) : 0.0f;
-    const float *src[TB];
-#pragma unroll
-    for (int t = 0; t < TB; ++t)
-        src[t] = reinterpret_cast<const float *>(coeff + (int64_t)(t0 + min(t, nt - 1)) * c_ts + offj) + partj;
+    // signs of K-steps 0 and 1: degree l = q + l0 + 2 k + s with l0 a multiple of 8, so
+    // the variant's (-1)^l factor depends on s only
+    const float sg0 = vr.valid ? (float)vr.sign * ((vr.lsign && (q & 1)) ? -1.0f : 1.0f) : 0.0f;
+    const float sg1 = vr.lsign ? -sg0 : sg0;
+    const float2 *src = coeff + (int64_t)(t0 + min(tt, nt - 1)) * c_ts + offj;
 
-    fill_rec_consts(cst, L, p, q);
     const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
+    // the pair's recurrence constants, staged in LDS (a read there has a short latency)
+    const double *rcp = rc + 3 * rc_off[tri];
+    for (int i = lane; i < 3 * (L - 1 + REC_PAD); i += 64) cst[i] = rcp[i];
     double dcur = (beta < N) ? seeds[tri * N + beta] : 0.0;
     const double cb = (beta < N) ? cosb[beta] : 0.0;
     double dprev = 0.0;
 
-    f4_t acc[TB][4];
-    double accd[TB][4][4];
+    f4_t ar[4], ai[4];
+    double dr[4][4], di[4][4];
 #pragma unroll
-    for (int t = 0; t < TB; ++t)
+    for (int g = 0; g < 4; ++g) {
+        ar[g] = ai[g] = f4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            acc[t][g] = f4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int r = 0; r < 4; ++r) accd[t][g][r] = 0.0;
-        }
-    __syncthreads();
+        for (int r = 0; r < 4; ++r) dr[g][r] = di[g][r] = 0.0;
+    }
+    __syncthreads();   // cst
 
-    // raw coefficient of column j at degree l0 + kk (clamped into the column; past-the-end
-    // degrees enter with a zero sign)
-    auto loadb = [&](float (&dst)[TB], int l0) {
-        const int lc = min(l0 + kk, L - 1);
-#pragma unroll
-        for (int t = 0; t < TB; ++t) dst[t] = src[t][2 * (int64_t)lc];
+    // coefficients of degrees l0 + 2k and l0 + 2k + 1 (re, im, re, im), one 16-byte read
+    // (8-byte aligned: the memory system splits it where needed).  The pair is clamped
+    // into the slice: when l0 + 2k is the last degree it arrives as the second half
+    // (picked where it is used), and past-the-end degrees enter with a zero sign; a
+    // one-degree slice is read alone, into both halves.
+    auto loadb = [&](int l0) -> float4 {
+        if (L >= 2) {
+            const float2 *pp = src + min(l0 + 2 * kk, L - 2);
+            float4 v;
+            __builtin_memcpy(&v, pp, sizeof(v));
+            return v;
+        }
+        const float2 x = src[0];
+        return make_float4(x.x, x.y, x.x, x.y);
     };
-    // d for degrees l0..l0+3 of this lane's beta -> tile buf (then advanced past them);
-    // zero past the last degree
+    // d for degrees l0 .. l0 + 7 of this lane's beta -> tile buf (then advanced past
+    // them); zero past the last degree
     auto dtile = [&](int buf, int l0) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < INV_TD; ++k) {
             float hi, lo;
             split(dcur, hi, lo);
             Dl[buf][k][lane] = make_float2(hi, lo);
             rec_step(cst, l0 + k, cb, dcur, dprev);
         }
     };
-    // one 4-degree step: MFMAs on tile (l0 >> 2) & 1 with the ring slot r (loaded three
-    // steps earlier, reloaded here for step l0 + 12), interleaved with the recurrence
-    // that fills the other tile for the next step
-    auto step = [&](int l0, float (&r)[TB]) {
-        const int buf = (l0 >> 2) & 1;
-        float2 a[4];
+    // one 8-degree tile: 2 K-steps x 4 beta tiles x (re, im) x (hi, lo) MFMAs with the
+    // ring slot r (loaded two tiles earlier, reloaded here for tile l0 + 16),
+    // interleaved with the recurrence that fills the other LDS tile for the next one
+    auto tile = [&](int l0, float4 &r) {
+        const int buf = (l0 >> 3) & 1;
+        float2 a[2][4];
 #pragma unroll
-        for (int g = 0; g < 4; ++g) a[g] = Dl[buf][kk][16 * g + j];
-        const float sz = (l0 + kk < L) ? sgn : 0.0f;
-        float bv[TB];
+        for (int s2 = 0; s2 < 2; ++s2)
 #pragma unroll
-        for (int t = 0; t < TB; ++t) bv[t] = sz * r[t];
-        loadb(r, l0 + 12);
-        // all hi products, then all lo products: no back-to-back dependent MFMAs
+            for (int g = 0; g < 4; ++g) a[s2][g] = Dl[buf][2 * kk + s2][16 * g + j];
+        const int lk = l0 + 2 * kk;
+        const float z0 = lk < L ? sg0 : 0.0f, z1 = lk + 1 < L ? sg1 : 0.0f;
+        const bool last = lk > L - 2;   // the clamped pair starts one degree early
+        const float br0 = z0 * (last ? r.z : r.x), bi0 = z0 * (last ? r.w : r.y);
+        const float br1 = z1 * r.z, bi1 = z1 * r.w;
+        r = loadb(l0 + 16);
+        // per K-step the hi products, then the lo products: eight independent
+        // accumulators between dependent MFMAs
 #pragma unroll
-        for (int g = 0; g < 4; ++g)
+        for (int g = 0; g < 4; ++g) {
+            ar[g] = mma32(a[0][g].x, br0, ar[g]);
+            ai[g] = mma32(a[0][g].x, bi0, ai[g]);
+        }
 #pragma unroll
-            for (int t = 0; t < TB; ++t) acc[t][g] = mma32(a[g].x, bv[t], acc[t][g]);
+        for (int g = 0; g < 4; ++g) {
+            ar[g] = mma32(a[0][g].y, br0, ar[g]);
+            ai[g] = mma32(a[0][g].y, bi0, ai[g]);
+        }
 #pragma unroll
-        for (int g = 0; g < 4; ++g)
+        for (int g = 0; g < 4; ++g) {
+            ar[g] = mma32(a[1][g].x, br1, ar[g]);
+            ai[g] = mma32(a[1][g].x, bi1, ai[g]);
+        }
 #pragma unroll
-            for (int t = 0; t < TB; ++t) acc[t][g] = mma32(a[g].y, bv[t], acc[t][g]);
-        dtile(buf ^ 1, l0 + 4);
+        for (int g = 0; g < 4; ++g) {
+            ar[g] = mma32(a[1][g].y, br1, ar[g]);
+            ai[g] = mma32(a[1][g].y, bi1, ai[g]);
+        }
+        dtile(buf ^ 1, l0 + INV_TD);
         // issue order: each MFMA followed by a few VALU instructions of the recurrence,
         // which run while the matrix core works
 #pragma unroll
-        for (int i = 0; i < 8 * TB; ++i) {
+        for (int i = 0; i < 32; ++i) {
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, 8 / TB, 0);
-        }
-        if (((l0 >> 2) & 3) == 3 || l0 + 4 >= L) {  // flush every 16 degrees
-#pragma unroll
-            for (int t = 0; t < TB; ++t)
-#pragma unroll
-                for (int g = 0; g < 4; ++g) {
-#pragma unroll
-                    for (int rr = 0; rr < 4; ++rr) accd[t][g][rr] += (double)acc[t][g][rr];
-                    acc[t][g] = f4_t{0.f, 0.f, 0.f, 0.f};
-                }
+            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
         }
     };
-    float r0[TB], r1[TB], r2[TB];
-    loadb(r0, 0);
-    loadb(r1, 4);
-    loadb(r2, 8);
-    dtile(0, 0);
-    // LDS operations of one wave execute in order, so a tile written by one step is read
-    // by the next with no barrier; the three ring slots rotate without register copies
-    for (int l0 = 0;;) {
-        step(l0, r0);
-        if ((l0 += 4) >= L) break;
-        step(l0, r1);
-        if ((l0 += 4) >= L) break;
-        step(l0, r2);
-        if ((l0 += 4) >= L) break;
-    }
-    // item r of lane l: beta row 4(l>>4) + r of tile g, column j = l & 15
-    if (!vr.valid) return;
-    const int64_t colbase = ((int64_t)wrap_m(vr.n, N) * N + wrap_m(vr.m, N)) * N;
+    // FP32 sums of (at most) 16 degrees into the FP64 totals
+    auto flush = [&]() {
 #pragma unroll
-    for (int t = 0; t < TB; ++t) {
-        if (t >= nt) continue;
-        float *Ft = reinterpret_cast<float *>(F + (int64_t)(t0 + t) * f_ts + colbase) + partj;
-#pragma unroll
-        for (int g = 0; g < 4; ++g)
+        for (int g = 0; g < 4; ++g) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int bt = b0 + 16 * g + 4 * kk + r;
-                if (bt < N) Ft[2 * (int64_t)(vr.rev ? N - 1 - bt : bt)] = (float)accd[t][g][r];
+                dr[g][r] += (double)ar[g][r];
+                di[g][r] += (double)ai[g][r];
             }
+            ar[g] = ai[g] = f4_t{0.f, 0.f, 0.f, 0.f};
+        }
+    };
+
+    float4 r0 = loadb(0), r1 = loadb(8);
+    dtile(0, 0);
+    // LDS operations of one wave execute in order, so a tile written during one step is
+    // read by the next with no barrier; two ring slots, one per tile of a 16-degree flush
+    for (int l0 = 0;; l0 += 16) {
+        tile(l0, r0);
+        if (l0 + 8 >= L) break;
+        tile(l0 + 8, r1);
+        if (l0 + 16 >= L) break;
+        flush();
+    }
+    flush();
+
+    // item r of lane l: beta row 4(l>>4) + r of tile g, column j = l & 15: betas
+    // b0 + 16 g + 4 k .. + 3 of one column, stored as two 16-byte writes (reversed in
+    // memory for the variants at pi - beta)
+    if (!vr.valid || tt >= nt) return;
+    const int64_t colbase = ((int64_t)wrap_m(vr.n, N) * N + wrap_m(vr.m, N)) * N;
+    float2 *Ft = F + (int64_t)(t0 + tt) * f_ts + colbase;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const int bt = b0 + 16 * g + 4 * kk;
+        float2 v[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = make_float2((float)dr[g][r], (float)di[g][r]);
+        if (bt + 3 < N) {
+            const int m0 = vr.rev ? N - 4 - bt : bt;
+            const float4 w0 = vr.rev ? make_float4(v[3].x, v[3].y, v[2].x, v[2].y)
+                                     : make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
+            const float4 w1 = vr.rev ? make_float4(v[1].x, v[1].y, v[0].x, v[0].y)
+                                     : make_float4(v[2].x, v[2].y, v[3].x, v[3].y);
+            *reinterpret_cast<float4 *>(Ft + m0) = w0;
+            *reinterpret_cast<float4 *>(Ft + m0 + 2) = w1;
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (bt + r < N) Ft[vr.rev ? N - 1 - bt - r : bt + r] = v[r];
+        }
     }
 }
 
 template <int TB>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dwt_inverse_mix(const float2 *__restrict__ coeff, int64_t c_ts,
-                                                      float2 *__restrict__ F, int64_t f_ts, int T,
-                                                      const double *__restrict__ cosb, const double *__restrict__ seeds,
-                                                      const int64_t *__restrict__ slice,
-                                                      const int32_t *__restrict__ pairs, int bw, int N, int npair,
-                                                      int nch, int nunit)
+__global__ __launch_bounds__(64) void dwt_inverse_mix(
+    const float2 *__restrict__ coeff, int64_t c_ts, float2 *__restrict__ F, int64_t f_ts, int T,
+    const double *__restrict__ cosb, const double *__restrict__ seeds, const double *__restrict__ rc,
+    const int64_t *__restrict__ rc_off, const int64_t *__restrict__ slice, const int32_t *__restrict__ pairs, int bw,
+    int N, int npair, int nch, int nunit)
 {
-    __shared__ double cst[3 * CST_CAP];
-    __shared__ float2 Dl[2][4][INV_DS];    // [step parity][degree in step][beta] = (hi, lo)
+    __shared__ float2 Dl[2][INV_TD][INV_DS];   // [tile parity][degree in tile][beta] = (hi, lo)
+    __shared__ double cst[3 * (256 - 1 + REC_PAD)];
     const int id = blockIdx.x;
     const int chunk = (id >> 3) % nch;
     const int unit = ((id >> 3) / nch) * 8 + (id & 7);  // chunks of a unit share an XCD
     if (unit >= nunit) return;
     const int pq = pairs[unit % npair];
-    inv_mix_item<TB>(coeff, c_ts, F, f_ts, T, cosb, seeds, slice, bw, N, pq & 0xffff, pq >> 16,
+    inv_mix_item<TB>(coeff, c_ts, F, f_ts, T, cosb, seeds, rc, rc_off, slice, bw, N, pq & 0xffff, pq >> 16,
                      (unit / npair) * TB, chunk, cst, Dl);
 }
 
 // grouped launch over transforms of several bandwidths: a persistent grid walks the
 // work list (item = segment, p | q << 16, first transform, chunk; segment -1 = padding)
 template <int TB>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dwt_inverse_mix_grouped(const float2 *__restrict__ coeff,
-                                                              float2 *__restrict__ F,
-                                                              const MixSegment *__restrict__ segs,
-                                                              const int4 *__restrict__ items, int nitem)
+__global__ __launch_bounds__(64) void dwt_inverse_mix_grouped(
+    const float2 *__restrict__ coeff, float2 *__restrict__ F, const MixSegment *__restrict__ segs,
+    const int4 *__restrict__ items, int nitem)
 {
-    __shared__ double cst[3 * CST_CAP];
-    __shared__ float2 Dl[2][4][INV_DS];
+    __shared__ float2 Dl[2][INV_TD][INV_DS];
+    __shared__ double cst[3 * (256 - 1 + REC_PAD)];
     for (int it = blockIdx.x; it < nitem; it += gridDim.x) {
         const int4 w = items[it];
         if (w.x < 0) continue;
         const MixSegment sg = segs[w.x];
         __syncthreads();  // the previous item's LDS reads are done
         inv_mix_item<TB>(coeff + sg.coff, sg.nc, F + sg.goff, (int64_t)sg.N * sg.N * sg.N, sg.T, sg.cosb, sg.seeds,
-                         sg.slice, sg.bw, sg.N, w.y & 0xffff, w.y >> 16, w.z, w.w, cst, Dl);
+                         sg.rc, sg.rc_off, sg.slice, sg.bw, sg.N, w.y & 0xffff, w.y >> 16, w.z, w.w, cst, Dl);
     }
 }
 
@@ -291,9 +298,10 @@ constexpr int FT_DS = 66;   // padded row of the forward tile (float2 entries)
 template <int TB>
 __device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64_t f_ts,
                                              float2 *__restrict__ coeff, int64_t c_ts, int T,
-                                             const double *__restrict__ seeds, const int64_t *__restrict__ slice,
-                                             int bw, int N, int p, int q, int t0, double *cst,
-                                             float2 (*Dt)[FT_DS], double2 (*rs)[64])
+                                             const double *__restrict__ seeds, const double *__restrict__ rc,
+                                             const int64_t *__restrict__ rc_off, const int64_t *__restrict__ slice,
+                                             int bw, int N, int p, int q, int t0, float2 (*Dt)[FT_DS],
+                                             double2 (*rs)[64])
 {
     constexpr int TG = (TB + 1) / 2;   // transform pairs: column j = variant j & 7 of transform 2 g + (j >> 3)
     constexpr int KS = 4;              // accumulators per (pair, part): K-step s uses s % KS
@@ -310,16 +318,16 @@ __device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64
     for (int g = 0; g < TG; ++g)
         src[g] = F + (int64_t)max(0, min(t0 + 2 * g + tt, T - 1)) * f_ts + colj;
 
-    fill_rec_consts(cst, L, p, q);
     // chain states (d^l, d^{l-1}) per chunk; rows nc .. 2 nc - 1 of rs hold (cos beta, w),
-    // staged by the caller, so that the only global loads in the loop are the B operands
+    // staged by the caller, so that the only vector loads in the loop are the B operands
     // (vector loads complete in order)
     const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
+    const double *rcp = rc + 3 * rc_off[tri];
     for (int c = 0; c < nc; ++c) {
         const int beta = 64 * c + lane;
         rs[c][lane] = make_double2(beta < N ? seeds[tri * N + beta] : 0.0, 0.0);
     }
-    __syncthreads();   // cst, rs (one wave: orders the LDS writes before the reads)
+    __syncthreads();   // rs (one wave: orders the LDS writes before the reads)
 
     // B operands of K-steps 2h, 2h+1 of chunk c for transform pair g: the samples at
     // betas b0 and b0 + 1, b0 = 64 c + 8 h + 2 k (clamped into the column; rows past N
@@ -358,7 +366,7 @@ __device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64
                 float hi, lo;
                 split(wt * dcur, hi, lo);
                 Dt[k][lane] = make_float2(hi, lo);
-                rec_step(cst, l0 + k, cb, dcur, dprev);
+                rec_step(rcp, l0 + k, cb, dcur, dprev);
             }
             rs[c][lane] = make_double2(dcur, dprev);
             asm volatile("" ::: "memory");
@@ -426,13 +434,11 @@ __device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64
 }
 
 // LDS of the forward item: Dt [16][FT_DS] float2 | rs [2 nc][64] double2 (chain states,
-// then (cos beta, w)) | cst [3 (bw - 1 + CST_PAD)] double
-__device__ __forceinline__ void fwd_lds_carve(double *smx, int N, float2 (*&Dt)[FT_DS], double2 (*&rs)[64],
-                                              double *&cst)
+// then (cos beta, w))
+__device__ __forceinline__ void fwd_lds_carve(double *smx, float2 (*&Dt)[FT_DS], double2 (*&rs)[64])
 {
     Dt = reinterpret_cast<float2 (*)[FT_DS]>(smx);
     rs = reinterpret_cast<double2 (*)[64]>(smx + 16 * FT_DS);
-    cst = smx + 16 * FT_DS + 4 * 64 * ((N + 63) >> 6);
 }
 
 // (cos beta, w(beta)) rows of the chain state table
@@ -450,6 +456,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void dw
                                                       float2 *__restrict__ coeff, int64_t c_ts, int T,
                                                       const double *__restrict__ w, const double *__restrict__ cosb,
                                                       const double *__restrict__ seeds,
+                                                      const double *__restrict__ rc,
+                                                      const int64_t *__restrict__ rc_off,
                                                       const int64_t *__restrict__ slice,
                                                       const int32_t *__restrict__ pairs, int bw, int N, int npair,
                                                       int nunit)
@@ -457,14 +465,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void dw
     extern __shared__ double smx[];
     float2 (*Dt)[FT_DS];
     double2 (*rs)[64];
-    double *cst;
-    fwd_lds_carve(smx, N, Dt, rs, cst);
+    fwd_lds_carve(smx, Dt, rs);
     const int unit = blockIdx.x;
     if (unit >= nunit) return;
     const int pq = pairs[unit % npair];
     fwd_stage_sampling(rs, cosb, w, N);
-    fwd_mix_item<TB>(F, f_ts, coeff, c_ts, T, seeds, slice, bw, N, pq & 0xffff, pq >> 16, (unit / npair) * TB, cst,
-                     Dt, rs);
+    fwd_mix_item<TB>(F, f_ts, coeff, c_ts, T, seeds, rc, rc_off, slice, bw, N, pq & 0xffff, pq >> 16,
+                     (unit / npair) * TB, Dt, rs);
 }
 
 // grouped forward over transforms of several bandwidths (see dwt_inverse_mix_grouped);
@@ -483,24 +490,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void dw
         __syncthreads();  // the previous item's LDS reads are done
         float2 (*Dt)[FT_DS];
         double2 (*rs)[64];
-        double *cst;
-        fwd_lds_carve(smx, sg.N, Dt, rs, cst);
+        fwd_lds_carve(smx, Dt, rs);
         fwd_stage_sampling(rs, sg.cosb, sg.w, sg.N);
-        fwd_mix_item<TB>(F + sg.goff, (int64_t)sg.N * sg.N * sg.N, coeff + sg.coff, sg.nc, sg.T, sg.seeds, sg.slice,
-                         sg.bw, sg.N, wk.y & 0xffff, wk.y >> 16, wk.z, cst, Dt, rs);
+        fwd_mix_item<TB>(F + sg.goff, (int64_t)sg.N * sg.N * sg.N, coeff + sg.coff, sg.nc, sg.T, sg.seeds, sg.rc,
+                         sg.rc_off, sg.slice, sg.bw, sg.N, wk.y & 0xffff, wk.y >> 16, wk.z, Dt, rs);
     }
 }
 
 static int grid_m(int nunit, int nch) { return (nunit + 7) / 8 * 8 * nch; }
 
-// dynamic LDS bytes of dwt_forward_mix (A tile of 16 degrees, chunk states and sampling,
-// recurrence constants)
-static size_t fwd_lds(int bw, int N)
+// dynamic LDS bytes of dwt_forward_mix (A tile of 16 degrees, chunk states and sampling)
+static size_t fwd_lds(int N)
 {
-    return sizeof(double) * (16 * FT_DS + 4 * 64 * ((N + 63) / 64) + 3 * (bw - 1 + CST_PAD));
+    return sizeof(double) * (16 * FT_DS + 4 * 64 * ((N + 63) / 64));
 }
 
-// the forward's 16-byte B loads
+// the 16-byte fft-array accesses of both directions
 static bool fwd_aligned(const float2 *F, int64_t f_ts)
 {
     return (reinterpret_cast<uintptr_t>(F) & 15) == 0 && (f_ts & 1) == 0;
@@ -509,16 +514,16 @@ static bool fwd_aligned(const float2 *F, int64_t f_ts)
 hipError_t launch_dwt_inverse_mixed(const DeviceTables &tb, int bw, int N, int npair, int mode, const float2 *coeff,
                                     int64_t c_ts, const float2 *aux, float2 *F, int64_t f_ts, int T, hipStream_t s)
 {
-    if (mode != DWT_COMPLEX || aux) return hipErrorInvalidValue;
+    if (mode != DWT_COMPLEX || aux || !fwd_aligned(F, f_ts)) return hipErrorInvalidValue;
     if (npair <= 0 || T <= 0) return hipSuccess;
     const int nch = (N + 63) / 64;
 #define SO3_MINV(TBV)                                                                                          \
     do {                                                                                                       \
         const int nunit = npair * ((T + TBV - 1) / TBV);                                                       \
         hipLaunchKernelGGL((dwt_inverse_mix<TBV>), dim3(grid_m(nunit, nch)), dim3(64), 0, s, coeff, c_ts, F, f_ts, \
-                           T, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, npair, nch, nunit);                  \
+                           T, tb.cosb, tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nch, nunit); \
     } while (0)
-    if (T >= 2) SO3_MINV(2);  // TB = 4 needs 288 registers (one wave per SIMD)
+    if (T >= 2) SO3_MINV(2);  // one transform pair per wave
     else SO3_MINV(1);
 #undef SO3_MINV
     return hipGetLastError();
@@ -532,8 +537,8 @@ hipError_t launch_dwt_forward_mixed(const DeviceTables &tb, int bw, int N, int n
 #define SO3_MFWD(TBV)                                                                                          \
     do {                                                                                                       \
         const int nunit = npair * ((T + TBV - 1) / TBV);                                                       \
-        hipLaunchKernelGGL((dwt_forward_mix<TBV>), dim3(nunit), dim3(64), fwd_lds(bw, N), s, F, f_ts, coeff, c_ts, \
-                           T, tb.w, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, npair, nunit);                \
+        hipLaunchKernelGGL((dwt_forward_mix<TBV>), dim3(nunit), dim3(64), fwd_lds(N), s, F, f_ts, coeff, c_ts, T,      \
+                           tb.w, tb.cosb, tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit);                 \
     } while (0)
     // four transforms per wave from bw = 96 (more MFMA work per recurrence step), else two
     if (bw >= 96 && T >= 4) SO3_MFWD(4);
@@ -548,7 +553,7 @@ hipError_t launch_dwt_inverse_mixed_grouped(int TB, const float2 *coeff, float2 
                                             const int4 *items, int nitem, int grid, hipStream_t s)
 {
     if (nitem <= 0) return hipSuccess;
-    if (grid <= 0) return hipErrorInvalidValue;
+    if (grid <= 0 || !fwd_aligned(F, 0)) return hipErrorInvalidValue;
     if (TB == 2)
         hipLaunchKernelGGL((dwt_inverse_mix_grouped<2>), dim3(grid), dim3(64), 0, s, coeff, F, segs, items, nitem);
     else if (TB == 1)
@@ -563,7 +568,7 @@ hipError_t launch_dwt_forward_mixed_grouped(int TB, const float2 *F, float2 *coe
 {
     if (nitem <= 0) return hipSuccess;
     if (grid <= 0 || !fwd_aligned(F, 0)) return hipErrorInvalidValue;
-    const size_t lds = fwd_lds(max_bw, 2 * max_bw);
+    const size_t lds = fwd_lds(2 * max_bw);
     if (TB == 4)
         hipLaunchKernelGGL((dwt_forward_mix_grouped<4>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
     else if (TB == 2)
@@ -584,7 +589,7 @@ int mixed_grouped_occupancy(int inverse, int TB, int max_bw)
         if (TB == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_inverse_mix_grouped<2>, 64, 0);
         else if (TB == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_inverse_mix_grouped<1>, 64, 0);
     } else {
-        const size_t lds = fwd_lds(max_bw, 2 * max_bw);
+        const size_t lds = fwd_lds(2 * max_bw);
         if (TB == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<4>, 64, lds);
         else if (TB == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<2>, 64, lds);
         else if (TB == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<1>, 64, lds);

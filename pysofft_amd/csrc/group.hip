@@ -593,6 +593,7 @@ int so3ft_group_create(const int64_t *bws, int64_t n, int64_t *out)
         if ((rc = plan_device_tables(sg.plan, &sg.tb, &bw, &lmax))) break;
         MixSegment m{};
         m.cosb = sg.tb.cosb; m.w = sg.tb.w; m.seeds = sg.tb.seeds; m.slice = sg.tb.slice;
+        m.rc = sg.tb.rc; m.rc_off = sg.tb.rc_off;
         m.bw = (int)sg.bw; m.N = (int)sg.N; m.T = (int)sg.T; m.pad = 0;
         m.coff = sg.coff; m.goff = sg.goff; m.nc = sg.nc;
         ms.push_back(m);

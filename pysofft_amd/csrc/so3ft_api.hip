@@ -73,7 +73,8 @@ Plan *get(so3ft_handle h)
 void free_plan_device(Plan *P)
 {
     hipFree(P->tb.cosb); hipFree(P->tb.w); hipFree(P->tb.seeds); hipFree(P->tb.slice);
-    hipFree(P->tb.pairs); hipFree(P->tb.tw); hipFree(P->scratch); hipFree(P->red); hipFree(P->cstage);
+    hipFree(P->tb.pairs); hipFree(P->tb.tw); hipFree(P->tb.rc); hipFree(P->tb.rc_off);
+    hipFree(P->scratch); hipFree(P->red); hipFree(P->cstage);
     P->tb = DeviceTables();
     P->scratch = nullptr; P->scratch_T = 0; P->red = nullptr;
     P->cstage = nullptr; P->cstage_T = 0;
@@ -150,6 +151,25 @@ std::vector<int32_t> lmn_perm_table(int64_t bw)
     return perm;
 }
 
+// constants (a, x, b) of the normalized Kostelec step from degree l to l + 1 of the pair
+// (p, q) (wigner.f90:227-260): d^{l+1} = a (cos(beta) - x) d^l - b d^{l-1}
+void rec_consts_host(int64_t l, int64_t p, int64_t q, double &a, double &x, double &b)
+{
+    const int64_t lp = l + 1;
+    const int64_t den = (lp * lp - p * p) * (lp * lp - q * q);
+    double cc = (double)lp * std::sqrt(1.0 / (double)den);
+    cc = cc * std::sqrt((double)(2 * l + 3));
+    a = cc * std::sqrt((double)(2 * l + 1));
+    x = (l == 0) ? 0.0 : (double)(p * q) / (double)(l * (l + 1));
+    if (l == q) {
+        b = 0.0;
+    } else {
+        double c1 = std::sqrt((double)((l * l - p * p) * (l * l - q * q))) / (double)l;
+        c1 = c1 / std::sqrt((double)(2 * l - 1));
+        b = cc * c1;
+    }
+}
+
 int build_plan(Plan *P, int64_t bw, int64_t lmax, int precompute, int init_ffts, int64_t rt, int32_t flags)
 {
     if (bw < 1 || bw > 256) return fail(SO3FT_EINVAL, "bw must satisfy 1 <= bw <= 256, got " + std::to_string(bw));
@@ -200,6 +220,19 @@ int build_plan(Plan *P, int64_t bw, int64_t lmax, int precompute, int init_ffts,
         const int64_t base = (m * (2 * bw - m + 1)) / 2;
         std::memcpy(&seeds[base * N], tmp.data(), sizeof(double) * (bw - m) * N);
     }
+    // recurrence constants per pair (Kostelec recurrence, wigner.f90:227-260, normalized)
+    std::vector<int64_t> rc_off(ntri);
+    std::vector<double> rct;
+    for (int64_t pp = 0; pp < bw; ++pp)
+        for (int64_t qq = pp; qq < bw; ++qq) {
+            const int64_t tri = pp * (2 * bw - pp + 1) / 2 + (qq - pp), L = bw - qq;
+            rc_off[tri] = (int64_t)rct.size() / 3;
+            for (int64_t i = 0; i < L - 1 + REC_PAD; ++i) {
+                double a = 0.0, x = 0.0, b = 0.0;
+                if (i < L - 1) rec_consts_host(qq + i, pp, qq, a, x, b);
+                rct.push_back(a); rct.push_back(x); rct.push_back(b);
+            }
+        }
     // pairs with q <= lmax, most work (largest bw-q) first
     std::vector<int32_t> pairs;
     for (int64_t q = 0; q <= lmax; ++q)
@@ -227,6 +260,8 @@ int build_plan(Plan *P, int64_t bw, int64_t lmax, int precompute, int init_ffts,
     if ((rc = upload(&P->tb.slice, slice_table(bw)))) return rc;
     if ((rc = upload(&P->tb.pairs, pairs))) return rc;
     if ((rc = upload(&P->tb.tw, tw))) return rc;
+    if ((rc = upload(&P->tb.rc, rct))) return rc;
+    if ((rc = upload(&P->tb.rc_off, rc_off))) return rc;
     // recurrence_type = Risbo without precomputed Wigners: coefficients in lmn order
     // (soft.py:98-106); the transforms reorder them and run the same d values (Kostelec
     // recurrence, equal to the Risbo values to rounding for bw <= 256)

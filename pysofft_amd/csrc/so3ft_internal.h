@@ -56,7 +56,15 @@ struct DeviceTables {
     int64_t *slice = nullptr;  // [(2bw-1)^2] mnl start of f^l_{m,n}
     int32_t *pairs = nullptr;  // [npair] p | q << 16, ordered by decreasing bw-q
     double2 *tw = nullptr;     // [N]    exp(-2 pi i k / N)
+    double *rc = nullptr;      // [sum over pairs of (bw - q - 1 + REC_PAD)][3] recurrence constants (a, x, b)
+    int64_t *rc_off = nullptr; // [ntri]  first entry of pair tri(p, q) in rc
 };
+
+// recurrence constants of pair (p, q) for the mixed-precision kernels: entry i = (a, x, b)
+// of the step from degree q + i to q + i + 1, d^{l+1} = a (cos(beta) - x) d^l - b d^{l-1},
+// for 0 <= i < L - 1 (L = bw - q); the next REC_PAD entries are zero, so a chain run past
+// the last degree stays at zero with no per-degree test
+constexpr int REC_PAD = 16;
 
 struct FftDesc {
     int N = 0;
@@ -100,8 +108,8 @@ hipError_t launch_dwt_forward_mixed(const DeviceTables &t, int bw, int N, int np
 // offsets of its first transform in the packed coefficient / grid arrays (the segment's
 // transforms are contiguous, nc resp. N^3 apart)
 struct MixSegment {
-    const double *cosb, *w, *seeds;
-    const int64_t *slice;
+    const double *cosb, *w, *seeds, *rc;
+    const int64_t *slice, *rc_off;
     int bw, N, T, pad;
     int64_t coff, goff, nc;
 };
