@@ -64,8 +64,8 @@ constexpr int INV_TD = 8;   // degrees per tile: two K-steps of 4
 // one inverse work item: order pair (p, q), transforms t0 .. t0+TB-1 (< T), betas
 // 64 chunk .. 64 chunk + 63.  MFMA columns j = variant j & 7 of transform t0 + (j >> 3),
 // real and imaginary parts in separate MFMAs; degrees go in tiles of 8, K-step s of a
-// tile taking degree l0 + 2 k + s in lane row k, so that one 16-byte load gives a lane
-// its coefficients of both K-steps.  rc / rc_off: recurrence constants; cst / Dl: the
+// tile taking degree l0 + 2 k + s in lane row k, so that a lane's coefficients of both
+// K-steps are adjacent in memory.  rc / rc_off: recurrence constants; cst / Dl: the
 // block's LDS.
 template <int TB>
 __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, int64_t c_ts,
@@ -111,20 +111,13 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
     }
     __syncthreads();   // cst
 
-    // coefficients of degrees l0 + 2k and l0 + 2k + 1 (re, im, re, im), one 16-byte read
-    // (8-byte aligned: the memory system splits it where needed).  The pair is clamped
-    // into the slice: when l0 + 2k is the last degree it arrives as the second half
-    // (picked where it is used), and past-the-end degrees enter with a zero sign; a
-    // one-degree slice is read alone, into both halves.
-    auto loadb = [&](int l0) -> float4 {
-        if (L >= 2) {
-            const float2 *pp = src + min(l0 + 2 * kk, L - 2);
-            float4 v;
-            __builtin_memcpy(&v, pp, sizeof(v));
-            return v;
-        }
-        const float2 x = src[0];
-        return make_float4(x.x, x.y, x.x, x.y);
+    // coefficients of degrees l0 + 2k and l0 + 2k + 1 (re, im, re, im), each clamped into
+    // the slice (past-the-end degrees enter with a zero sign); no branch, so that the
+    // ring's loads stay in flight across tiles (a wait after a branch merge is for all)
+    auto loadb = [&](int l0, float2 &x0, float2 &x1) {
+        const int lk = l0 + 2 * kk;
+        x0 = src[min(lk, L - 1)];
+        x1 = src[min(lk + 1, L - 1)];
     };
     // d for degrees l0 .. l0 + 7 of this lane's beta -> tile buf (then advanced past
     // them); zero past the last degree
@@ -140,7 +133,7 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
     // one 8-degree tile: 2 K-steps x 4 beta tiles x (re, im) x (hi, lo) MFMAs with the
     // ring slot r (loaded two tiles earlier, reloaded here for tile l0 + 16),
     // interleaved with the recurrence that fills the other LDS tile for the next one
-    auto tile = [&](int l0, float4 &r) {
+    auto tile = [&](int l0, float2 &ra, float2 &rb) {
         const int buf = (l0 >> 3) & 1;
         float2 a[2][4];
 #pragma unroll
@@ -149,10 +142,9 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
             for (int g = 0; g < 4; ++g) a[s2][g] = Dl[buf][2 * kk + s2][16 * g + j];
         const int lk = l0 + 2 * kk;
         const float z0 = lk < L ? sg0 : 0.0f, z1 = lk + 1 < L ? sg1 : 0.0f;
-        const bool last = lk > L - 2;   // the clamped pair starts one degree early
-        const float br0 = z0 * (last ? r.z : r.x), bi0 = z0 * (last ? r.w : r.y);
-        const float br1 = z1 * r.z, bi1 = z1 * r.w;
-        r = loadb(l0 + 16);
+        const float br0 = z0 * ra.x, bi0 = z0 * ra.y, br1 = z1 * rb.x, bi1 = z1 * rb.y;
+        loadb(l0 + 16, ra, rb);
+        __builtin_amdgcn_sched_barrier(0);   // the reload is issued here, two tiles ahead
         // per K-step the hi products, then the lo products: eight independent
         // accumulators between dependent MFMAs
 #pragma unroll
@@ -197,17 +189,22 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
         }
     };
 
-    float4 r0 = loadb(0), r1 = loadb(8);
+    float2 r0a, r0b, r1a, r1b;
+    loadb(0, r0a, r0b);
+    loadb(8, r1a, r1b);
     dtile(0, 0);
     // LDS operations of one wave execute in order, so a tile written during one step is
-    // read by the next with no barrier; two ring slots, one per tile of a 16-degree flush
-    for (int l0 = 0;; l0 += 16) {
-        tile(l0, r0);
-        if (l0 + 8 >= L) break;
-        tile(l0 + 8, r1);
-        if (l0 + 16 >= L) break;
+    // read by the next with no barrier; two ring slots, one per tile of a 16-degree flush.
+    // No exit between a reload and the tiles after it: a load whose value one exit path
+    // does not need would be moved past the exit test, next to its use.
+    int l0 = 0;
+    for (; l0 + 16 < L; l0 += 16) {
+        tile(l0, r0a, r0b);
+        tile(l0 + 8, r1a, r1b);
         flush();
     }
+    tile(l0, r0a, r0b);
+    if (l0 + 8 < L) tile(l0 + 8, r1a, r1b);
     flush();
 
     // item r of lane l: beta row 4(l>>4) + r of tile g, column j = l & 15: betas
