@@ -27,11 +27,16 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int wrap_m(int m, int N) { return m >= 0 ? m : N + m; }
 
+// the plan's recurrence-constant table, read-only for a kernel's lifetime: addressed as
+// constant memory, a load whose index is the same in every lane becomes a scalar load
+// (also when the table pointer itself was read from memory, as in the grouped kernels)
+typedef const __attribute__((address_space(4))) double *const_dptr;
+typedef const __attribute__((address_space(4))) int64_t *const_i64ptr;
+
 // one step of the chain: (dcur, dprev) <- (d^{l+1}, d^l) with entry i of the pair's
-// recurrence constants (zero past the last degree), from global memory (i is the same in
-// every lane: scalar loads) or from LDS
-__device__ __forceinline__ void rec_step(const double *__restrict__ rcp, int i, double cb, double &dcur,
-                                         double &dprev)
+// recurrence constants (zero past the last degree), from the constant table or from LDS
+template <class P>
+__device__ __forceinline__ void rec_step(P rcp, int i, double cb, double &dcur, double &dprev)
 {
     const double t = rcp[3 * i] * (cb - rcp[3 * i + 1]);   // one scalar operand per instruction
     const double dn = fma(t, dcur, -rcp[3 * i + 2] * dprev);
@@ -39,9 +44,13 @@ __device__ __forceinline__ void rec_step(const double *__restrict__ rcp, int i, 
     dcur = dn;
 }
 
+// plan tables live in global memory; the grouped kernels read their pointers from the
+// segment table, so the address space is restated for the compiler (no flat accesses)
+template <class T> using gptr = const __attribute__((address_space(1))) T *;
+
 __device__ __forceinline__ int64_t slice_m(const int64_t *slice, int m, int n, int bw)
 {
-    return slice[(int64_t)(m + bw - 1) * (2 * bw - 1) + (n + bw - 1)];
+    return ((gptr<int64_t>)slice)[(int64_t)(m + bw - 1) * (2 * bw - 1) + (n + bw - 1)];
 }
 
 __device__ __forceinline__ f4_t mma32(float a, float b, f4_t c)
@@ -95,10 +104,10 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
 
     const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
     // the pair's recurrence constants, staged in LDS (a read there has a short latency)
-    const double *rcp = rc + 3 * rc_off[tri];
+    const const_dptr rcp = (const_dptr)rc + 3 * ((const_i64ptr)rc_off)[tri];
     for (int i = lane; i < 3 * (L - 1 + REC_PAD); i += 64) cst[i] = rcp[i];
-    double dcur = (beta < N) ? seeds[tri * N + beta] : 0.0;
-    const double cb = (beta < N) ? cosb[beta] : 0.0;
+    double dcur = (beta < N) ? ((gptr<double>)seeds)[tri * N + beta] : 0.0;
+    const double cb = (beta < N) ? ((gptr<double>)cosb)[beta] : 0.0;
     double dprev = 0.0;
 
     f4_t ar[4], ai[4];
@@ -319,10 +328,10 @@ __device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64
     // staged by the caller, so that the only vector loads in the loop are the B operands
     // (vector loads complete in order)
     const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
-    const double *rcp = rc + 3 * rc_off[tri];
+    const const_dptr rcp = (const_dptr)rc + 3 * ((const_i64ptr)rc_off)[tri];
     for (int c = 0; c < nc; ++c) {
         const int beta = 64 * c + lane;
-        rs[c][lane] = make_double2(beta < N ? seeds[tri * N + beta] : 0.0, 0.0);
+        rs[c][lane] = make_double2(beta < N ? ((gptr<double>)seeds)[tri * N + beta] : 0.0, 0.0);
     }
     __syncthreads();   // rs (one wave: orders the LDS writes before the reads)
 
@@ -444,7 +453,8 @@ __device__ __forceinline__ void fwd_stage_sampling(double2 (*rs)[64], const doub
     const int lane = threadIdx.x & 63, nc = (N + 63) >> 6;
     for (int c = 0; c < nc; ++c) {
         const int beta = 64 * c + lane;
-        rs[nc + c][lane] = beta < N ? make_double2(cosb[beta], w[beta]) : make_double2(0.0, 0.0);
+        rs[nc + c][lane] = beta < N ? make_double2(((gptr<double>)cosb)[beta], ((gptr<double>)w)[beta])
+                                    : make_double2(0.0, 0.0);
     }
 }
 
