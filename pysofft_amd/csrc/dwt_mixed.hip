@@ -70,6 +70,12 @@ __device__ __forceinline__ void split(double d, float &hi, float &lo)
 constexpr int INV_DS = 72;  // padded row of the inverse d tile (float2 entries)
 constexpr int INV_TD = 8;   // degrees per tile: two K-steps of 4
 
+// dynamic LDS of the inverse: Dl [2][INV_TD][INV_DS] float2 | cst [3 (bw - 1 + REC_PAD)] double
+static size_t inv_lds(int bw)
+{
+    return sizeof(float2) * 2 * INV_TD * INV_DS + sizeof(double) * 3 * (bw - 1 + REC_PAD);
+}
+
 // one inverse work item: order pair (p, q), transforms t0 .. t0+TB-1 (< T), betas
 // 64 chunk .. 64 chunk + 63.  MFMA columns j = variant j & 7 of transform t0 + (j >> 3),
 // real and imaginary parts in separate MFMAs; degrees go in tiles of 8, K-step s of a
@@ -245,14 +251,15 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
 }
 
 template <int TB>
-__global__ __launch_bounds__(64) void dwt_inverse_mix(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dwt_inverse_mix(
     const float2 *__restrict__ coeff, int64_t c_ts, float2 *__restrict__ F, int64_t f_ts, int T,
     const double *__restrict__ cosb, const double *__restrict__ seeds, const double *__restrict__ rc,
     const int64_t *__restrict__ rc_off, const int64_t *__restrict__ slice, const int32_t *__restrict__ pairs, int bw,
     int N, int npair, int nch, int nunit)
 {
-    __shared__ float2 Dl[2][INV_TD][INV_DS];   // [tile parity][degree in tile][beta] = (hi, lo)
-    __shared__ double cst[3 * (256 - 1 + REC_PAD)];
+    extern __shared__ double smx[];   // [tile parity][degree in tile][beta] = (hi, lo) | constants
+    float2 (*Dl)[INV_TD][INV_DS] = reinterpret_cast<float2 (*)[INV_TD][INV_DS]>(smx);
+    double *cst = smx + 2 * INV_TD * INV_DS;
     const int id = blockIdx.x;
     const int chunk = (id >> 3) % nch;
     const int unit = ((id >> 3) / nch) * 8 + (id & 7);  // chunks of a unit share an XCD
@@ -265,12 +272,13 @@ __global__ __launch_bounds__(64) void dwt_inverse_mix(
 // grouped launch over transforms of several bandwidths: a persistent grid walks the
 // work list (item = segment, p | q << 16, first transform, chunk; segment -1 = padding)
 template <int TB>
-__global__ __launch_bounds__(64) void dwt_inverse_mix_grouped(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dwt_inverse_mix_grouped(
     const float2 *__restrict__ coeff, float2 *__restrict__ F, const MixSegment *__restrict__ segs,
     const int4 *__restrict__ items, int nitem)
 {
-    __shared__ float2 Dl[2][INV_TD][INV_DS];
-    __shared__ double cst[3 * (256 - 1 + REC_PAD)];
+    extern __shared__ double smx[];   // sized for the group's largest bandwidth
+    float2 (*Dl)[INV_TD][INV_DS] = reinterpret_cast<float2 (*)[INV_TD][INV_DS]>(smx);
+    double *cst = smx + 2 * INV_TD * INV_DS;
     for (int it = blockIdx.x; it < nitem; it += gridDim.x) {
         const int4 w = items[it];
         if (w.x < 0) continue;
@@ -527,8 +535,9 @@ hipError_t launch_dwt_inverse_mixed(const DeviceTables &tb, int bw, int N, int n
 #define SO3_MINV(TBV)                                                                                          \
     do {                                                                                                       \
         const int nunit = npair * ((T + TBV - 1) / TBV);                                                       \
-        hipLaunchKernelGGL((dwt_inverse_mix<TBV>), dim3(grid_m(nunit, nch)), dim3(64), 0, s, coeff, c_ts, F, f_ts, \
-                           T, tb.cosb, tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nch, nunit); \
+        hipLaunchKernelGGL((dwt_inverse_mix<TBV>), dim3(grid_m(nunit, nch)), dim3(64), inv_lds(bw), s, coeff,    \
+                           c_ts, F, f_ts, T, tb.cosb, tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N,     \
+                           npair, nch, nunit);                                                                  \
     } while (0)
     if (T >= 2) SO3_MINV(2);  // one transform pair per wave
     else SO3_MINV(1);
@@ -557,14 +566,15 @@ hipError_t launch_dwt_forward_mixed(const DeviceTables &tb, int bw, int N, int n
 
 // ---- grouped launches: TB (transforms per wave) and grid from the tuned configuration
 hipError_t launch_dwt_inverse_mixed_grouped(int TB, const float2 *coeff, float2 *F, const MixSegment *segs,
-                                            const int4 *items, int nitem, int grid, hipStream_t s)
+                                            const int4 *items, int nitem, int grid, int max_bw, hipStream_t s)
 {
     if (nitem <= 0) return hipSuccess;
     if (grid <= 0 || !fwd_aligned(F, 0)) return hipErrorInvalidValue;
+    const size_t lds = inv_lds(max_bw);
     if (TB == 2)
-        hipLaunchKernelGGL((dwt_inverse_mix_grouped<2>), dim3(grid), dim3(64), 0, s, coeff, F, segs, items, nitem);
+        hipLaunchKernelGGL((dwt_inverse_mix_grouped<2>), dim3(grid), dim3(64), lds, s, coeff, F, segs, items, nitem);
     else if (TB == 1)
-        hipLaunchKernelGGL((dwt_inverse_mix_grouped<1>), dim3(grid), dim3(64), 0, s, coeff, F, segs, items, nitem);
+        hipLaunchKernelGGL((dwt_inverse_mix_grouped<1>), dim3(grid), dim3(64), lds, s, coeff, F, segs, items, nitem);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -587,14 +597,15 @@ hipError_t launch_dwt_forward_mixed_grouped(int TB, const float2 *F, float2 *coe
     return hipGetLastError();
 }
 
-// resident blocks per CU of a grouped kernel (forward: LDS for bandwidth max_bw)
+// resident blocks per CU of a grouped kernel (LDS for bandwidth max_bw)
 int mixed_grouped_occupancy(int inverse, int TB, int max_bw)
 {
     int nb = 0;
     hipError_t e = hipErrorInvalidValue;
     if (inverse) {
-        if (TB == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_inverse_mix_grouped<2>, 64, 0);
-        else if (TB == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_inverse_mix_grouped<1>, 64, 0);
+        const size_t lds = inv_lds(max_bw);
+        if (TB == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_inverse_mix_grouped<2>, 64, lds);
+        else if (TB == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_inverse_mix_grouped<1>, 64, lds);
     } else {
         const size_t lds = fwd_lds(2 * max_bw);
         if (TB == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<4>, 64, lds);

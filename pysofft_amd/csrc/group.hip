@@ -253,7 +253,8 @@ inline int grid_of(int g, int nitem) { return g > 0 ? g : std::max(1, nitem); }
 
 int launch_inv_dwt(Group &G, const Config &c, const Items &it, const float2 *coeff, hipStream_t s)
 {
-    GCHK(launch_dwt_inverse_mixed_grouped(c.inv_tb, coeff, G.F, G.dsegs, it.d, it.n, grid_of(c.inv_grid, it.n), s));
+    GCHK(launch_dwt_inverse_mixed_grouped(c.inv_tb, coeff, G.F, G.dsegs, it.d, it.n, grid_of(c.inv_grid, it.n),
+                                          (int)G.max_bw, s));
     return SO3FT_OK;
 }
 
@@ -433,7 +434,7 @@ int tune(Group &G, hipStream_t s)
         std::vector<int4> v;
         Items it;
         if ((rc = build_items(G, true, TB, 1, v)) || (rc = upload_items(v, it))) break;
-        for (int g : grid_candidates(mixed_grouped_occupancy(1, TB, 0))) {
+        for (int g : grid_candidates(mixed_grouped_occupancy(1, TB, (int)G.max_bw))) {
             Config c = G.cfg;
             c.inv_tb = TB;
             c.inv_grid = g;

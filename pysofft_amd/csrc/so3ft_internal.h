@@ -116,7 +116,7 @@ struct MixSegment {
 // work items: x = segment (-1 = padding), y = p | q << 16, z = first transform, w = chunk
 // (64 betas for the inverse; 0 for the forward, whose item covers every beta and degree)
 hipError_t launch_dwt_inverse_mixed_grouped(int TB, const float2 *coeff, float2 *F, const MixSegment *segs,
-                                            const int4 *items, int nitem, int grid, hipStream_t s);
+                                            const int4 *items, int nitem, int grid, int max_bw, hipStream_t s);
 hipError_t launch_dwt_forward_mixed_grouped(int TB, const float2 *F, float2 *coeff, const MixSegment *segs,
                                             const int4 *items, int nitem, int grid, int max_bw, hipStream_t s);
 int mixed_grouped_occupancy(int inverse, int TB, int max_bw);
