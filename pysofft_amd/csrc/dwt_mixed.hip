@@ -1,24 +1,24 @@
 // Mixed-precision discrete Wigner transforms: complex64 coefficients and fft arrays,
 // FP64 Kostelec recurrence (pysofft/wigner.f90:227-260), contraction on
 // v_mfma_f32_16x16x4f32 with d split into FP32 hi + lo parts (two MFMAs per product,
-// products of an exact FP32 operand with hi and lo), FP32 accumulation over 16 terms
-// and FP64 accumulation across those blocks.  Work unit: an order pair 0 <= p <= q with
-// its up to 8 symmetric variants (pysofft/soft.f08:534-631, 937-1023); the MFMA N
-// dimension is the 16 columns (8 variants x re/im) of one transform.
+// products of an exact FP32 operand with hi and lo), FP32 accumulation over at most 16
+// terms and FP64 accumulation across those blocks.  Work unit: an order pair
+// 0 <= p <= q with its up to 8 symmetric variants (pysofft/soft.f08:534-631, 937-1023);
+// the MFMA N dimension is 16 columns = 8 variants x 2 transforms, real and imaginary
+// parts in separate MFMAs, so a lane always loads or stores whole complex values.
 //
 // MFMA operand layouts (v_mfma_f32_16x16x4f32, probed on gfx950): lane i + 16k holds
 // A[i][k] and B[k][i]; item r of the 4-float result of lane l is row 4(l>>4) + r, column
 // l & 15.
 //
-// inverse: one wave per (pair, 64-beta chunk, TB transforms).  Lane = beta runs the
-//   recurrence; every 4 degrees the wave writes d (hi, lo) for its 64 betas to a small
-//   LDS tile and reads it back as the A operands of four 16-beta tiles (M = beta,
-//   K = degree); B = the coefficients (K = degree, N = variant x re/im).
+// inverse: one wave per (pair, 64-beta chunk, 2 transforms).  Lane = beta runs the
+//   recurrence; every 8 degrees the wave writes d (hi, lo) for its 64 betas to an LDS
+//   tile and reads it back as the A operands of four 16-beta tiles (M = beta,
+//   K = degree); B = the coefficients (K = degree, N = columns).
 // forward: one wave per (pair, TB transforms), lane = beta within 64-beta chunks; for
 //   each 16-degree tile and chunk the lanes advance their recurrence and store
 //   w(beta) d (hi, lo) in LDS as [degree][beta]; A = that tile (M = degree, K = beta),
-//   B = the fft-array samples (K = beta, N = 8 variants x 2 transforms; real and
-//   imaginary parts in separate MFMAs, so a lane loads whole complex samples).
+//   B = the fft-array samples (K = beta, N = columns).
 #include "so3ft_internal.h"
 
 namespace so3 {
@@ -146,7 +146,7 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
         }
     };
     // one 8-degree tile: 2 K-steps x 4 beta tiles x (re, im) x (hi, lo) MFMAs with the
-    // ring slot r (loaded two tiles earlier, reloaded here for tile l0 + 16),
+    // ring slot (ra, rb) (loaded two tiles earlier, reloaded here for tile l0 + 16),
     // interleaved with the recurrence that fills the other LDS tile for the next one
     auto tile = [&](int l0, float2 &ra, float2 &rb) {
         const int buf = (l0 >> 3) & 1;
@@ -414,7 +414,10 @@ __device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64
                 // K-steps 2h, 2h+1 consumed: reload them for chunk cn (chunk 0 of the next
                 // tile after the last chunk; once more, unused, after the last tile), so
                 // that the loads have a whole chunk of work to arrive in
-                if (s4 & 1) load_b(cn, s4 >> 1);
+                if (s4 & 1) {
+                    load_b(cn, s4 >> 1);
+                    __builtin_amdgcn_sched_barrier(0);   // issued here, not after the last K-step
+                }
             }
 #pragma unroll
             for (int g = 0; g < TG; ++g)
