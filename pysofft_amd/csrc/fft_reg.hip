@@ -11,6 +11,8 @@
 // path and the mixed-precision path (FP64 butterflies on complex64 data).
 #include "so3ft_internal.h"
 
+#include <algorithm>
+
 namespace so3 {
 
 struct FftRegK {
@@ -245,10 +247,13 @@ static hipError_t launch_reg_typed(int N, const double2 *tw, int bw, int lmax, c
     k.col0 = col0; k.ncols = ncols;
     const int threads = (R1 > R2 ? R1 : R2) * k.C;
     const int bytes = (int)sizeof(double2) * (N + N * k.C);
-    // complex64 input: each block walks about four column tiles (prefetching the next);
-    // complex128 input: one tile per block
+    // complex64 input: each block walks up to 16 column tiles (prefetching the next; a
+    // block's first tile is the only one whose load latency is not hidden), as long as
+    // the grid keeps about 4096 blocks; complex128 input: one tile per block
     const int ntile = (ncols + k.C - 1) / k.C;
-    const int gx = (sizeof(TI) == 8) ? (ntile + 3) / 4 : ntile;
+    const int64_t lines = (int64_t)N * T;
+    const int gfill = (int)std::min<int64_t>(ntile, (4096 + lines - 1) / lines);
+    const int gx = (sizeof(TI) == 8) ? std::max((ntile + 15) / 16, gfill) : ntile;
     dim3 grid(gx, N, T), block(threads);
 #define SO3_REG(A, B)                                                                                   \
     do {                                                                                                \
