@@ -232,9 +232,10 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
  * the pipeline `reps` times on `stream` between two HIP events and returns the mean
  * duration of one launch.  which: 0 inverse DWT (a = coeffs, b = T x (2bw)^3 scratch),
  * 1 forward DWT (a = scratch, b = coeffs), 2 FFT pass along alpha (a -> b),
- * 3 FFT pass along gamma (a -> b), 4 / 5 as 0 / 1 on the VALU engine, 6 / 7 the
- * mixed-precision (complex64) inverse / forward DWT, 8 / 9 the complex64 FFT passes
- * along alpha / gamma.  T transforms per launch. */
+ * 3 FFT pass along gamma (a -> b), 4 / 5 the default engine's FP64 inverse (VALU) /
+ * forward (FP64 MFMA, lane = beta), 6 / 7 the mixed-precision (complex64) inverse /
+ * forward DWT, 8 / 9 the complex64 FFT passes along alpha / gamma, 12 the FP64 forward
+ * on the VALU.  T transforms per launch. */
 int so3ft_bench_kernel(so3ft_handle h, int which, int reps, void *a, void *b, int64_t T, void *stream,
                        float *ms_per_launch);
 

@@ -515,6 +515,131 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void dw
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 forward DWT on v_mfma_f64_16x16x4f64 with the structure of the mixed forward:
+// one wave per (pair, 2 transforms), lane = beta within 64-beta chunks, chain state in
+// LDS between 16-degree tiles, A = w d (FP64) tile [degree][beta] in LDS, B = the
+// complex128 samples (8 variants x 2 transforms; real and imaginary parts in separate
+// MFMAs), FP64 accumulation in the MFMA accumulators across every chunk.  K-step s of
+// lane row k takes beta 4 s + k of the chunk (the four rows of a column read 64
+// contiguous bytes).  f64 16x16x4 result item r of lane l: row (l >> 4) + 4 r, column
+// l & 15 (probed on gfx950).
+// ---------------------------------------------------------------------------
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4_t mma64(double a, double b, d4_t c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr int F64_DS = 66;   // padded row of the FP64 A tile (doubles)
+
+template <int TB>
+__device__ __forceinline__ void fwd_m64_item(const double2 *__restrict__ F, int64_t f_ts,
+                                             double2 *__restrict__ coeff, int64_t c_ts, int T,
+                                             const double *__restrict__ seeds, const double *__restrict__ rc,
+                                             const int64_t *__restrict__ rc_off, const int64_t *__restrict__ slice,
+                                             int bw, int N, int p, int q, int t0, double (*Dt)[F64_DS],
+                                             double2 (*rs)[64])
+{
+    static_assert(TB == 1 || TB == 2, "one transform pair per wave");
+    const int lane = threadIdx.x & 63;
+    const int L = bw - q;
+    const int nc = (N + 63) >> 6;
+    const int nt = min(TB, T - t0);
+    const int j = lane & 15, kk = lane >> 4;
+    const int vj = j & 7, tt = j >> 3;
+    const Variant vr = variant(vj, p, q);
+    const int64_t colj = ((int64_t)wrap_m(vr.n, N) * N + wrap_m(vr.m, N)) * N;
+    const double2 *src = F + (int64_t)max(0, min(t0 + tt, T - 1)) * f_ts + colj;
+
+    const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
+    const const_dptr rcp = (const_dptr)rc + 3 * ((const_i64ptr)rc_off)[tri];
+    for (int c = 0; c < nc; ++c) {
+        const int beta = 64 * c + lane;
+        rs[c][lane] = make_double2(beta < N ? ((gptr<double>)seeds)[tri * N + beta] : 0.0, 0.0);
+    }
+    __syncthreads();   // rs (one wave: orders the LDS writes before the reads)
+
+    // B operand of K-step s of chunk c: the sample at beta 64 c + 4 s + k (clamped into
+    // the column; rows past N meet zero A columns)
+    double2 bv[16];
+    auto load_b = [&](int c, int s4) {
+        const int bt = min(64 * c + 4 * s4 + kk, N - 1);
+        bv[s4] = src[vr.rev ? N - 1 - bt : bt];
+    };
+#pragma unroll
+    for (int s4 = 0; s4 < 16; ++s4) load_b(0, s4);
+
+#pragma unroll 1
+    for (int l0 = 0; l0 < L; l0 += 16) {
+        d4_t are[2], aim[2];   // two independent chains per part
+#pragma unroll
+        for (int u = 0; u < 2; ++u) are[u] = aim[u] = d4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int c = 0; c < nc; ++c) {
+            const int cn = c + 1 == nc ? 0 : c + 1;   // chunk whose operands the K-steps reload
+            const double2 cw = rs[nc + c][lane];
+            const double cb = cw.x, wt = cw.y;
+            asm volatile("" ::: "memory");
+            double2 st = rs[c][lane];
+            double dcur = st.x, dprev = st.y;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {   // the chain is zero past the last degree
+                Dt[k][lane] = wt * dcur;
+                rec_step(rcp, l0 + k, cb, dcur, dprev);
+            }
+            rs[c][lane] = make_double2(dcur, dprev);
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int s4 = 0; s4 < 16; ++s4) {
+                const double a = Dt[j][4 * s4 + kk];
+                const int u = s4 & 1;
+                are[u] = mma64(a, bv[s4].x, are[u]);
+                aim[u] = mma64(a, bv[s4].y, aim[u]);
+                load_b(cn, s4);   // consumed: reload for the next chunk (once more, unused, at the end)
+                if ((s4 & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        // item r of lane l: degree row (l >> 4) + 4 r, column j = l & 15
+        if (vr.valid && tt < nt) {
+            const int64_t o = slice_m(slice, vr.m, vr.n, bw);
+            double2 *ct = coeff + (int64_t)(t0 + tt) * c_ts + o;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int li = l0 + kk + 4 * r;
+                if (li < L) {
+                    const double sg = vr.sign * ((vr.lsign && ((q + li) & 1)) ? -1.0 : 1.0);
+                    ct[li] = make_double2(sg * (are[0][r] + are[1][r]), sg * (aim[0][r] + aim[1][r]));
+                }
+            }
+        }
+    }
+}
+
+template <int TB>
+__global__ __launch_bounds__(64) void dwt_forward_m64(const double2 *__restrict__ F, int64_t f_ts,
+                                                      double2 *__restrict__ coeff, int64_t c_ts, int T,
+                                                      const double *__restrict__ w, const double *__restrict__ cosb,
+                                                      const double *__restrict__ seeds,
+                                                      const double *__restrict__ rc,
+                                                      const int64_t *__restrict__ rc_off,
+                                                      const int64_t *__restrict__ slice,
+                                                      const int32_t *__restrict__ pairs, int bw, int N, int npair,
+                                                      int nunit)
+{
+    // dynamic LDS: Dt [16][F64_DS] double | rs [2 nc][64] double2 (chain states, then (cos beta, w))
+    extern __shared__ double smx[];
+    double (*Dt)[F64_DS] = reinterpret_cast<double (*)[F64_DS]>(smx);
+    double2 (*rs)[64] = reinterpret_cast<double2 (*)[64]>(smx + 16 * F64_DS);
+    const int unit = blockIdx.x;
+    if (unit >= nunit) return;
+    const int pq = pairs[unit % npair];
+    fwd_stage_sampling(rs, cosb, w, N);
+    fwd_m64_item<TB>(F, f_ts, coeff, c_ts, T, seeds, rc, rc_off, slice, bw, N, pq & 0xffff, pq >> 16,
+                     (unit / npair) * TB, Dt, rs);
+}
+
 static int grid_m(int nunit, int nch) { return (nunit + 7) / 8 * 8 * nch; }
 
 // dynamic LDS bytes of dwt_forward_mix (A tile of 16 degrees, chunk states and sampling)
@@ -564,6 +689,24 @@ hipError_t launch_dwt_forward_mixed(const DeviceTables &tb, int bw, int N, int n
     else if (T >= 2) SO3_MFWD(2);
     else SO3_MFWD(1);
 #undef SO3_MFWD
+    return hipGetLastError();
+}
+
+// FP64 forward on FP64 MFMA (complex mode; 16-byte aligned fft array, even stride)
+hipError_t launch_dwt_forward_m64(const DeviceTables &tb, int bw, int N, int npair, const double2 *F, int64_t f_ts,
+                                  double2 *coeff, int64_t c_ts, int T, hipStream_t s)
+{
+    if (reinterpret_cast<uintptr_t>(F) & 15) return hipErrorInvalidValue;
+    if (npair <= 0 || T <= 0) return hipSuccess;
+    const size_t lds = sizeof(double) * (16 * F64_DS + 4 * 64 * ((N + 63) / 64));
+    const int TB = T >= 2 ? 2 : 1;
+    const int nunit = npair * ((T + TB - 1) / TB);
+    if (TB == 2)
+        hipLaunchKernelGGL((dwt_forward_m64<2>), dim3(nunit), dim3(64), lds, s, F, f_ts, coeff, c_ts, T, tb.w, tb.cosb,
+                           tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit);
+    else
+        hipLaunchKernelGGL((dwt_forward_m64<1>), dim3(nunit), dim3(64), lds, s, F, f_ts, coeff, c_ts, T, tb.w, tb.cosb,
+                           tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit);
     return hipGetLastError();
 }
 
