@@ -539,7 +539,7 @@ __device__ __forceinline__ void fwd_m64_item(const double2 *__restrict__ F, int6
                                              double2 *__restrict__ coeff, int64_t c_ts, int T,
                                              const double *__restrict__ seeds, const double *__restrict__ rc,
                                              const int64_t *__restrict__ rc_off, const int64_t *__restrict__ slice,
-                                             int bw, int N, int p, int q, int t0, double (*Dt)[F64_DS],
+                                             int bw, int N, int p, int q, int t0, int real, double (*Dt)[F64_DS],
                                              double2 (*rs)[64])
 {
     static_assert(TB == 1 || TB == 2, "one transform pair per wave");
@@ -550,7 +550,15 @@ __device__ __forceinline__ void fwd_m64_item(const double2 *__restrict__ F, int6
     const int j = lane & 15, kk = lane >> 4;
     const int vj = j & 7, tt = j >> 3;
     const Variant vr = variant(vj, p, q);
-    const int64_t colj = ((int64_t)wrap_m(vr.n, N) * N + wrap_m(vr.m, N)) * N;
+    // real mode (rsoft, half-complex fft array): the odd variants read the conjugate of an
+    // m >= 0 column (soft.f08:1650, 1662-1706)
+    int mm = vr.m, nn = vr.n;
+    if (real && (vj & 1)) {
+        const Variant s2 = variant(vj == 1 ? (p == q ? 0 : 2) : (vj == 3 ? 0 : vj - 1), p, q);
+        mm = s2.m;
+        nn = s2.n;
+    }
+    const int64_t colj = ((int64_t)wrap_m(nn, N) * N + wrap_m(mm, N)) * N;
     const double2 *src = F + (int64_t)max(0, min(t0 + tt, T - 1)) * f_ts + colj;
 
     const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
@@ -610,7 +618,8 @@ __device__ __forceinline__ void fwd_m64_item(const double2 *__restrict__ F, int6
                 const int li = l0 + kk + 4 * r;
                 if (li < L) {
                     const double sg = vr.sign * ((vr.lsign && ((q + li) & 1)) ? -1.0 : 1.0);
-                    ct[li] = make_double2(sg * (are[0][r] + are[1][r]), sg * (aim[0][r] + aim[1][r]));
+                    const double sy = (real && (vj & 1)) ? -sg : sg;
+                    ct[li] = make_double2(sg * (are[0][r] + are[1][r]), sy * (aim[0][r] + aim[1][r]));
                 }
             }
         }
@@ -626,7 +635,7 @@ __global__ __launch_bounds__(64) void dwt_forward_m64(const double2 *__restrict_
                                                       const int64_t *__restrict__ rc_off,
                                                       const int64_t *__restrict__ slice,
                                                       const int32_t *__restrict__ pairs, int bw, int N, int npair,
-                                                      int nunit)
+                                                      int nunit, int real)
 {
     // dynamic LDS: Dt [16][F64_DS] double | rs [2 nc][64] double2 (chain states, then (cos beta, w))
     extern __shared__ double smx[];
@@ -637,7 +646,7 @@ __global__ __launch_bounds__(64) void dwt_forward_m64(const double2 *__restrict_
     const int pq = pairs[unit % npair];
     fwd_stage_sampling(rs, cosb, w, N);
     fwd_m64_item<TB>(F, f_ts, coeff, c_ts, T, seeds, rc, rc_off, slice, bw, N, pq & 0xffff, pq >> 16,
-                     (unit / npair) * TB, Dt, rs);
+                     (unit / npair) * TB, real, Dt, rs);
 }
 
 static int grid_m(int nunit, int nch) { return (nunit + 7) / 8 * 8 * nch; }
@@ -692,21 +701,22 @@ hipError_t launch_dwt_forward_mixed(const DeviceTables &tb, int bw, int N, int n
     return hipGetLastError();
 }
 
-// FP64 forward on FP64 MFMA (complex mode; 16-byte aligned fft array, even stride)
-hipError_t launch_dwt_forward_m64(const DeviceTables &tb, int bw, int N, int npair, const double2 *F, int64_t f_ts,
-                                  double2 *coeff, int64_t c_ts, int T, hipStream_t s)
+// FP64 forward on FP64 MFMA (modes DWT_COMPLEX and DWT_REAL)
+hipError_t launch_dwt_forward_m64(const DeviceTables &tb, int bw, int N, int npair, int mode, const double2 *F,
+                                  int64_t f_ts, double2 *coeff, int64_t c_ts, int T, hipStream_t s)
 {
-    if (reinterpret_cast<uintptr_t>(F) & 15) return hipErrorInvalidValue;
+    if (mode != DWT_COMPLEX && mode != DWT_REAL) return hipErrorInvalidValue;
+    const int real = mode == DWT_REAL;
     if (npair <= 0 || T <= 0) return hipSuccess;
     const size_t lds = sizeof(double) * (16 * F64_DS + 4 * 64 * ((N + 63) / 64));
     const int TB = T >= 2 ? 2 : 1;
     const int nunit = npair * ((T + TB - 1) / TB);
     if (TB == 2)
         hipLaunchKernelGGL((dwt_forward_m64<2>), dim3(nunit), dim3(64), lds, s, F, f_ts, coeff, c_ts, T, tb.w, tb.cosb,
-                           tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit);
+                           tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit, real);
     else
         hipLaunchKernelGGL((dwt_forward_m64<1>), dim3(nunit), dim3(64), lds, s, F, f_ts, coeff, c_ts, T, tb.w, tb.cosb,
-                           tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit);
+                           tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit, real);
     return hipGetLastError();
 }
 

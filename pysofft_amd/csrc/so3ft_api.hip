@@ -391,8 +391,8 @@ int dwt_forward(Plan *P, int mode, const double2 *F, int64_t f_ts, double2 *c, i
     const int bw = (int)P->bw, N = (int)P->N, np = (int)P->npair;
     if (P->dwt_engine == SO3FT_DWT_MFMA)
         HIPCHK(launch_dwt_forward(P->tb, bw, N, np, mode, F, f_ts, c, c_ts, (int)T, s));
-    else if (mode == DWT_COMPLEX)   // lane-beta FP64 MFMA forward (FP64 throughout)
-        HIPCHK(launch_dwt_forward_m64(P->tb, bw, N, np, F, f_ts, c, c_ts, (int)T, s));
+    else if (mode == DWT_COMPLEX || mode == DWT_REAL)   // lane-beta FP64 MFMA forward (FP64 throughout)
+        HIPCHK(launch_dwt_forward_m64(P->tb, bw, N, np, mode, F, f_ts, c, c_ts, (int)T, s));
     else
         HIPCHK(launch_dwt_forward_valu(P->tb, bw, N, np, mode, F, f_ts, c, c_ts, (int)T, s));
     return SO3FT_OK;
@@ -1028,7 +1028,7 @@ int so3ft_bench_kernel(so3ft_handle h, int which, int reps, void *a, void *b, in
                                            reinterpret_cast<const double2 *>(a), P->nc,
                                            reinterpret_cast<double2 *>(b), N3, (int)T, s);
         case 5:   // the default FP64 forward (complex mode)
-            return launch_dwt_forward_m64(P->tb, (int)P->bw, (int)N, (int)P->npair,
+            return launch_dwt_forward_m64(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX,
                                           reinterpret_cast<const double2 *>(a), N3, reinterpret_cast<double2 *>(b),
                                           P->nc, (int)T, s);
         case 12:  // the FP64 forward on the vector ALUs

@@ -120,10 +120,10 @@ hipError_t launch_dwt_inverse_mixed_grouped(int TB, const float2 *coeff, float2 
 hipError_t launch_dwt_forward_mixed_grouped(int TB, const float2 *F, float2 *coeff, const MixSegment *segs,
                                             const int4 *items, int nitem, int grid, int max_bw, hipStream_t s);
 int mixed_grouped_occupancy(int inverse, int TB, int max_bw);
-// FP64 forward DWT (complex mode) on FP64 MFMA with the lane-beta structure of the mixed
-// forward (dwt_mixed.hip)
-hipError_t launch_dwt_forward_m64(const DeviceTables &tb, int bw, int N, int npair, const double2 *F, int64_t f_ts,
-                                  double2 *coeff, int64_t c_ts, int T, hipStream_t s);
+// FP64 forward DWT (modes DWT_COMPLEX and DWT_REAL) on FP64 MFMA with the lane-beta
+// structure of the mixed forward (dwt_mixed.hip)
+hipError_t launch_dwt_forward_m64(const DeviceTables &tb, int bw, int N, int npair, int mode, const double2 *F,
+                                  int64_t f_ts, double2 *coeff, int64_t c_ts, int T, hipStream_t s);
 
 struct FftAxis {
     int64_t in_t, in_o, in_r;      // element strides: transform, outer line, fft row
