@@ -67,9 +67,11 @@ int so3ft_wigners_are_precomputed(so3ft_handle h, int *yes);
 int so3ft_get_fftw_flags(so3ft_handle h, int32_t *flags);
 
 /* ---- FP64 DWT engine selection (no counterpart in the reference, which has one
- * CPU GEMV path): SO3FT_DWT_VALU (default) runs the contraction on the FP64 vector
- * ALUs, SO3FT_DWT_MFMA on v_mfma_f64_16x16x4f64.  Both are FP64 throughout and give
- * the same results to rounding.  Kept across so3ft_reset. */
+ * CPU GEMV path): SO3FT_DWT_VALU (default) runs the inverse contraction on the FP64
+ * vector ALUs (lane = beta) and the forward on v_mfma_f64_16x16x4f64 with lane = beta
+ * recurrence chains; SO3FT_DWT_MFMA runs both on v_mfma_f64_16x16x4f64 with 16-degree
+ * x 64-beta tiles of d per order pair.  All are FP64 throughout and give the same
+ * results to rounding.  Kept across so3ft_reset. */
 enum { SO3FT_DWT_VALU = 0, SO3FT_DWT_MFMA = 1 };
 int so3ft_set_dwt_engine(so3ft_handle h, int engine);
 int so3ft_get_dwt_engine(so3ft_handle h, int *engine);

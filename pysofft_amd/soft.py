@@ -141,7 +141,9 @@ class Soft:
 
     @property
     def dwt_engine(self):
-        """FP64 DWT engine: "valu" (vector FP64 FMA, default) or "mfma" (v_mfma_f64_16x16x4f64)."""
+        """FP64 DWT engine: "valu" (default: inverse on the FP64 vector ALUs, forward on
+        FP64 MFMA with lane-beta recurrence chains) or "mfma" (both directions on
+        v_mfma_f64_16x16x4f64 with per-pair d tiles)."""
         return {0: "valu", 1: "mfma"}[self._get(_lib.lib().so3ft_get_dwt_engine, C.c_int)]
 
     @dwt_engine.setter
