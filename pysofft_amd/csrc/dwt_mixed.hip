@@ -240,8 +240,11 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
                                      : make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
             const float4 w1 = vr.rev ? make_float4(v[1].x, v[1].y, v[0].x, v[0].y)
                                      : make_float4(v[2].x, v[2].y, v[3].x, v[3].y);
-            *reinterpret_cast<float4 *>(Ft + m0) = w0;
-            *reinterpret_cast<float4 *>(Ft + m0 + 2) = w1;
+            // written once, read by the next pass from HBM: non-temporal stores
+            typedef float f4v __attribute__((ext_vector_type(4)));
+            const f4v x0 = {w0.x, w0.y, w0.z, w0.w}, x1 = {w1.x, w1.y, w1.z, w1.w};
+            __builtin_nontemporal_store(x0, reinterpret_cast<f4v *>(Ft + m0));
+            __builtin_nontemporal_store(x1, reinterpret_cast<f4v *>(Ft + m0 + 2));
         } else {
 #pragma unroll
             for (int r = 0; r < 4; ++r)
