@@ -239,7 +239,10 @@ __global__ __launch_bounds__(64) void dwt_inverse_valu(const double2 *__restrict
 #pragma unroll
         for (int t = 0; t < TB; ++t) {
             if (t >= nt) continue;
-            F[(int64_t)(t0 + t) * f_ts + col] = make_double2(sr.sign * acc[t][2 * v], sy * acc[t][2 * v + 1]);
+            // written once, read by the next pass from HBM: a non-temporal store
+            typedef double d2v __attribute__((ext_vector_type(2)));
+            const d2v x = {sr.sign * acc[t][2 * v], sy * acc[t][2 * v + 1]};
+            __builtin_nontemporal_store(x, reinterpret_cast<d2v *>(F + (int64_t)(t0 + t) * f_ts + col));
         }
     }
 }
