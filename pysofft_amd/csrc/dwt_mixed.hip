@@ -652,6 +652,110 @@ __global__ __launch_bounds__(64) void dwt_forward_m64(const double2 *__restrict_
                      (unit / npair) * TB, real, Dt, rs);
 }
 
+// FP64 forward, one block of nc = N / 64 waves per (pair, 2 transforms): wave c owns beta
+// chunk c, its recurrence chains and its B operands (the complex128 samples of its chunk,
+// 16 registers' worth of K-steps) stay in registers across all degree tiles, so every
+// fft-array byte is read once (the single-wave kernel re-reads each column once per
+// 16-degree tile and is bound by those bytes).  Per tile each wave writes its w d tile
+// to its own LDS tile, runs its 16 K-steps, and leaves its chunk's partial sums in that
+// tile; after a barrier the block adds the partials in chunk order and writes the
+// tile's coefficients.
+template <int TB>
+__global__ __launch_bounds__(512) void dwt_forward_m64w(
+    const double2 *__restrict__ F, int64_t f_ts, double2 *__restrict__ coeff, int64_t c_ts, int T,
+    const double *__restrict__ w, const double *__restrict__ cosb, const double *__restrict__ seeds,
+    const double *__restrict__ rc, const int64_t *__restrict__ rc_off, const int64_t *__restrict__ slice,
+    const int32_t *__restrict__ pairs, int bw, int N, int npair, int nunit, int real)
+{
+    static_assert(TB == 1 || TB == 2, "one transform pair per wave");
+    constexpr int NV = 2 * 4;   // partials per lane: (re, im) x result item
+    static_assert(NV * 64 <= 16 * F64_DS, "partials fit the wave's tile");
+    // dynamic LDS: Dt [nc][16][F64_DS] double; after a tile's K-steps each wave's tile
+    // holds its partials [NV][64] double
+    extern __shared__ double smx[];
+    const int unit = blockIdx.x;
+    if (unit >= nunit) return;
+    const int nc = blockDim.x >> 6;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double (*Dt)[F64_DS] = reinterpret_cast<double (*)[F64_DS]>(smx) + 16 * wv;
+    auto red = [&](int c, int v) -> double * { return smx + (int64_t)c * 16 * F64_DS + v * 64; };
+    const int pq = pairs[unit % npair], p = pq & 0xffff, q = pq >> 16;
+    const int t0 = (unit / npair) * TB;
+    const int L = bw - q;
+    const int nt = min(TB, T - t0);
+    const int j = lane & 15, kk = lane >> 4;
+    const int vj = j & 7, tt = j >> 3;
+    const Variant vr = variant(vj, p, q);
+    int mm = vr.m, nn = vr.n;   // real mode: odd variants read the conjugate of an m >= 0 column
+    if (real && (vj & 1)) {
+        const Variant s2 = variant(vj == 1 ? (p == q ? 0 : 2) : (vj == 3 ? 0 : vj - 1), p, q);
+        mm = s2.m;
+        nn = s2.n;
+    }
+    const int64_t colj = ((int64_t)wrap_m(nn, N) * N + wrap_m(mm, N)) * N;
+    const double2 *src = F + (int64_t)max(0, min(t0 + tt, T - 1)) * f_ts + colj;
+
+    const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
+    const const_dptr rcp = (const_dptr)rc + 3 * ((const_i64ptr)rc_off)[tri];
+    const int beta = 64 * wv + lane;
+    const bool ok = beta < N;
+    const double cb = ok ? ((gptr<double>)cosb)[beta] : 0.0, wt = ok ? ((gptr<double>)w)[beta] : 0.0;
+    double dcur = ok ? ((gptr<double>)seeds)[tri * N + beta] : 0.0, dprev = 0.0;
+    double2 bv[16];   // K-step s: the sample at beta 64 c + 4 s + k (clamped; zero A rows past N)
+#pragma unroll
+    for (int s4 = 0; s4 < 16; ++s4) {
+        const int bt = min(64 * wv + 4 * s4 + kk, N - 1);
+        bv[s4] = src[vr.rev ? N - 1 - bt : bt];
+    }
+    // this thread's output column (lane l = threadIdx.x & 63 of the result layout)
+    const int out_l = threadIdx.x & 63, out_j = out_l & 15, out_tt = out_j >> 3;
+    const Variant vo = variant(out_j & 7, p, q);
+    const int64_t out_off = vo.valid ? slice_m(slice, vo.m, vo.n, bw) : 0;
+    const bool out_ok = vo.valid && out_tt < nt;
+
+#pragma unroll 1
+    for (int l0 = 0; l0 < L; l0 += 16) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {   // the chain is zero past the last degree
+            Dt[k][lane] = wt * dcur;
+            rec_step(rcp, l0 + k, cb, dcur, dprev);
+        }
+        d4_t are[2], aim[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) are[u] = aim[u] = d4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s4 = 0; s4 < 16; ++s4) {
+            const double a = Dt[j][4 * s4 + kk];
+            const int u = s4 & 1;
+            are[u] = mma64(a, bv[s4].x, are[u]);
+            aim[u] = mma64(a, bv[s4].y, aim[u]);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            red(wv, r)[lane] = are[0][r] + are[1][r];
+            red(wv, 4 + r)[lane] = aim[0][r] + aim[1][r];
+        }
+        __syncthreads();
+        // item r of lane l: degree row (l >> 4) + 4 r, column l & 15; thread t of the block
+        // takes lane t & 63 and rows r = t >> 6, t >> 6 + nc, ...
+        if (out_ok) {
+            for (int r = threadIdx.x >> 6; r < 4; r += nc) {
+                const int li = l0 + (out_l >> 4) + 4 * r;
+                if (li >= L) continue;
+                double sr = 0.0, si = 0.0;
+                for (int c = 0; c < nc; ++c) {
+                    sr += red(c, r)[out_l];
+                    si += red(c, 4 + r)[out_l];
+                }
+                const double sg = vo.sign * ((vo.lsign && ((q + li) & 1)) ? -1.0 : 1.0);
+                const double sy = (real && (out_j & 1)) ? -sg : sg;
+                coeff[(int64_t)(t0 + out_tt) * c_ts + out_off + li] = make_double2(sg * sr, sy * si);
+            }
+        }
+        __syncthreads();   // the tiles (holding the partials) are rewritten by the next tile
+    }
+}
+
 static int grid_m(int nunit, int nch) { return (nunit + 7) / 8 * 8 * nch; }
 
 // dynamic LDS bytes of dwt_forward_mix (A tile of 16 degrees, chunk states and sampling)
@@ -711,9 +815,20 @@ hipError_t launch_dwt_forward_m64(const DeviceTables &tb, int bw, int N, int npa
     if (mode != DWT_COMPLEX && mode != DWT_REAL) return hipErrorInvalidValue;
     const int real = mode == DWT_REAL;
     if (npair <= 0 || T <= 0) return hipSuccess;
-    const size_t lds = sizeof(double) * (16 * F64_DS + 4 * 64 * ((N + 63) / 64));
     const int TB = T >= 2 ? 2 : 1;
     const int nunit = npair * ((T + TB - 1) / TB);
+    const int nc = (N + 63) / 64;
+    if (nc >= 2 && nc <= 8) {   // one wave per 64-beta chunk: fft-array samples read once
+        const size_t ldsw = sizeof(double) * (size_t)nc * 16 * F64_DS;
+        if (TB == 2)
+            hipLaunchKernelGGL((dwt_forward_m64w<2>), dim3(nunit), dim3(64 * nc), ldsw, s, F, f_ts, coeff, c_ts, T,
+                               tb.w, tb.cosb, tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit, real);
+        else
+            hipLaunchKernelGGL((dwt_forward_m64w<1>), dim3(nunit), dim3(64 * nc), ldsw, s, F, f_ts, coeff, c_ts, T,
+                               tb.w, tb.cosb, tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit, real);
+        return hipGetLastError();
+    }
+    const size_t lds = sizeof(double) * (16 * F64_DS + 4 * 64 * ((N + 63) / 64));
     if (TB == 2)
         hipLaunchKernelGGL((dwt_forward_m64<2>), dim3(nunit), dim3(64), lds, s, F, f_ts, coeff, c_ts, T, tb.w, tb.cosb,
                            tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit, real);
