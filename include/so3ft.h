@@ -91,8 +91,8 @@ int so3ft_rsoft_many(so3ft_handle h, const void *so3funcs_real, void *coeffs, in
 /* ---- mixed precision (no counterpart in the reference, whose transforms are FP64
  * only): complex64 coefficients (n, NC) and complex64 SO(3) functions (n, 2bw, 2bw,
  * 2bw); the Wigner-d recurrence, quadrature weights and FFT butterflies run in FP64,
- * the DWT contraction on FP32 MFMA with d split into FP32 hi + lo parts and FP64
- * accumulation across 16-degree blocks.  Needs bw = 2^a or 3*2^a (else
+ * the DWT contraction on the matrix cores with d split into FP32 hi + lo parts (hi
+ * products in FP32, lo products in bfloat16) and FP64 accumulation across 16-degree blocks.  Needs bw = 2^a or 3*2^a (else
  * SO3FT_EUNSUPPORTED).  Same semantics as so3ft_isoft_many / so3ft_soft_many. */
 int so3ft_isoft_many_c64(so3ft_handle h, const void *coeffs, void *so3funcs, int64_t n, void *stream);
 int so3ft_soft_many_c64(so3ft_handle h, const void *so3funcs, void *coeffs, int64_t n, void *stream);

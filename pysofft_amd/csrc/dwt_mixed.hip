@@ -1,8 +1,9 @@
 // Mixed-precision discrete Wigner transforms: complex64 coefficients and fft arrays,
-// FP64 Kostelec recurrence (pysofft/wigner.f90:227-260), contraction on
-// v_mfma_f32_16x16x4f32 with d split into FP32 hi + lo parts (two MFMAs per product,
-// products of an exact FP32 operand with hi and lo), FP32 accumulation over at most 16
-// terms and FP64 accumulation across those blocks.  Work unit: an order pair
+// FP64 Kostelec recurrence (pysofft/wigner.f90:227-260), contraction on the matrix cores
+// with d split into FP32 hi + lo parts: the hi products on v_mfma_f32_16x16x4f32 (exact
+// FP32 operands), the lo products, four K-steps at a time, on v_mfma_f32_16x16x16_bf16
+// (they are below 2^-24 of the result, so bfloat16 operands suffice); FP32 accumulation
+// over at most 16 terms and FP64 accumulation across those blocks.  Work unit: an order pair
 // 0 <= p <= q with its up to 8 symmetric variants (pysofft/soft.f08:534-631, 937-1023);
 // the MFMA N dimension is 16 columns = 8 variants x 2 transforms, real and imaginary
 // parts in separate MFMAs, so a lane always loads or stores whole complex values.
@@ -12,14 +13,17 @@
 // l & 15.
 //
 // inverse: one wave per (pair, 64-beta chunk, 2 transforms).  Lane = beta runs the
-//   recurrence; every 8 degrees the wave writes d (hi, lo) for its 64 betas to an LDS
-//   tile and reads it back as the A operands of four 16-beta tiles (M = beta,
-//   K = degree); B = the coefficients (K = degree, N = columns).
+//   recurrence; every 8 degrees the wave writes d for its 64 betas to LDS (hi parts as
+//   an FP32 tile, lo parts as bfloat16 in the slot order of the bfloat16 MFMA) and reads
+//   them back as the A operands of four 16-beta tiles (M = beta, K = degree); B = the
+//   coefficients (K = degree, N = columns).
 // forward: one wave per (pair, TB transforms), lane = beta within 64-beta chunks; for
 //   each 16-degree tile and chunk the lanes advance their recurrence and store
 //   w(beta) d (hi, lo) in LDS as [degree][beta]; A = that tile (M = degree, K = beta),
 //   B = the fft-array samples (K = beta, N = columns).
 #include "so3ft_internal.h"
+
+#include <type_traits>
 
 namespace so3 {
 
@@ -58,6 +62,20 @@ __device__ __forceinline__ f4_t mma32(float a, float b, f4_t c)
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// the lo products on the bfloat16 matrix path: v_mfma_f32_16x16x16_bf16, lane i + 16k
+// holding A[i][4k + r] and B[4k + r][i] in slot r (probed on gfx950), result layout as
+// mma32.  A lo part is below 2^-24 of its d, so rounding it and the other operand to
+// bfloat16 (8-bit significands) changes a product by under 2^-32 of d times that operand:
+// four K-steps of lo products cost one instruction instead of four FP32 ones.
+typedef __bf16 bf4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf2_t __attribute__((ext_vector_type(2)));
+typedef short s4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f4_t mma_bf16(bf4_t a, bf4_t b, f4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4_t, a), __builtin_bit_cast(s4_t, b), c, 0,
+                                                     0, 0);
+}
+
 __device__ __forceinline__ void split(double d, float &hi, float &lo)
 {
     hi = (float)d;
@@ -67,20 +85,25 @@ __device__ __forceinline__ void split(double d, float &hi, float &lo)
 // ---------------------------------------------------------------------------
 // inverse
 // ---------------------------------------------------------------------------
-constexpr int INV_DS = 72;  // padded row of the inverse d tile (float2 entries)
+constexpr int INV_DS = 72;  // padded row of the inverse hi tile (floats)
+constexpr int INV_LS = 80;  // padded row of the inverse lo table (4 bfloat16 per beta)
 constexpr int INV_TD = 8;   // degrees per tile: two K-steps of 4
 
-// dynamic LDS of the inverse: Dl [2][INV_TD][INV_DS] float2 | cst [3 (bw - 1 + REC_PAD)] double
+// dynamic LDS of the inverse: Dh [2][INV_TD][INV_DS] float | Dlo [4][INV_LS][2] uint32 |
+// cst [3 (bw - 1 + REC_PAD)] double
+constexpr int INV_LDS_D = (2 * INV_TD * INV_DS + 4 * INV_LS * 2) / 2;   // doubles before cst
 static size_t inv_lds(int bw)
 {
-    return sizeof(float2) * 2 * INV_TD * INV_DS + sizeof(double) * 3 * (bw - 1 + REC_PAD);
+    return sizeof(double) * (INV_LDS_D + 3 * (bw - 1 + REC_PAD));
 }
+typedef float (*InvHi)[INV_TD][INV_DS];
+typedef uint32_t (*InvLo)[INV_LS][2];
 
 // one inverse work item: order pair (p, q), transforms t0 .. t0+TB-1 (< T), betas
 // 64 chunk .. 64 chunk + 63.  MFMA columns j = variant j & 7 of transform t0 + (j >> 3),
 // real and imaginary parts in separate MFMAs; degrees go in tiles of 8, K-step s of a
 // tile taking degree l0 + 2 k + s in lane row k, so that a lane's coefficients of both
-// K-steps are adjacent in memory.  rc / rc_off: recurrence constants; cst / Dl: the
+// K-steps are adjacent in memory.  rc / rc_off: recurrence constants; cst / Dh / Dlo: the
 // block's LDS.
 template <int TB>
 __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, int64_t c_ts,
@@ -88,7 +111,7 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
                                              const double *__restrict__ cosb, const double *__restrict__ seeds,
                                              const double *__restrict__ rc, const int64_t *__restrict__ rc_off,
                                              const int64_t *__restrict__ slice, int bw, int N, int p, int q, int t0,
-                                             int chunk, double *cst, float2 (*Dl)[INV_TD][INV_DS])
+                                             int chunk, double *cst, InvHi Dh, InvLo Dlo)
 {
     static_assert(TB == 1 || TB == 2, "one transform pair per wave");
     const int lane = threadIdx.x;
@@ -134,63 +157,96 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
         x0 = src[min(lk, L - 1)];
         x1 = src[min(lk + 1, L - 1)];
     };
-    // d for degrees l0 .. l0 + 7 of this lane's beta -> tile buf (then advanced past
-    // them); zero past the last degree
-    auto dtile = [&](int buf, int l0) {
+    // d for degrees l0 .. l0 + 7 of this lane's beta (then advanced past them; zero past
+    // the last degree): hi parts -> tile buf, lo parts as bfloat16 -> the lo table, slots
+    // 2 h, 2 h + 1 of row k for degrees l0 + 2 k, l0 + 2 k + 1, h = the tile's place in
+    // its flush block
+    auto dtile = [&](auto hc, int buf, int l0) {
+        constexpr int h = decltype(hc)::value;
 #pragma unroll
-        for (int k = 0; k < INV_TD; ++k) {
-            float hi, lo;
-            split(dcur, hi, lo);
-            Dl[buf][k][lane] = make_float2(hi, lo);
-            rec_step(cst, l0 + k, cb, dcur, dprev);
+        for (int k = 0; k < INV_TD / 2; ++k) {
+            float hi0, lo0, hi1, lo1;
+            split(dcur, hi0, lo0);
+            Dh[buf][2 * k][lane] = hi0;
+            rec_step(cst, l0 + 2 * k, cb, dcur, dprev);
+            split(dcur, hi1, lo1);
+            Dh[buf][2 * k + 1][lane] = hi1;
+            rec_step(cst, l0 + 2 * k + 1, cb, dcur, dprev);
+            const bf2_t pk = {(__bf16)lo0, (__bf16)lo1};
+            Dlo[k][lane][h] = __builtin_bit_cast(uint32_t, pk);
         }
     };
-    // one 8-degree tile: 2 K-steps x 4 beta tiles x (re, im) x (hi, lo) MFMAs with the
-    // ring slot (ra, rb) (loaded two tiles earlier, reloaded here for tile l0 + 16),
-    // interleaved with the recurrence that fills the other LDS tile for the next one
-    auto tile = [&](int l0, float2 &ra, float2 &rb) {
+    // lo products of a 16-degree flush block: bfloat16 slot 2 h + s holds K-step s of
+    // the block's tile h (degree l0 + 8 h + 2 k + s in lane row k), for A and B alike
+    bf4_t blr, bli;
+    // one 8-degree tile (h = 0 / 1: first / second of its flush block): 2 K-steps x 4
+    // beta tiles x (re, im) FP32 MFMAs of the hi parts with the ring slot (ra, rb)
+    // (loaded two tiles earlier, reloaded here for tile l0 + 16); the second tile adds
+    // the block's lo products, 8 bfloat16 MFMAs; interleaved with the recurrence that
+    // fills the other LDS tile for the next one
+    auto tile = [&](auto hc, int l0, float2 &ra, float2 &rb) {
+        constexpr int h = decltype(hc)::value;
         const int buf = (l0 >> 3) & 1;
-        float2 a[2][4];
+        float a[2][4];
 #pragma unroll
         for (int s2 = 0; s2 < 2; ++s2)
 #pragma unroll
-            for (int g = 0; g < 4; ++g) a[s2][g] = Dl[buf][2 * kk + s2][16 * g + j];
+            for (int g = 0; g < 4; ++g) a[s2][g] = Dh[buf][2 * kk + s2][16 * g + j];
+        // the block's lo parts, read before this tile's dtile rewrites slots 0, 1 (the
+        // LDS operations of a wave execute in order)
+        bf4_t alo[4];
+        if (h == 1) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                alo[g] = __builtin_bit_cast(bf4_t, *reinterpret_cast<const uint2 *>(&Dlo[kk][16 * g + j][0]));
+        }
         const int lk = l0 + 2 * kk;
         const float z0 = lk < L ? sg0 : 0.0f, z1 = lk + 1 < L ? sg1 : 0.0f;
         const float br0 = z0 * ra.x, bi0 = z0 * ra.y, br1 = z1 * rb.x, bi1 = z1 * rb.y;
         loadb(l0 + 16, ra, rb);
         __builtin_amdgcn_sched_barrier(0);   // the reload is issued here, two tiles ahead
-        // per K-step the hi products, then the lo products: eight independent
-        // accumulators between dependent MFMAs
+        blr[2 * h] = (__bf16)br0; blr[2 * h + 1] = (__bf16)br1;
+        bli[2 * h] = (__bf16)bi0; bli[2 * h + 1] = (__bf16)bi1;
+        if (h == 1) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                ar[g] = mma_bf16(alo[g], blr, ar[g]);
+                ai[g] = mma_bf16(alo[g], bli, ai[g]);
+            }
+        }
+        // per K-step the hi products: eight independent accumulators between dependent
+        // MFMAs
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-            ar[g] = mma32(a[0][g].x, br0, ar[g]);
-            ai[g] = mma32(a[0][g].x, bi0, ai[g]);
+            ar[g] = mma32(a[0][g], br0, ar[g]);
+            ai[g] = mma32(a[0][g], bi0, ai[g]);
         }
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-            ar[g] = mma32(a[0][g].y, br0, ar[g]);
-            ai[g] = mma32(a[0][g].y, bi0, ai[g]);
+            ar[g] = mma32(a[1][g], br1, ar[g]);
+            ai[g] = mma32(a[1][g], bi1, ai[g]);
         }
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            ar[g] = mma32(a[1][g].x, br1, ar[g]);
-            ai[g] = mma32(a[1][g].x, bi1, ai[g]);
-        }
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            ar[g] = mma32(a[1][g].y, br1, ar[g]);
-            ai[g] = mma32(a[1][g].y, bi1, ai[g]);
-        }
-        dtile(buf ^ 1, l0 + INV_TD);
+        dtile(std::integral_constant<int, 1 - h>{}, buf ^ 1, l0 + INV_TD);
         // issue order: each MFMA followed by a few VALU instructions of the recurrence,
         // which run while the matrix core works
 #pragma unroll
-        for (int i = 0; i < 32; ++i) {
+        for (int i = 0; i < (h ? 24 : 16); ++i) {
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, h ? 3 : 4, 0);
         }
     };
+    // the lo products of a flush block without a second tile (its B slots 2, 3 zero)
+    auto lo_only = [&]() {
+        blr[2] = blr[3] = bli[2] = bli[3] = (__bf16)0.0f;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const bf4_t alo = __builtin_bit_cast(bf4_t, *reinterpret_cast<const uint2 *>(&Dlo[kk][16 * g + j][0]));
+            ar[g] = mma_bf16(alo, blr, ar[g]);
+            ai[g] = mma_bf16(alo, bli, ai[g]);
+        }
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
     // FP32 sums of (at most) 16 degrees into the FP64 totals
     auto flush = [&]() {
 #pragma unroll
@@ -207,19 +263,20 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
     float2 r0a, r0b, r1a, r1b;
     loadb(0, r0a, r0b);
     loadb(8, r1a, r1b);
-    dtile(0, 0);
+    dtile(std::integral_constant<int, 0>{}, 0, 0);
     // LDS operations of one wave execute in order, so a tile written during one step is
     // read by the next with no barrier; two ring slots, one per tile of a 16-degree flush.
     // No exit between a reload and the tiles after it: a load whose value one exit path
     // does not need would be moved past the exit test, next to its use.
     int l0 = 0;
     for (; l0 + 16 < L; l0 += 16) {
-        tile(l0, r0a, r0b);
-        tile(l0 + 8, r1a, r1b);
+        tile(I0{}, l0, r0a, r0b);
+        tile(I1{}, l0 + 8, r1a, r1b);
         flush();
     }
-    tile(l0, r0a, r0b);
-    if (l0 + 8 < L) tile(l0 + 8, r1a, r1b);
+    tile(I0{}, l0, r0a, r0b);
+    if (l0 + 8 < L) tile(I1{}, l0 + 8, r1a, r1b);
+    else lo_only();
     flush();
 
     // item r of lane l: beta row 4(l>>4) + r of tile g, column j = l & 15: betas
@@ -260,16 +317,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dw
     const int64_t *__restrict__ rc_off, const int64_t *__restrict__ slice, const int32_t *__restrict__ pairs, int bw,
     int N, int npair, int nch, int nunit)
 {
-    extern __shared__ double smx[];   // [tile parity][degree in tile][beta] = (hi, lo) | constants
-    float2 (*Dl)[INV_TD][INV_DS] = reinterpret_cast<float2 (*)[INV_TD][INV_DS]>(smx);
-    double *cst = smx + 2 * INV_TD * INV_DS;
+    extern __shared__ double smx[];   // hi tiles | lo table | recurrence constants
+    const InvHi Dh = reinterpret_cast<InvHi>(smx);
+    const InvLo Dlo = reinterpret_cast<InvLo>(smx + INV_TD * INV_DS);
+    double *cst = smx + INV_LDS_D;
     const int id = blockIdx.x;
     const int chunk = (id >> 3) % nch;
     const int unit = ((id >> 3) / nch) * 8 + (id & 7);  // chunks of a unit share an XCD
     if (unit >= nunit) return;
     const int pq = pairs[unit % npair];
     inv_mix_item<TB>(coeff, c_ts, F, f_ts, T, cosb, seeds, rc, rc_off, slice, bw, N, pq & 0xffff, pq >> 16,
-                     (unit / npair) * TB, chunk, cst, Dl);
+                     (unit / npair) * TB, chunk, cst, Dh, Dlo);
 }
 
 // grouped launch over transforms of several bandwidths: a persistent grid walks the
@@ -280,15 +338,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dw
     const int4 *__restrict__ items, int nitem)
 {
     extern __shared__ double smx[];   // sized for the group's largest bandwidth
-    float2 (*Dl)[INV_TD][INV_DS] = reinterpret_cast<float2 (*)[INV_TD][INV_DS]>(smx);
-    double *cst = smx + 2 * INV_TD * INV_DS;
+    const InvHi Dh = reinterpret_cast<InvHi>(smx);
+    const InvLo Dlo = reinterpret_cast<InvLo>(smx + INV_TD * INV_DS);
+    double *cst = smx + INV_LDS_D;
     for (int it = blockIdx.x; it < nitem; it += gridDim.x) {
         const int4 w = items[it];
         if (w.x < 0) continue;
         const MixSegment sg = segs[w.x];
         __syncthreads();  // the previous item's LDS reads are done
         inv_mix_item<TB>(coeff + sg.coff, sg.nc, F + sg.goff, (int64_t)sg.N * sg.N * sg.N, sg.T, sg.cosb, sg.seeds,
-                         sg.rc, sg.rc_off, sg.slice, sg.bw, sg.N, w.y & 0xffff, w.y >> 16, w.z, w.w, cst, Dl);
+                         sg.rc, sg.rc_off, sg.slice, sg.bw, sg.N, w.y & 0xffff, w.y >> 16, w.z, w.w, cst, Dh,
+                         Dlo);
     }
 }
 
@@ -392,6 +452,9 @@ __device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64
             for (int g = 0; g < TG; ++g)
 #pragma unroll
                 for (int u = 0; u < KS; ++u) ar[g][u] = ai[g][u] = f4_t{0.f, 0.f, 0.f, 0.f};
+            // lo products of K-steps 4 m .. 4 m + 3 (slot s4 & 3, for A and B alike) in one
+            // bfloat16 MFMA per (transform pair, part)
+            bf4_t alo, blr[TG], bli[TG];
 #pragma unroll
             for (int s4 = 0; s4 < 16; ++s4) {
                 const float2 a = Dt[j][8 * (s4 >> 1) + 2 * kk + (s4 & 1)];
@@ -409,10 +472,18 @@ __device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64
                     ar[g][u] = mma32(a.x, bre[g], ar[g][u]);
                     ai[g][u] = mma32(a.x, bim[g], ai[g][u]);
                 }
+                alo[s4 & 3] = (__bf16)a.y;
 #pragma unroll
                 for (int g = 0; g < TG; ++g) {
-                    ar[g][u] = mma32(a.y, bre[g], ar[g][u]);
-                    ai[g][u] = mma32(a.y, bim[g], ai[g][u]);
+                    blr[g][s4 & 3] = (__bf16)bre[g];
+                    bli[g][s4 & 3] = (__bf16)bim[g];
+                }
+                if ((s4 & 3) == 3) {
+#pragma unroll
+                    for (int g = 0; g < TG; ++g) {
+                        ar[g][1] = mma_bf16(alo, blr[g], ar[g][1]);
+                        ai[g][1] = mma_bf16(alo, bli[g], ai[g][1]);
+                    }
                 }
                 // K-steps 2h, 2h+1 consumed: reload them for chunk cn (chunk 0 of the next
                 // tile after the last chunk; once more, unused, after the last tile), so

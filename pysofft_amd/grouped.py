@@ -1,8 +1,8 @@
 """Mixed-precision SO(3) transforms of a batch of uneven bandwidths.
 
 ``SoftGroup(bws)`` fixes one bandwidth per transform.  ``isoft`` / ``soft`` run the
-whole batch with complex64 storage (FP64 recurrence and FFT arithmetic, FP32 MFMA
-contraction with d split into hi + lo parts) as grouped DWT launches over work lists and
+whole batch with complex64 storage (FP64 recurrence and FFT arithmetic, MFMA
+contraction with d split into FP32 hi + bfloat16 lo parts) as grouped DWT launches over work lists and
 per-bandwidth FFT passes on parallel branches of a HIP graph captured on first use; the
 launch shapes and the schedule (one grouped DWT launch, or one branch per bandwidth so that
 FFT passes overlap other bandwidths' DWTs) are chosen by timing (so3ft_group_* in
