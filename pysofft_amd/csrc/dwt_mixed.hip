@@ -48,6 +48,15 @@ __device__ __forceinline__ void rec_step(P rcp, int i, double cb, double &dcur, 
     dcur = dn;
 }
 
+// the same step with entry i given as (a, -a x, b)
+__device__ __forceinline__ void rec_step_ax(const double *c, int i, double cb, double &dcur, double &dprev)
+{
+    const double t = fma(c[3 * i], cb, c[3 * i + 1]);
+    const double dn = fma(t, dcur, -c[3 * i + 2] * dprev);
+    dprev = dcur;
+    dcur = dn;
+}
+
 // plan tables live in global memory; the grouped kernels read their pointers from the
 // segment table, so the address space is restated for the compiler (no flat accesses)
 template <class T> using gptr = const __attribute__((address_space(1))) T *;
@@ -70,6 +79,20 @@ __device__ __forceinline__ f4_t mma32(float a, float b, f4_t c)
 typedef __bf16 bf4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf2_t __attribute__((ext_vector_type(2)));
 typedef short s4_t __attribute__((ext_vector_type(4)));
+// two floats as bfloat16 (rounded toward zero) in one word, first in the low half.  Bit
+// operations on purpose: a conversion of a float that was itself rounded from a double is
+// folded by the compiler into a correctly rounded double -> bfloat16 conversion, about ten
+// instructions per value; the lo parts need no such rounding.
+__device__ __forceinline__ uint32_t bf16x2_trunc(float a, float b)
+{
+    // v_perm_b32: result byte i = byte sel_i of (b : a) (selectors 0-3 pick from a);
+    // = (a >> 16) | (b & 0xffff0000)
+    return __builtin_amdgcn_perm(__builtin_bit_cast(uint32_t, b), __builtin_bit_cast(uint32_t, a), 0x07060302u);
+}
+__device__ __forceinline__ bf4_t bf4_of(uint32_t lo, uint32_t hi)
+{
+    return __builtin_bit_cast(bf4_t, make_uint2(lo, hi));
+}
 __device__ __forceinline__ f4_t mma_bf16(bf4_t a, bf4_t b, f4_t c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4_t, a), __builtin_bit_cast(s4_t, b), c, 0,
@@ -134,7 +157,13 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
     const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
     // the pair's recurrence constants, staged in LDS (a read there has a short latency)
     const const_dptr rcp = (const_dptr)rc + 3 * ((const_i64ptr)rc_off)[tri];
-    for (int i = lane; i < 3 * (L - 1 + REC_PAD); i += 64) cst[i] = rcp[i];
+    // as (a, -a x, b), so that a step's factor a (cos(beta) - x) is one fma
+    for (int i = lane; i < L - 1 + REC_PAD; i += 64) {
+        const double ra = rcp[3 * i];
+        cst[3 * i] = ra;
+        cst[3 * i + 1] = -ra * rcp[3 * i + 1];
+        cst[3 * i + 2] = rcp[3 * i + 2];
+    }
     double dcur = (beta < N) ? ((gptr<double>)seeds)[tri * N + beta] : 0.0;
     const double cb = (beta < N) ? ((gptr<double>)cosb)[beta] : 0.0;
     double dprev = 0.0;
@@ -168,17 +197,16 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
             float hi0, lo0, hi1, lo1;
             split(dcur, hi0, lo0);
             Dh[buf][2 * k][lane] = hi0;
-            rec_step(cst, l0 + 2 * k, cb, dcur, dprev);
+            rec_step_ax(cst, l0 + 2 * k, cb, dcur, dprev);
             split(dcur, hi1, lo1);
             Dh[buf][2 * k + 1][lane] = hi1;
-            rec_step(cst, l0 + 2 * k + 1, cb, dcur, dprev);
-            const bf2_t pk = {(__bf16)lo0, (__bf16)lo1};
-            Dlo[k][lane][h] = __builtin_bit_cast(uint32_t, pk);
+            rec_step_ax(cst, l0 + 2 * k + 1, cb, dcur, dprev);
+            Dlo[k][lane][h] = bf16x2_trunc(lo0, lo1);
         }
     };
     // lo products of a 16-degree flush block: bfloat16 slot 2 h + s holds K-step s of
     // the block's tile h (degree l0 + 8 h + 2 k + s in lane row k), for A and B alike
-    bf4_t blr, bli;
+    uint32_t blr[2], bli[2];
     // one 8-degree tile (h = 0 / 1: first / second of its flush block): 2 K-steps x 4
     // beta tiles x (re, im) FP32 MFMAs of the hi parts with the ring slot (ra, rb)
     // (loaded two tiles earlier, reloaded here for tile l0 + 16); the second tile adds
@@ -205,13 +233,14 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
         const float br0 = z0 * ra.x, bi0 = z0 * ra.y, br1 = z1 * rb.x, bi1 = z1 * rb.y;
         loadb(l0 + 16, ra, rb);
         __builtin_amdgcn_sched_barrier(0);   // the reload is issued here, two tiles ahead
-        blr[2 * h] = (__bf16)br0; blr[2 * h + 1] = (__bf16)br1;
-        bli[2 * h] = (__bf16)bi0; bli[2 * h + 1] = (__bf16)bi1;
+        blr[h] = bf16x2_trunc(br0, br1);
+        bli[h] = bf16x2_trunc(bi0, bi1);
         if (h == 1) {
+            const bf4_t lr = bf4_of(blr[0], blr[1]), li = bf4_of(bli[0], bli[1]);
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                ar[g] = mma_bf16(alo[g], blr, ar[g]);
-                ai[g] = mma_bf16(alo[g], bli, ai[g]);
+                ar[g] = mma_bf16(alo[g], lr, ar[g]);
+                ai[g] = mma_bf16(alo[g], li, ai[g]);
             }
         }
         // per K-step the hi products: eight independent accumulators between dependent
@@ -237,12 +266,12 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
     };
     // the lo products of a flush block without a second tile (its B slots 2, 3 zero)
     auto lo_only = [&]() {
-        blr[2] = blr[3] = bli[2] = bli[3] = (__bf16)0.0f;
+        const bf4_t lr = bf4_of(blr[0], 0u), li = bf4_of(bli[0], 0u);
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const bf4_t alo = __builtin_bit_cast(bf4_t, *reinterpret_cast<const uint2 *>(&Dlo[kk][16 * g + j][0]));
-            ar[g] = mma_bf16(alo, blr, ar[g]);
-            ai[g] = mma_bf16(alo, bli, ai[g]);
+            ar[g] = mma_bf16(alo, lr, ar[g]);
+            ai[g] = mma_bf16(alo, li, ai[g]);
         }
     };
     using I0 = std::integral_constant<int, 0>;
@@ -453,8 +482,10 @@ __device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64
 #pragma unroll
                 for (int u = 0; u < KS; ++u) ar[g][u] = ai[g][u] = f4_t{0.f, 0.f, 0.f, 0.f};
             // lo products of K-steps 4 m .. 4 m + 3 (slot s4 & 3, for A and B alike) in one
-            // bfloat16 MFMA per (transform pair, part)
-            bf4_t alo, blr[TG], bli[TG];
+            // bfloat16 MFMA per (transform pair, part); slots packed in pairs (word
+            // (s4 >> 1) & 1) from the values of the even K-step (pa, pr, pi) and the odd one
+            uint32_t alo[2], blr[TG][2], bli[TG][2];
+            float pa, pr[TG], pi[TG];
 #pragma unroll
             for (int s4 = 0; s4 < 16; ++s4) {
                 const float2 a = Dt[j][8 * (s4 >> 1) + 2 * kk + (s4 & 1)];
@@ -472,17 +503,28 @@ __device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64
                     ar[g][u] = mma32(a.x, bre[g], ar[g][u]);
                     ai[g][u] = mma32(a.x, bim[g], ai[g][u]);
                 }
-                alo[s4 & 3] = (__bf16)a.y;
-#pragma unroll
-                for (int g = 0; g < TG; ++g) {
-                    blr[g][s4 & 3] = (__bf16)bre[g];
-                    bli[g][s4 & 3] = (__bf16)bim[g];
-                }
-                if ((s4 & 3) == 3) {
+                if (s4 & 1) {
+                    const int w2 = (s4 >> 1) & 1;
+                    alo[w2] = bf16x2_trunc(pa, a.y);
 #pragma unroll
                     for (int g = 0; g < TG; ++g) {
-                        ar[g][1] = mma_bf16(alo, blr[g], ar[g][1]);
-                        ai[g][1] = mma_bf16(alo, bli[g], ai[g][1]);
+                        blr[g][w2] = bf16x2_trunc(pr[g], bre[g]);
+                        bli[g][w2] = bf16x2_trunc(pi[g], bim[g]);
+                    }
+                } else {
+                    pa = a.y;
+#pragma unroll
+                    for (int g = 0; g < TG; ++g) {
+                        pr[g] = bre[g];
+                        pi[g] = bim[g];
+                    }
+                }
+                if ((s4 & 3) == 3) {
+                    const bf4_t la = bf4_of(alo[0], alo[1]);
+#pragma unroll
+                    for (int g = 0; g < TG; ++g) {
+                        ar[g][1] = mma_bf16(la, bf4_of(blr[g][0], blr[g][1]), ar[g][1]);
+                        ai[g][1] = mma_bf16(la, bf4_of(bli[g][0], bli[g][1]), ai[g][1]);
                     }
                 }
                 // K-steps 2h, 2h+1 consumed: reload them for chunk cn (chunk 0 of the next
