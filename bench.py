@@ -287,6 +287,12 @@ def main():
             else:
                 k["gbs"] = 2.0 * esz * N ** 3 * B / (t * 1e-3) / 1e9   # one read + one write of the grid
             kernels[name] = k
+        # bandwidth reference: a plain streaming copy of the same complex64 grids (f32 -> the
+        # first half of the complex128 grid buffer), one 16-byte vector per thread
+        _lib.check(L.so3ft_bench_kernel(s._handle, 13, 5, C.c_void_p(f32.data_ptr()),
+                                        C.c_void_p(f64.data_ptr()), B, sp, C.byref(out)))
+        copy_gbs = 2.0 * 8 * N ** 3 * B / (out.value * 1e-3) / 1e9
+        kernels["copy_c64_grid"] = {"ms_per_launch": out.value, "gbs": copy_gbs}
         # share of the headline step: each FFT pass runs twice per isoft+soft
         per_step = {"dwt_inverse_mixed": kernels["dwt_inverse_mixed"]["ms_per_launch"],
                     "dwt_forward_mixed": kernels["dwt_forward_mixed"]["ms_per_launch"],
@@ -301,7 +307,7 @@ def main():
         else:
             ach = 2.0 * 8 * N ** 3 * B / (t * 1e-3) / 1e9
             roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": ach / PEAK_HBM_GBS}
+                    "frac": ach / PEAK_HBM_GBS, "copy_gbs": copy_gbs, "frac_of_copy": ach / copy_gbs}
         roof["traffic"] = traffic_from_profiles(dom)
         roof["share_of_step"] = per_step[dom] / sum(per_step.values())
     del f64, r64

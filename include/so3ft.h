@@ -237,7 +237,8 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
  * 3 FFT pass along gamma (a -> b), 4 / 5 the default engine's FP64 inverse (VALU) /
  * forward (FP64 MFMA, lane = beta), 6 / 7 the mixed-precision (complex64) inverse /
  * forward DWT, 8 / 9 the complex64 FFT passes along alpha / gamma, 12 the FP64 forward
- * on the VALU.  T transforms per launch. */
+ * on the VALU, 13 a plain streaming copy of T complex64 grids a -> b (the bandwidth
+ * reference of the FFT passes).  T transforms per launch. */
 int so3ft_bench_kernel(so3ft_handle h, int which, int reps, void *a, void *b, int64_t T, void *stream,
                        float *ms_per_launch);
 

@@ -222,6 +222,27 @@ __global__ __launch_bounds__(256) void fft_reg_kernel(const void *__restrict__ i
     }
 }
 
+// streaming copy, one 16-byte vector per thread with non-temporal accesses: the bandwidth
+// reference for the FFT passes (the fastest of the copy shapes measured on MI355X; more
+// vectors per thread ran slower)
+__global__ __launch_bounds__(256) void copy16_kernel(const float4 *__restrict__ a, float4 *__restrict__ b, int64_t n)
+{
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(reinterpret_cast<const f4v *>(a) + i),
+                                    reinterpret_cast<f4v *>(b) + i);
+}
+
+hipError_t launch_copy16(const void *a, void *b, int64_t nvec, hipStream_t s)
+{
+    if (nvec <= 0) return hipSuccess;
+    if ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)((nvec + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<const float4 *>(a), reinterpret_cast<float4 *>(b), nvec);
+    return hipGetLastError();
+}
+
 static bool split_pow2(int N, int &R1, int &R2)
 {
     switch (N) {

@@ -1063,11 +1063,13 @@ int so3ft_bench_kernel(so3ft_handle h, int which, int reps, void *a, void *b, in
             else { ax.in_o = N; ax.in_r = N2; ax.out_o = N; ax.out_r = N2; }
             return launch_fft_reg((int)N, P->tb.tw, (int)P->bw, (int)P->lmax, a, b, ax, (int)T, 1, 1, 0, (int)N, s);
         }
+        case 13:  // copy of the T complex64 grids a -> b: the streaming bandwidth reference
+            return launch_copy16(a, b, T * N3 / 2, s);
         default:
             return hipErrorInvalidValue;
         }
     };
-    if (which < 0 || (which > 9 && which != 12)) return fail(SO3FT_EINVAL, "unknown kernel id");
+    if (which < 0 || (which > 9 && which != 12 && which != 13)) return fail(SO3FT_EINVAL, "unknown kernel id");
     if (which >= 8 && !fft_reg_supported((int)N)) return fail(SO3FT_EUNSUPPORTED, "no complex64 FFT for this bw");
     HIPCHK(launch());  // warm-up
     HIPCHK(hipEventRecord(e0, s));
