@@ -35,6 +35,25 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b)
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// W_R^k = exp(sign 2 pi i k / R): compile-time constants when R divides 16 (the DFTs'
+// inner twiddles; fewer registers than table loads), else the sign-adjusted table
+__device__ __forceinline__ double c16(int j)   // cos(2 pi j / 16), 0 <= j <= 4
+{
+    return j == 0 ? 1.0 : j == 1 ? 0.9238795325112867 : j == 2 ? 0.7071067811865476 : j == 3 ? 0.3826834323650898 : 0.0;
+}
+template <int R>
+__device__ __forceinline__ double2 twc(int k, const double2 *tw, int NR, int sign)
+{
+    if constexpr (16 % R == 0) {
+        const int j = k * (16 / R);   // W_16^j, 0 < j < 8
+        const double c = j <= 4 ? c16(j) : -c16(8 - j);
+        const double sn = j <= 4 ? c16(4 - j) : c16(j - 4);
+        return make_double2(c, sign < 0 ? -sn : sn);
+    } else {
+        return tw[k * NR];
+    }
+}
+
 // In-register DFT of length R = 2^a or 3 * 2^a: recursive radix-2 decimation in time
 // down to a radix-3 (or trivial) base; W_R^j = tw[j * (N/R)] with tw the
 // sign-adjusted length-N table in LDS.
@@ -62,7 +81,7 @@ struct Dft {
                 double2 t;
                 if (k == 0) t = o[0];
                 else if (4 * k == R) t = (sign < 0) ? make_double2(o[k].y, -o[k].x) : make_double2(-o[k].y, o[k].x);
-                else t = cmul2(o[k], tw[k * NR]);
+                else t = cmul2(o[k], twc<R>(k, tw, NR, sign));
                 v[k] = make_double2(e[k].x + t.x, e[k].y + t.y);
                 v[k + R / 2] = make_double2(e[k].x - t.x, e[k].y - t.y);
             }
@@ -243,6 +262,88 @@ hipError_t launch_copy16(const void *a, void *b, int64_t nvec, hipStream_t s)
     return hipGetLastError();
 }
 
+// One column tile per block, no prefetch, the step exchange through LDS in two FP64 halves
+// (real, then imaginary parts): 36 KB of LDS and at most 168 registers, so three blocks
+// share a CU (the kernel above holds two).  Measured faster for complex128 passes (11% at
+// N = 256, 31% at N = 128) and complex64 passes up to N = 128; the prefetching kernel stays
+// ahead for complex64 at N = 256.
+template <int R1, int R2, class TI, class TO>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void fft_reg_lean(
+    const void *__restrict__ in, void *__restrict__ out, const double2 *__restrict__ twg, FftRegK k)
+{
+    constexpr int N = R1 * R2;
+    constexpr int RM = R1 > R2 ? R1 : R2;
+    extern __shared__ double2 sm[];
+    double2 *tw = sm;
+    double *Yd = reinterpret_cast<double *>(sm + N);   // [k1][n2][c]
+    const int C = k.C, tid = threadIdx.x;
+    const int c = tid % C, grp = tid / C;
+    const int64_t o = blockIdx.y, t = blockIdx.z;
+    if (k.mask_outer && !order_ok2((int)o, N, k.bw, k.lmax)) return;
+    const int jt = blockIdx.x;
+    const int cb = k.col0 + jt * C;
+    const bool colok = (cb + c) < (k.col0 + k.ncols);
+    for (int i = tid; i < N; i += blockDim.x) {
+        double2 w = twg[i];
+        if (k.sign > 0) w.y = -w.y;
+        tw[i] = w;
+    }
+    double2 v[RM];
+    {
+        const int64_t ib = t * k.in_t + o * k.in_o + cb + c + (int64_t)grp * k.in_r;
+        const TI *pb = reinterpret_cast<const TI *>(in) + ib;
+        const uint32_t rstep = (uint32_t)(R2 * k.in_r);
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) {
+            const int r = R2 * n1 + grp;
+            TI x = zero_of<TI>();
+            if (colok && grp < R2 && (!k.mask_rows || order_ok2(r, N, k.bw, k.lmax))) x = ld_nt(pb + (uint32_t)n1 * rstep);
+            v[n1] = widen(x);
+        }
+    }
+    __syncthreads();   // twiddles
+    if (grp < R2) {
+        const int n2 = grp;
+        Dft<R1>::run(v, tw, N / R1, k.sign);
+#pragma unroll
+        for (int k1 = 1; k1 < R1; ++k1)
+            if (n2 != 0) v[k1] = cmul2(v[k1], tw[n2 * k1]);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) Yd[(k1 * R2 + n2) * C + c] = v[k1].x;
+    }
+    __syncthreads();
+    double re[R2];
+    if (grp < R1) {
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) re[n2] = Yd[(grp * R2 + n2) * C + c];
+    }
+    __syncthreads();
+    if (grp < R2) {
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) Yd[(k1 * R2 + grp) * C + c] = v[k1].y;
+    }
+    __syncthreads();
+    if (grp < R1) {
+        const int k1 = grp;
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) v[n2] = make_double2(re[n2], Yd[(k1 * R2 + n2) * C + c]);
+        Dft<R2>::run(v, tw, N / R2, k.sign);
+        if (colok) {
+            const int64_t ob = t * k.out_t + o * k.out_o + cb + c + (int64_t)k1 * k.out_r;
+            const uint32_t ostep = (uint32_t)(R1 * k.out_r);
+            TO *po = reinterpret_cast<TO *>(out) + ob;
+            typename Scalar<TO>::type *pr = reinterpret_cast<typename Scalar<TO>::type *>(out) + ob;
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) {
+                const uint32_t off = (uint32_t)k2 * ostep;
+                const double2 x = make_double2(v[k2].x * k.scale, v[k2].y * k.scale);
+                if (k.out_real) pr[off] = (typename Scalar<TO>::type)x.x;
+                else st<TO>(po, off, x);
+            }
+        }
+    }
+}
+
 static bool split_pow2(int N, int &R1, int &R2)
 {
     switch (N) {
@@ -271,6 +372,14 @@ bool fft_reg_supported(int N)
     return split_pow2(N, a, b);
 }
 
+// the one-tile kernel exists for N <= 256 only (larger lengths would spill)
+template <int R1, int R2, class TI, class TO>
+static void launch_lean(dim3 grid, dim3 block, int bytes, hipStream_t s, const void *in, void *out,
+                        const double2 *tw, const FftRegK &k)
+{
+    if constexpr (R1 * R2 <= 256) hipLaunchKernelGGL((fft_reg_lean<R1, R2, TI, TO>), grid, block, bytes, s, in, out, tw, k);
+}
+
 template <class TI, class TO>
 static hipError_t launch_reg_typed(int N, const double2 *tw, int bw, int lmax, const void *in, void *out,
                                    const FftAxis &ax, int T, int col0, int ncols, hipStream_t s)
@@ -297,6 +406,10 @@ static hipError_t launch_reg_typed(int N, const double2 *tw, int bw, int lmax, c
     const int gfill = (int)std::min<int64_t>(ntile, (4096 + lines - 1) / lines);
     const int gx = (sizeof(TI) == 8) ? std::max((ntile + 15) / 16, gfill) : ntile;
     dim3 grid(gx, N, T), block(threads);
+    // one tile per block with the exchange in halves: complex128 input up to N = 256,
+    // complex64 input up to N = 128 (not for real input, read as scalars by the kernel above)
+    const bool lean = !ax.in_real && (sizeof(TI) == 16 ? N <= 256 : N <= 128);
+    const int lean_bytes = (int)(sizeof(double2) * N + sizeof(double) * N * k.C);
 #define SO3_REG(A, B)                                                                                   \
     do {                                                                                                \
         static bool attr = false;                                                                       \
@@ -306,7 +419,10 @@ static hipError_t launch_reg_typed(int N, const double2 *tw, int bw, int lmax, c
             if (e != hipSuccess) return e;                                                              \
             attr = true;                                                                                \
         }                                                                                               \
-        hipLaunchKernelGGL((fft_reg_kernel<A, B, TI, TO>), grid, block, bytes, s, in, out, tw, k);     \
+        if (lean)                                                                                       \
+            launch_lean<A, B, TI, TO>(dim3(ntile, N, T), block, lean_bytes, s, in, out, tw, k);         \
+        else                                                                                            \
+            hipLaunchKernelGGL((fft_reg_kernel<A, B, TI, TO>), grid, block, bytes, s, in, out, tw, k); \
     } while (0)
     switch (N) {
     case 4: SO3_REG(2, 2); break;
