@@ -87,8 +87,8 @@ def learned_alignment(args, dev, full_rate):
     return {"workload": f"{n} pairs at bw={args.align_bw} (inference batch {n}), SNR 10; MLP 452-512-512-512-6 "
                         f"(BF16 MFMA, FP32 accumulation) trained on {len(hist) * 4096} generated pairs in "
                         f"{t_train:.1f} s; refinement windows +-2 steps at grid bandwidths "
-                        f"{[lv[0] for lv in al.levels]}; full correlation fallback below normalised peak "
-                        f"{al.threshold}",
+                        f"{[lv[0] for lv in al.levels]}; below normalised peak {al.threshold} an exhaustive "
+                        f"fallback (full correlation of the degrees below bw/2, then windows at bw and 2 bw)",
             "alignments_per_s": n / dt, "speedup_vs_full_correlation": (n / dt) / full_rate,
             "fallback_fraction": float(fell.float().mean().item()),
             "proposal_error_deg_median": float(np.degrees(np.median(perr))),

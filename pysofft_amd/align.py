@@ -11,7 +11,9 @@ with g ~ D(R) f:
      grid bandwidth up to 64) around the estimate
      (``so3ft_refine_correlation_peaks``);
   4. fallback: pairs whose refined peak, normalised by ||f|| ||g||, stays below a
-     threshold are aligned by the full correlation and peak search.
+     threshold are aligned by an exhaustive search: the full correlation and peak search of
+     their degrees below bw/2 (a grid of (bw)^3 points, 8x cheaper than the full bandwidth),
+     then the refinement levels (bw, bw) and (2 bw, bw) around that peak.
 
 The network is trained on the GPU by the library's own forward/backward/Adam kernels on
 pairs generated with exact Wigner-D rotations of the l <= 8 part (``training_batch``).
@@ -274,15 +276,20 @@ class Aligner:
         norm = torch.sqrt((f.abs() ** 2).sum(1) * (g.abs() ** 2).sum(1))
         bad = score / norm < self.threshold
         if bool(bad.any()):
-            if bw not in self._soft:
-                self._soft[bw] = Soft(bw)
-            s = self._soft[bw]
+            fb = max(bw // 2, 2)
+            if fb not in self._soft:
+                self._soft[fb] = Soft(fb)
+            s = self._soft[fb]
             idx = torch.nonzero(bad).flatten()
-            pk, _ = s.correlation_peaks(f[idx], g[idx])
-            N = 2 * bw
-            a = pk[:, 0].to(torch.float64) * (math.pi / bw)
+            fi, gi = f[idx].contiguous(), g[idx].contiguous()
+            pk, _ = s.correlation_peaks(fi[:, :fb * fb].contiguous(), gi[:, :fb * fb].contiguous())
+            N = 2 * fb
+            a = pk[:, 0].to(torch.float64) * (math.pi / fb)
             b = (2 * pk[:, 1].to(torch.float64) + 1) * (math.pi / (2 * N))
-            c = pk[:, 2].to(torch.float64) * (math.pi / bw)
+            c = pk[:, 2].to(torch.float64) * (math.pi / fb)
+            # the coarse peak is within half a step (pi / (2 fb)) of the optimum: a window of
+            # +-2 steps at grid bandwidth bw covers it, then one at 2 bw
+            ef, _ = self.refine(fi, gi, torch.stack([a, b, c], 1), levels=((bw, bw), (2 * bw, bw)))
             e = e.clone()
-            e[idx] = torch.stack([a, b, c], 1)
+            e[idx] = ef
         return e, bad
