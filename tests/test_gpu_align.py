@@ -151,3 +151,21 @@ def test_align_end_to_end_within_one_grid_step():
     assert np.all(err <= math.pi / 64), np.degrees(err.max())
     # most pairs are resolved by proposal + refinement, without the full correlation
     assert fell_back.float().mean().item() < 0.5
+
+
+def test_align_fallback_search_within_one_grid_step():
+    """The fallback alone (threshold above every normalised peak, so that every pair takes
+    it): full correlation of the degrees below bw/2, then windows at bw and 2 bw.  Every
+    rotation within one bw=64 grid step of the truth; the refined estimate beats the
+    coarse grid's half step on most pairs."""
+    from pysofft_amd.align import Aligner, AlignmentMLP
+    from pysofft_amd import workloads as W
+    rng = np.random.default_rng(21)
+    f, g, e = W.rotated_pairs(24, 64, rng, snr=10.0)
+    al = Aligner(mlp=AlignmentMLP(seed=3), threshold=2.0)   # no network training needed
+    al.fit_normalisation(n=4096)
+    est, fell_back = al.align(torch.from_numpy(f).cuda(), torch.from_numpy(g).cuda())
+    assert bool(fell_back.all())
+    err = W.angular_distance(W.euler_to_matrix(est.cpu().numpy()), W.euler_to_matrix(e))
+    assert np.all(err <= math.pi / 64), np.degrees(err.max())
+    assert np.median(err) < math.pi / 128, np.degrees(np.median(err))

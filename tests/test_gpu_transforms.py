@@ -497,3 +497,21 @@ def test_rfft_irfft_vs_numpy(bw):
     back = s._irfft(X)
     assert rel_l2(back, np.fft.irfftn(X, s=(N, N), axes=(0, 1)) * N) < 1e-13
     assert rel_l2(s._irfft(got), f) < 1e-13
+
+
+def test_bench_hook_copy_reference():
+    """so3ft_bench_kernel id 13, the streaming copy that serves as the FFT passes' bandwidth
+    reference: it copies T complex64 grids exactly and reports a positive duration."""
+    import ctypes as C
+    from pysofft_amd import _lib
+    bw, T = 8, 3
+    N = 2 * bw
+    s = _soft(bw)
+    a = torch.randn(T, N, N, N, dtype=torch.complex64, device="cuda")
+    b = torch.zeros_like(a)
+    out = C.c_float()
+    _lib.check(_lib.lib().so3ft_bench_kernel(s._handle, 13, 2, C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()),
+                                             T, C.c_void_p(torch.cuda.current_stream().cuda_stream), C.byref(out)))
+    torch.cuda.synchronize()
+    assert out.value > 0
+    assert torch.equal(a, b)
