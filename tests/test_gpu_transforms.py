@@ -62,9 +62,11 @@ def test_isoft_soft_vs_oracle_medium(bw):
     assert rel_l2(np.asarray(s.soft(f)), c) < 1e-12
 
 
-def test_bw128_round_trip_and_samples():
-    bw = 128
-    rng = np.random.default_rng(128)
+@pytest.mark.parametrize("bw", [96, 128])
+def test_large_bw_round_trip_and_samples(bw):
+    """FP64 round trip at bw = 3 * 2^5 and 2^7 (FFT lengths 192 and 256) and grid samples of
+    the l <= 6 part against a direct sum."""
+    rng = np.random.default_rng(bw)
     s = _soft(bw)
     nc = s.n_coeffs
     c = torch.from_numpy(_rand_c(rng, nc)).cuda()
@@ -77,7 +79,8 @@ def test_bw128_round_trip_and_samples():
     fn = f.cpu().numpy()
     a = np.arange(2 * bw) * np.pi / bw
     b = O.beta_samples(2 * bw)
-    pts = [(0, 0, 0), (5, 17, 100), (255, 128, 3), (77, 200, 255)]
+    N = 2 * bw
+    pts = [(0, 0, 0), (5, 17, 100), (N - 1, bw, 3), (77, N - 56, N - 1)]
     from oracle import brute
     sel = [i for i in range(len(deg)) if deg[i][0] <= 6]
     cl = np.zeros(nc, np.complex128)
