@@ -351,6 +351,35 @@ def test_mixed_c64_vs_oracle(bw):
         assert rel_l2(c2[i], c[i]) <= 2e-7, (bw, i, rel_l2(c2[i], c[i]))
 
 
+@pytest.mark.parametrize("bw", [96, 128, 256])
+def test_mixed_c64_large_bw_vs_fp64(bw):
+    """complex64 transforms at bandwidths beyond the oracle's reach (FFT lengths 192, 256,
+    512) against the FP64 path, itself pinned to the oracle at smaller bandwidths and to
+    direct sums at bw = 96 and 128: relative L2 <= 1e-7 for isoft and soft."""
+    n = 2 if bw < 256 else 1
+    N = 2 * bw
+    s = _soft(bw)
+
+    def rel_l2(a, b):   # on the device: the grids hold up to 2^27 points
+        a = a.to(torch.complex128)
+        return ((a - b).abs().pow(2).sum() / b.abs().pow(2).sum()).sqrt().item()
+
+    g = torch.Generator(device="cuda").manual_seed(bw)
+    c32 = torch.view_as_complex(torch.randn(n, s.n_coeffs, 2, generator=g, device="cuda")).contiguous()
+    f32 = s.isoft_many_c64(c32)
+    f64 = s.isoft_many(c32.to(torch.complex128))
+    for i in range(n):
+        e = rel_l2(f32[i], f64[i])
+        assert e <= 1e-7, (bw, i, e)
+    del f64
+    r32 = s.soft_many_c64(f32)
+    r64 = s.soft_many(f32.to(torch.complex128))
+    for i in range(n):
+        e = rel_l2(r32[i], r64[i])
+        assert e <= 1e-7, (bw, i, e)
+    assert f32.shape == (n, N, N, N)
+
+
 def test_mixed_c64_rejects_unsupported_bw():
     from pysofft_amd import _lib
     s = _soft(5)
