@@ -265,8 +265,8 @@ hipError_t launch_copy16(const void *a, void *b, int64_t nvec, hipStream_t s)
 // One column tile per block, no prefetch, the step exchange through LDS in two FP64 halves
 // (real, then imaginary parts): 36 KB of LDS and at most 168 registers, so three blocks
 // share a CU (the kernel above holds two).  Measured faster for complex128 passes (11% at
-// N = 256, 31% at N = 128) and complex64 passes up to N = 128; the prefetching kernel stays
-// ahead for complex64 at N = 256.
+// N = 256, 31% at N = 128) and complex64 passes up to N = 192 (10% at N = 128, 2% at
+// N = 192); the prefetching kernel stays ahead for complex64 at N = 256.
 template <int R1, int R2, class TI, class TO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void fft_reg_lean(
     const void *__restrict__ in, void *__restrict__ out, const double2 *__restrict__ twg, FftRegK k)
@@ -408,8 +408,8 @@ static hipError_t launch_reg_typed(int N, const double2 *tw, int bw, int lmax, c
     const int gx = (sizeof(TI) == 8) ? std::max((ntile + 7) / 8, gfill) : ntile;
     dim3 grid(gx, N, T), block(threads);
     // one tile per block with the exchange in halves: complex128 input up to N = 256,
-    // complex64 input up to N = 128 (not for real input, read as scalars by the kernel above)
-    const bool lean = !ax.in_real && (sizeof(TI) == 16 ? N <= 256 : N <= 128);
+    // complex64 input up to N = 192 (not for real input, read as scalars by the kernel above)
+    const bool lean = !ax.in_real && (sizeof(TI) == 16 ? N <= 256 : N <= 192);
     const int lean_bytes = (int)(sizeof(double2) * N + sizeof(double) * N * k.C);
 #define SO3_REG(A, B)                                                                                   \
     do {                                                                                                \
