@@ -185,8 +185,9 @@ def main():
                     help="also run a 1 GiB device copy (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark measures one GPU: --gpus must be 1; sharding across GPUs is "
-                                   "described in the design notes and not run"}))
+        print(json.dumps({"error": "this benchmark measures one GPU: --gpus must be 1 (a batch of transforms "
+                                   "would shard by transform, one share per GPU with no communication; "
+                                   "that is not run here)"}))
         sys.exit(2)
 
     import torch
