@@ -420,11 +420,12 @@ __device__ __forceinline__ double2 sh_coeff(const double2 *__restrict__ x, int l
 // (pysofft/soft.f08:2242-2341; real inputs soft.f08:2806-2872, whose half-complex
 // branches are this formula with the negative orders completed).  Transform t = blockIdx.y
 // reads its sets at t * t_ts; nr shells r at stride r_ts are summed with weights w (w ==
-// nullptr: one shell of weight 1).  One block per (m, n).
+// nullptr: one shell of weight 1).  Degrees l > lcap are zero (the Risbo loops of
+// soft.f08:2564-2580, 3046-3074 stop at l = lmax).  One block per (m, n).
 __global__ void corr_coeffs_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g, int64_t t_ts,
                                    int layout, int nr, int64_t r_ts, const double *__restrict__ w,
                                    double2 *__restrict__ out, int64_t c_ts, const int64_t *__restrict__ slice,
-                                   int bw)
+                                   int bw, int lcap)
 {
     const int s = blockIdx.x;
     const int m = s / (2 * bw - 1) - (bw - 1), n = s % (2 * bw - 1) - (bw - 1);
@@ -434,7 +435,8 @@ __global__ void corr_coeffs_kernel(const double2 *__restrict__ f, const double2 
     const double sg = ((m + n) & 1) ? -1.0 : 1.0;
     for (int l = lmin + (int)threadIdx.x; l < bw; l += blockDim.x) {
         double re = 0.0, im = 0.0;
-        for (int r = 0; r < nr; ++r) {
+        const int nrl = l <= lcap ? nr : 0;
+        for (int r = 0; r < nrl; ++r) {
             const double2 a = sh_coeff(f + t * t_ts + r * r_ts, layout, l, m, bw);
             const double2 b = sh_coeff(g + t * t_ts + r * r_ts, layout, l, n, bw);
             const double wr = w ? w[r] : 1.0;
@@ -447,12 +449,12 @@ __global__ void corr_coeffs_kernel(const double2 *__restrict__ f, const double2 
 }
 
 hipError_t launch_corr_coeffs(const DeviceTables &tb, int bw, const double2 *f, const double2 *g, int64_t t_ts,
-                              int layout, int nr, int64_t r_ts, const double *w, double2 *out, int64_t c_ts, int T,
-                              hipStream_t s)
+                              int layout, int nr, int64_t r_ts, const double *w, double2 *out, int64_t c_ts,
+                              int lcap, int T, hipStream_t s)
 {
     const int nsl = (2 * bw - 1) * (2 * bw - 1);
     hipLaunchKernelGGL(corr_coeffs_kernel, dim3(nsl, T), dim3(64), 0, s, f, g, t_ts, layout, nr, r_ts, w, out, c_ts,
-                       tb.slice, bw);
+                       tb.slice, bw, lcap);
     return hipGetLastError();
 }
 

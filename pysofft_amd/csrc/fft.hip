@@ -300,22 +300,31 @@ hipError_t launch_enforce_real_sym(double2 *c, int64_t c_ts, const int64_t *slic
 }
 
 // lmn <-> mnl coefficient order: perm[i] = mnl position of lmn position i.  to_mnl:
-// out[perm[i]] = in[i]; else out[i] = in[perm[i]].  T sets at strides in_ts / out_ts.
+// out[perm[i]] = in[i] for i < nkeep and 0 beyond; else out[i] = in[perm[i]] for i < nkeep,
+// entries beyond left as they are.  The lmn order is degree-major, so nkeep = NC(L) keeps
+// exactly the degrees l < L.  T sets at strides in_ts / out_ts.
 template <class E>
 __global__ void permute_kernel(const E *__restrict__ in, E *__restrict__ out, const int32_t *__restrict__ perm,
-                               int64_t nc, int64_t in_ts, int64_t out_ts, int to_mnl)
+                               int64_t nc, int64_t nkeep, int64_t in_ts, int64_t out_ts, int to_mnl)
 {
     const int64_t t = blockIdx.y;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nc; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t lim = to_mnl ? nc : nkeep;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < lim; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t j = perm[i];
-        if (to_mnl) out[t * out_ts + j] = in[t * in_ts + i];
-        else out[t * out_ts + i] = in[t * in_ts + j];
+        if (to_mnl) {
+            E v = {};
+            if (i < nkeep) v = in[t * in_ts + i];
+            out[t * out_ts + j] = v;
+        } else {
+            out[t * out_ts + i] = in[t * in_ts + j];
+        }
     }
 }
 
-hipError_t launch_permute_lmn(const int32_t *perm, int64_t nc, const void *in, int64_t in_ts, void *out,
-                              int64_t out_ts, int esz, int to_mnl, int T, hipStream_t s)
+hipError_t launch_permute_lmn(const int32_t *perm, int64_t nc, int64_t nkeep, const void *in, int64_t in_ts,
+                              void *out, int64_t out_ts, int esz, int to_mnl, int T, hipStream_t s)
 {
+    if (nkeep < 0 || nkeep > nc) return hipErrorInvalidValue;
     int nb = (int)((nc + 255) / 256);
     if (nb > 2048) nb = 2048;
     for (int t0 = 0; t0 < T; t0 += 32768) {
@@ -323,11 +332,13 @@ hipError_t launch_permute_lmn(const int32_t *perm, int64_t nc, const void *in, i
         if (esz == 16)
             hipLaunchKernelGGL(permute_kernel<double2>, dim3(nb, nt), dim3(256), 0, s,
                                reinterpret_cast<const double2 *>(in) + (int64_t)t0 * in_ts,
-                               reinterpret_cast<double2 *>(out) + (int64_t)t0 * out_ts, perm, nc, in_ts, out_ts, to_mnl);
+                               reinterpret_cast<double2 *>(out) + (int64_t)t0 * out_ts, perm, nc, nkeep, in_ts,
+                               out_ts, to_mnl);
         else if (esz == 8)
             hipLaunchKernelGGL(permute_kernel<float2>, dim3(nb, nt), dim3(256), 0, s,
                                reinterpret_cast<const float2 *>(in) + (int64_t)t0 * in_ts,
-                               reinterpret_cast<float2 *>(out) + (int64_t)t0 * out_ts, perm, nc, in_ts, out_ts, to_mnl);
+                               reinterpret_cast<float2 *>(out) + (int64_t)t0 * out_ts, perm, nc, nkeep, in_ts,
+                               out_ts, to_mnl);
         else
             return hipErrorInvalidValue;
         hipError_t e = hipGetLastError();

@@ -54,6 +54,8 @@ struct Plan {
     double *radw = nullptr; // radial shell weights (3-D correlation)
     int64_t radw_cap = 0;
     int lmn = 0;                // coefficients in lmn order (Risbo layout)
+    int64_t lcap = 0;           // highest degree summed: lmax for lmn handles, else bw - 1
+    int64_t nkeep = 0;          // lmn positions of the degrees l <= lcap: NC(lcap + 1)
     int32_t *lmn_perm = nullptr;
     double2 *pstage = nullptr;  // coefficients reordered to mnl
     int64_t pstage_T = 0;
@@ -265,8 +267,15 @@ int build_plan(Plan *P, int64_t bw, int64_t lmax, int precompute, int init_ffts,
     // recurrence_type = Risbo without precomputed Wigners: coefficients in lmn order
     // (soft.py:98-106); the transforms reorder them and run the same d values (Kostelec
     // recurrence, equal to the Risbo values to rounding for bw <= 256)
+    // The Risbo transforms sum only the degrees l <= lmax (soft.f08:797,811,1155,1169,1464,
+    // 1837,2564,3046): the lmn reordering carries just those (the lmn order is degree-major)
+    // and the correlation coefficients of higher degrees are zero.
+    P->lcap = bw - 1;
+    P->nkeep = P->nc;
     if (rt == 1 && !precompute) {
         P->lmn = 1;
+        P->lcap = lmax;
+        P->nkeep = (4 * (lmax + 1) * (lmax + 1) * (lmax + 1) - (lmax + 1)) / 3;
         if ((rc = upload(&P->lmn_perm, lmn_perm_table(bw)))) return rc;
     }
     HIPCHK(hipMalloc((void **)&P->red, sizeof(double) * (2 * 1024 + 2)));
@@ -373,14 +382,17 @@ int dwt_inverse(Plan *P, int mode, const double2 *c, int64_t c_ts, const double2
                 int64_t T, hipStream_t s)
 {
     const int bw = (int)P->bw, N = (int)P->N, np = (int)P->npair;
+    // the MFMA engine forms correlation coefficients in its kernel; the other paths (and
+    // handles that drop degrees above lmax) stage them first
+    if (mode == DWT_CORR && (P->dwt_engine != SO3FT_DWT_MFMA || P->lcap < P->bw - 1)) {
+        if (int rc = ensure_cstage(P, T)) return rc;
+        HIPCHK(launch_corr_coeffs(P->tb, bw, c, aux, c_ts, 0, 1, 0, nullptr, P->cstage, P->nc, (int)P->lcap, (int)T,
+                                  s));
+        c = P->cstage; c_ts = P->nc; mode = DWT_COMPLEX;
+    }
     if (P->dwt_engine == SO3FT_DWT_MFMA) {
         HIPCHK(launch_dwt_inverse(P->tb, bw, N, np, mode, c, c_ts, aux, F, f_ts, (int)T, s));
         return SO3FT_OK;
-    }
-    if (mode == DWT_CORR) {
-        if (int rc = ensure_cstage(P, T)) return rc;
-        HIPCHK(launch_corr_coeffs(P->tb, bw, c, aux, c_ts, 0, 1, 0, nullptr, P->cstage, P->nc, (int)T, s));
-        c = P->cstage; c_ts = P->nc; mode = DWT_COMPLEX;
     }
     HIPCHK(launch_dwt_inverse_valu(P->tb, bw, N, np, mode, c, c_ts, F, f_ts, (int)T, s));
     return SO3FT_OK;
@@ -412,7 +424,7 @@ int run_inverse(Plan *P, int mode, const void *coeff, int64_t c_ts, const void *
         const int64_t T = std::min(ch, n - i0);
         const double2 *c = reinterpret_cast<const double2 *>(coeff) + i0 * c_ts;
         if (perm) {
-            HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, c, c_ts, P->pstage, P->nc, 16, 1, (int)T, s));
+            HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, P->nkeep, c, c_ts, P->pstage, P->nc, 16, 1, (int)T, s));
             c = P->pstage;
         }
         const double2 *ax = aux ? reinterpret_cast<const double2 *>(aux) + i0 * c_ts : nullptr;
@@ -437,7 +449,7 @@ int run_forward(Plan *P, int mode, const void *in, int in_real, void *coeff, int
         if ((rc = forward_fft_passes(P, f, in_real, P->scratch, T, s))) return rc;
         double2 *c = reinterpret_cast<double2 *>(coeff) + i0 * P->nc;
         if ((rc = dwt_forward(P, mode, P->scratch, N3, perm ? P->pstage : c, P->nc, T, s))) return rc;
-        if (perm) HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, P->pstage, P->nc, c, P->nc, 16, 0, (int)T, s));
+        if (perm) HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, P->nkeep, P->pstage, P->nc, c, P->nc, 16, 0, (int)T, s));
     }
     return SO3FT_OK;
 }
@@ -464,7 +476,7 @@ int stage_corr(Plan *P, int layout, const void *f, const void *g, int64_t T, int
     }
     HIPCHK(launch_corr_coeffs(P->tb, (int)P->bw, reinterpret_cast<const double2 *>(f),
                               reinterpret_cast<const double2 *>(g), nb, layout, hw ? nr : 1, nb, w, P->cstage, P->nc,
-                              (int)T, s));
+                              (int)P->lcap, (int)T, s));
     return SO3FT_OK;
 }
 
@@ -533,7 +545,7 @@ int run_inverse_c64(Plan *P, const float2 *coeff, float2 *out, int64_t n, hipStr
         const float2 *c = coeff + i0 * P->nc;
         if (P->lmn) {
             float2 *ps = reinterpret_cast<float2 *>(P->pstage);
-            HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, c, P->nc, ps, P->nc, 8, 1, (int)T, s));
+            HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, P->nkeep, c, P->nc, ps, P->nc, 8, 1, (int)T, s));
             c = ps;
         }
         HIPCHK(launch_dwt_inverse_mixed(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX, c, P->nc, nullptr, F,
@@ -559,7 +571,7 @@ int run_forward_c64(Plan *P, const float2 *in, float2 *coeff, int64_t n, hipStre
         float2 *ps = reinterpret_cast<float2 *>(P->pstage);
         HIPCHK(launch_dwt_forward_mixed(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX, F, N3,
                                         P->lmn ? ps : c, P->nc, (int)T, s));
-        if (P->lmn) HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, ps, P->nc, c, P->nc, 8, 0, (int)T, s));
+        if (P->lmn) HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, P->nkeep, ps, P->nc, c, P->nc, 8, 0, (int)T, s));
     }
     return SO3FT_OK;
 }
@@ -764,7 +776,7 @@ int so3ft_inverse_wigner_trf_cmplx(so3ft_handle h, const void *coeff, void *fft_
     const double2 *c = reinterpret_cast<const double2 *>(coeff);
     if (P->lmn) {
         if (int rc = ensure_pstage(P, 1)) return rc;
-        HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, c, P->nc, P->pstage, P->nc, 16, 1, 1, S(stream)));
+        HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, P->nkeep, c, P->nc, P->pstage, P->nc, 16, 1, 1, S(stream)));
         c = P->pstage;
     }
     if (int rc = dwt_inverse(P, DWT_COMPLEX, c, P->nc, nullptr, F, N3, 1, S(stream))) return rc;
@@ -783,7 +795,7 @@ int so3ft_forward_wigner_trf_cmplx(so3ft_handle h, const void *fft_array, void *
     if (int rc = dwt_forward(P, DWT_COMPLEX, reinterpret_cast<const double2 *>(fft_array), N3, P->lmn ? P->pstage : c,
                              P->nc, 1, S(stream)))
         return rc;
-    if (P->lmn) HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, P->pstage, P->nc, c, P->nc, 16, 0, 1, S(stream)));
+    if (P->lmn) HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, P->nkeep, P->pstage, P->nc, c, P->nc, 16, 0, 1, S(stream)));
     return SO3FT_OK;
 }
 
@@ -1143,9 +1155,9 @@ int so3ft_enforce_real_sym_lmn(int64_t bw, void *coeffs, int64_t n, void *stream
     hipError_t e = hipSuccess;
     for (int64_t i0 = 0; i0 < n && e == hipSuccess; i0 += ch) {
         const int T = (int)std::min(ch, n - i0);
-        e = launch_permute_lmn(tabs.second, nc, c + i0 * nc, nc, w, nc, 16, 1, T, s);
+        e = launch_permute_lmn(tabs.second, nc, nc, c + i0 * nc, nc, w, nc, 16, 1, T, s);
         if (e == hipSuccess) e = launch_enforce_real_sym(w, nc, tabs.first, (int)bw, T, 0, s);
-        if (e == hipSuccess) e = launch_permute_lmn(tabs.second, nc, w, nc, c + i0 * nc, nc, 16, 0, T, s);
+        if (e == hipSuccess) e = launch_permute_lmn(tabs.second, nc, nc, w, nc, c + i0 * nc, nc, 16, 0, T, s);
     }
     const hipError_t es = hipStreamSynchronize(s);  // the work buffer is freed below
     hipFree(w);

@@ -91,10 +91,10 @@ hipError_t launch_dwt_forward_valu(const DeviceTables &t, int bw, int N, int npa
 
 // mnl correlation coefficients of T pairs of SH sets (transform stride t_ts; layout 0 LMc,
 // 1 MLr of real functions), each the w-weighted sum over nr shells at stride r_ts (w ==
-// nullptr: nr = 1, weight 1), into out (stride c_ts)
+// nullptr: nr = 1, weight 1), into out (stride c_ts); degrees l > lcap are set to zero
 hipError_t launch_corr_coeffs(const DeviceTables &t, int bw, const double2 *f, const double2 *g, int64_t t_ts,
-                              int layout, int nr, int64_t r_ts, const double *w, double2 *out, int64_t c_ts, int T,
-                              hipStream_t s);
+                              int layout, int nr, int64_t r_ts, const double *w, double2 *out, int64_t c_ts,
+                              int lcap, int T, hipStream_t s);
 
 // mixed precision: complex64 coefficients and DWT arrays, FP64 recurrence, FP32 MFMA
 // with d split into hi + lo, FP64 accumulation across 16-term blocks
@@ -158,9 +158,11 @@ hipError_t launch_enforce_real_sym(double2 *c, int64_t c_ts, const int64_t *slic
                                    hipStream_t s);
 // half-complex (N, N/2+1, N) <-> full (N, N, N) complex arrays over the alpha axis
 hipError_t launch_halfcomplex(const double2 *in, double2 *out, int N, int to_full, hipStream_t s);
-// lmn <-> mnl reordering of T coefficient sets (esz 16: complex128, 8: complex64)
-hipError_t launch_permute_lmn(const int32_t *perm, int64_t nc, const void *in, int64_t in_ts, void *out,
-                              int64_t out_ts, int esz, int to_mnl, int T, hipStream_t s);
+// lmn <-> mnl reordering of T coefficient sets (esz 16: complex128, 8: complex64); only
+// the first nkeep lmn positions (the degrees l < L for nkeep = NC(L)) are carried: to mnl
+// the others become zero, from mnl they are not written
+hipError_t launch_permute_lmn(const int32_t *perm, int64_t nc, int64_t nkeep, const void *in, int64_t in_ts,
+                              void *out, int64_t out_ts, int esz, int to_mnl, int T, hipStream_t s);
 
 // per transform: first flat index (and value) of max |x| over n_per complex values;
 // pv / pi hold argmax_partials(n_per) * T partials

@@ -230,7 +230,14 @@ class Soft:
         (utils.enforce_real_sym_lmn if lmn else utils.enforce_real_sym_mnl)(coeff, self.bw)
 
     # -- transforms (pysofft/soft.py:255-517) ------------------------------------
-    def _run(self, fn, x, xdtype, xshape, out, odtype, oshape, many=False):
+    def _partial_coeffs(self):
+        """True when a forward transform writes only part of the coefficients: the
+        Kostelec loops stop at max(|m|,|n|) <= lmax (soft.f08:1024-1065) and the Risbo
+        loops at l <= lmax (soft.f08:1155-1183); the rest keep the values of ``out``,
+        zeros for a fresh array (soft.py:289-294 allocates it with get_coeff)."""
+        return self.lmax < self.bw - 1
+
+    def _run(self, fn, x, xdtype, xshape, out, odtype, oshape, many=False, partial=False):
         torch = _torch()
         with torch.cuda.device(self.device):
             xin = _Dev(x, xdtype, self.device, xshape)
@@ -240,7 +247,14 @@ class Soft:
             full = ((n,) + oshape) if many else oshape
             dev_out = out is not None and _is_torch(out) and out.is_cuda and out.dtype == odtype \
                 and out.is_contiguous() and tuple(out.shape) == full
-            o = out if dev_out else torch.empty(full, dtype=odtype, device=self.device)
+            if dev_out:
+                o = out
+            elif partial and out is not None:
+                o = _Dev(out, odtype, self.device).t.reshape(full).clone()
+            elif partial:
+                o = torch.zeros(full, dtype=odtype, device=self.device)
+            else:
+                o = torch.empty(full, dtype=odtype, device=self.device)
             s = _stream_ptr(self.device)
             if many:
                 _lib.check(fn(self._handle, xin.ptr, C.c_void_p(o.data_ptr()), n, s))
@@ -264,7 +278,7 @@ class Soft:
         torch = _torch()
         L = _lib.lib()
         res = self._run(L.so3ft_soft, so3func, torch.complex128, self.grid_shape, out, torch.complex128,
-                        (self.n_coeffs,))
+                        (self.n_coeffs,), partial=self._partial_coeffs())
         return self._wrap(res, out)
 
     def isoft(self, coeff, out=None, use_mp: bool = False):
@@ -277,7 +291,7 @@ class Soft:
         torch = _torch()
         L = _lib.lib()
         res = self._run(L.so3ft_rsoft, so3func, torch.float64, self.grid_shape, out, torch.complex128,
-                        (self.n_coeffs,))
+                        (self.n_coeffs,), partial=self._partial_coeffs())
         return self._wrap(res, out)
 
     def irsoft(self, coeff, out=None, use_mp: bool = False):
@@ -289,7 +303,7 @@ class Soft:
     def soft_many(self, so3funcs, out=None, use_mp: bool = False):
         torch = _torch()
         res = self._run(_lib.lib().so3ft_soft_many, so3funcs, torch.complex128, self.grid_shape, out,
-                        torch.complex128, (self.n_coeffs,), many=True)
+                        torch.complex128, (self.n_coeffs,), many=True, partial=self._partial_coeffs())
         return self._wrap(res, out)
 
     def isoft_many(self, coeffs, out=None, use_mp=False):
@@ -300,7 +314,7 @@ class Soft:
     def rsoft_many(self, so3funcs, out=None, use_mp: bool = False):
         torch = _torch()
         res = self._run(_lib.lib().so3ft_rsoft_many, so3funcs, torch.float64, self.grid_shape, out,
-                        torch.complex128, (self.n_coeffs,), many=True)
+                        torch.complex128, (self.n_coeffs,), many=True, partial=self._partial_coeffs())
         return self._wrap(res, out)
 
     def irsoft_many(self, coeffs, out=None, use_mp: bool = False):
@@ -318,7 +332,7 @@ class Soft:
     def soft_many_c64(self, so3funcs, out=None):
         torch = _torch()
         return self._run(_lib.lib().so3ft_soft_many_c64, so3funcs, torch.complex64, self.grid_shape, out,
-                         torch.complex64, (self.n_coeffs,), many=True)
+                         torch.complex64, (self.n_coeffs,), many=True, partial=self._partial_coeffs())
 
     def _wrap(self, res, out):
         if out is None and isinstance(res, np.ndarray):
@@ -334,7 +348,7 @@ class Soft:
     def _forward_wigner_trf_cmplx(self, so3func, coeff=None, use_mp=False):
         torch = _torch()
         return self._run(_lib.lib().so3ft_forward_wigner_trf_cmplx, so3func, torch.complex128, self.grid_shape,
-                         coeff, torch.complex128, (self.n_coeffs,))
+                         coeff, torch.complex128, (self.n_coeffs,), partial=self._partial_coeffs())
 
     def _fft(self, f1, f2=None):
         torch = _torch()

@@ -1,0 +1,273 @@
+"""GPU parity at the full sizes of the measured workloads, against the CPU oracle.
+
+- the batched rotational cross-correlation at bw=64 (rotated pairs, SNR 10): peak
+  search vs the argmax of the full grids, two pairs vs the oracle's correlation, and
+  the angular error of every pair;
+- FP64 isoft/soft at bw=96 and 128, complex64 transforms at bw=96 and 128 (one full
+  4-transform forward wave plus a tail) and the bw=256 real transforms, all directly
+  against the oracle;
+- the reference's own 3-D correlation checks (tests/test_fortran_backend.py:751-812):
+  the peak of the complex and real 3-D correlation of rotated shells is the grid
+  rotation, for both recurrence types with and without precomputed Wigners;
+- the lmax semantics of the two recurrence types: Kostelec handles sum every degree of
+  the orders max(|m|,|n|) <= lmax (soft.f08:632-675), Risbo handles only the degrees
+  l <= lmax (soft.f08:676-824, 1155-1183), and forward transforms leave the other
+  coefficients untouched (zeros in a fresh output, soft.py:289-294).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _soft(bw, **kw):
+    from pysofft_amd import Soft
+    return Soft(bw, **kw)
+
+
+def _rand_c(rng, n):
+    return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+def _dev_rel(a, b):
+    """relative L2 of two device tensors, in FP64 on the device"""
+    a = a.to(torch.complex128) if a.is_complex() else a.to(torch.float64)
+    b = b.to(a.dtype)
+    return ((a - b).abs().pow(2).sum() / b.abs().pow(2).sum()).sqrt().item()
+
+
+def test_rotated_pairs_bw64_snr10_correlation_and_peaks():
+    """The batched correlation workload at bw=64: pairs (f, g = D(R) f + noise at SNR 10)
+    with Haar-random R.  The fused correlation + peak search returns the first argmax of
+    |C| over the full grids (cross_correlation_ylm_cmplx_many); two pairs' grids equal the
+    oracle's correlation (soft.f08:2613-2656) to 1e-12; every peak is within one grid
+    step (pi/64) of the true rotation."""
+    from pysofft_amd import workloads as W
+    bw, n = 64, 64
+    N = 2 * bw
+    rng = np.random.default_rng(640)
+    f, g, eul = W.rotated_pairs(n, bw, rng, snr=10.0)
+    s = _soft(bw)
+    ft, gt = torch.from_numpy(f).cuda(), torch.from_numpy(g).cuda()
+    peak, val = s.correlation_peaks(ft, gt)
+    cc = s.cross_correlation_ylm_cmplx_many(ft, gt)
+    flat = cc.abs().reshape(n, -1)
+    am = flat.argmax(dim=1)
+    ig, rem = am // (N * N), am % (N * N)
+    ref_peak = torch.stack([rem // N, rem % N, ig], dim=1)
+    assert torch.equal(peak, ref_peak)
+    assert torch.allclose(val, flat.max(dim=1).values, rtol=1e-14, atol=0)
+    P = O.OraclePlan(bw)
+    for i in (0, n - 1):
+        ref = P.cross_correlation_ylm_cmplx(f[i], g[i])
+        assert rel_l2(cc[i].cpu().numpy(), ref) <= 1e-12, i
+    del cc, flat
+    pk = peak.cpu().numpy()
+    a = np.arange(N) * np.pi / bw
+    b = O.beta_samples(N)
+    found = np.stack([a[pk[:, 0]], b[pk[:, 1]], a[pk[:, 2]]], 1)
+    err = W.angular_distance(W.euler_to_matrix(found), W.euler_to_matrix(eul))
+    assert np.all(err <= np.pi / bw), np.degrees(err.max())
+
+
+@pytest.mark.parametrize("bw", [96, 128])
+def test_fp64_isoft_soft_vs_oracle_large(bw):
+    """FP64 isoft and soft at bw = 96 and 128 (FFT lengths 192 and 256) against the
+    oracle on the same seeded inputs: relative L2 <= 1e-12."""
+    rng = np.random.default_rng(9000 + bw)
+    s = _soft(bw)
+    P = O.OraclePlan(bw)
+    c = _rand_c(rng, P.nc)
+    f = s.isoft(torch.from_numpy(c).cuda())
+    fo = P.isoft(c)
+    assert rel_l2(f.cpu().numpy(), fo) <= 1e-12
+    g = _rand_c(rng, P.grid_shape)
+    co = P.soft(g)
+    assert rel_l2(s.soft(torch.from_numpy(g).cuda()).cpu().numpy(), co) <= 1e-12
+
+
+@pytest.mark.parametrize("bw", [96, 128])
+def test_mixed_c64_vs_oracle_large(bw):
+    """complex64 transforms at bw = 96 and 128 with 5 transforms (the forward DWT runs 4
+    transforms per wave from bw = 96: one full wave plus a tail) against the FP64 oracle
+    on the same complex64-rounded inputs: per-transform relative L2 <= 1e-7."""
+    n = 5
+    rng = np.random.default_rng(7000 + bw)
+    s = _soft(bw)
+    P = O.OraclePlan(bw)
+    c = _rand_c(rng, (n, P.nc)).astype(np.complex64)
+    f = s.isoft_many_c64(torch.from_numpy(c).cuda())
+    fo = P.isoft_many(c.astype(np.complex128))
+    fn = f.cpu().numpy()
+    for i in range(n):
+        assert rel_l2(fn[i], fo[i]) <= 1e-7, (bw, i, rel_l2(fn[i], fo[i]))
+    g = fo.astype(np.complex64)
+    del fo, fn
+    r = s.soft_many_c64(torch.from_numpy(g).cuda()).cpu().numpy()
+    ro = P.soft_many(g.astype(np.complex128))
+    for i in range(n):
+        assert rel_l2(r[i], ro[i]) <= 1e-7, (bw, i, rel_l2(r[i], ro[i]))
+        assert rel_l2(r[i], c[i]) <= 2e-7, (bw, i)
+
+
+def test_real_bw256_vs_oracle():
+    """Real FP64 transforms at bw=256 (512^3 float64 grid) against the oracle:
+    irsoft of Hermitian-symmetric coefficients (iid N(0,1) re/im scaled by (2l+1)^-1/2)
+    and rsoft of that grid, relative L2 <= 1e-11."""
+    bw = 256
+    s = _soft(bw)
+    P = O.OraclePlan(bw)
+    rng = np.random.default_rng(2560)
+    ls = O.coeff_degrees(bw)[:, 0]
+    c = (rng.standard_normal(P.nc) + 1j * rng.standard_normal(P.nc)) / np.sqrt(2 * ls + 1)
+    O.enforce_real_sym(c, bw)
+    fo = P.irsoft(c)
+    f = s.irsoft(torch.from_numpy(c).cuda())
+    assert rel_l2(f.cpu().numpy(), fo) <= 1e-11
+    co = P.rsoft(fo)
+    r = s.rsoft(torch.from_numpy(fo).cuda()).cpu().numpy()
+    assert rel_l2(r, co) <= 1e-11
+    assert rel_l2(r, c) <= 1e-11
+
+
+def _grid_rotations(rng, bw, k):
+    a = np.arange(2 * bw) * np.pi / bw
+    b = O.beta_samples(2 * bw)
+    idx = rng.integers(0, 2 * bw, (k, 3))
+    return idx, [(a[i[0]], b[i[1]], a[i[2]]) for i in idx]
+
+
+@pytest.mark.parametrize("precompute", [False, True])
+@pytest.mark.parametrize("rt", [0, 1])
+def test_cross_correlation_3d_peaks_at_grid_rotation(rt, precompute):
+    """tests/test_fortran_backend.py:751-781 at bw=32: bw shells of rng.random LMc
+    coefficients, every shell rotated by the same grid rotation, radial points 0..bw-1,
+    all shells summed; the peak of |C| is the rotation's grid index."""
+    from pysofft_amd import Soft
+    bw = 32
+    rng = np.random.default_rng(751 + 2 * rt + int(precompute))
+    coeff = rng.random((bw, bw * bw)).astype(complex)
+    s = Soft(bw, precompute_wigners=precompute, recurrence_type=rt)
+    rad = np.arange(bw).astype(float)
+    idx, eul = _grid_rotations(rng, bw, 5)
+    for i, e in zip(idx, eul):
+        g = np.asarray(Soft.rotate_ylm_cmplx(coeff, e))
+        cc = s.cross_correlation_ylm_cmplx_3d(coeff, g, radial_sampling_points=rad, radial_limits=[0, bw])
+        ig, ia, ib = np.unravel_index(np.argmax(np.abs(cc)), cc.shape)
+        assert [ia, ib, ig] == list(i), (rt, precompute, i)
+
+
+@pytest.mark.parametrize("precompute", [False, True])
+@pytest.mark.parametrize("rt", [0, 1])
+def test_real_cross_correlation_3d_peaks_at_grid_rotation(rt, precompute):
+    """tests/test_fortran_backend.py:782-812 at bw=32: real functions (rng.random MLr
+    coefficients) on bw shells, rotated by one grid rotation (rotate_ylm_real); the peak
+    of the real 3-D correlation is the rotation's grid index."""
+    from pysofft_amd import Soft
+    bw = 32
+    nr = bw * (bw + 1) // 2
+    rng = np.random.default_rng(782 + 2 * rt + int(precompute))
+    coeff = rng.random((bw, nr)).astype(complex)
+    s = Soft(bw, precompute_wigners=precompute, recurrence_type=rt)
+    rad = np.arange(bw).astype(float)
+    idx, eul = _grid_rotations(rng, bw, 5)
+    for i, e in zip(idx, eul):
+        g = np.asarray(Soft.rotate_ylm_real(coeff, e)).reshape(bw, nr)
+        cc = s.cross_correlation_ylm_real_3d(coeff, g, radial_sampling_points=rad, radial_limits=[0, bw])
+        ig, ia, ib = np.unravel_index(np.argmax(np.abs(cc)), cc.shape)
+        assert [ia, ib, ig] == list(i), (rt, precompute, i)
+
+
+@pytest.mark.parametrize("bw,lmax", [(8, 5), (9, 0), (16, 11)])
+def test_kostelec_lmax_forward_leaves_other_orders(bw, lmax):
+    """Kostelec handle with lmax < bw-1: soft writes the orders max(|m|,|n|) <= lmax (equal
+    to the oracle) and leaves the others: zeros in a fresh output, the caller's values in
+    a given one (soft.f08:1024-1065 loops m1 <= m2 <= lmax only)."""
+    rng = np.random.default_rng(100 + lmax)
+    s = _soft(bw, lmax=lmax)
+    P = O.OraclePlan(bw, lmax)
+    g = _rand_c(rng, P.grid_shape)
+    deg = s.coeff_indices
+    keep = np.maximum(np.abs(deg[:, 1]), np.abs(deg[:, 2])) <= lmax
+    co = np.asarray(s.soft(g))
+    assert rel_l2(co[keep], P.soft(g)[keep]) < 1e-12
+    assert np.all(co[~keep] == 0)
+    ct = s.soft(torch.from_numpy(g).cuda())
+    assert torch.all(ct[torch.from_numpy(~keep).cuda()] == 0)
+    pre = _rand_c(rng, P.nc)
+    out = pre.copy()
+    s.soft(g, out=out)
+    np.testing.assert_array_equal(out[~keep], pre[~keep])
+    assert rel_l2(out[keep], co[keep]) == 0
+    ot = torch.from_numpy(pre).cuda()
+    s.soft(torch.from_numpy(g).cuda(), out=ot)
+    np.testing.assert_array_equal(ot.cpu().numpy()[~keep], pre[~keep])
+    # the real forward transform follows the same rule
+    gr = g.real.copy()
+    cr = np.asarray(s.rsoft(gr))
+    assert np.all(cr[~keep] == 0)
+    assert rel_l2(cr[keep], P.rsoft(gr)[keep]) < 1e-12
+
+
+@pytest.mark.parametrize("bw,lmax", [(8, 5), (9, 0), (16, 11), (12, 11)])
+def test_risbo_lmax_sums_degrees_up_to_lmax(bw, lmax):
+    """Risbo handle (lmn order) with lmax: isoft sums only l <= lmax, i.e. equals the
+    Kostelec isoft (lmax = bw-1) of the same coefficients with l > lmax zeroed; soft
+    writes only l <= lmax (equal to the full soft there) and leaves the rest; the same for
+    the real transforms, the complex64 path and the correlation (soft.f08:2564-2580)."""
+    from pysofft_amd import utils
+    rng = np.random.default_rng(200 + bw + lmax)
+    s = _soft(bw, lmax=lmax, recurrence_type=1)
+    assert s.coeff_order == 'lmn' and s.lmax == lmax
+    P = O.OraclePlan(bw)
+    perm = utils.lmn_to_mnl(bw)
+    deg_lmn = s.coeff_indices
+    low = deg_lmn[:, 0] <= lmax
+    c = _rand_c(rng, P.nc)
+    cz = np.where(low, c, 0)
+    cm = np.empty_like(cz)
+    cm[perm] = cz
+    assert rel_l2(s.isoft(c), P.isoft(cm)) < 1e-12
+    # forward: l <= lmax equal to the full transform, l > lmax untouched
+    g = _rand_c(rng, P.grid_shape)
+    full = P.soft(g)[perm]
+    co = np.asarray(s.soft(g))
+    assert rel_l2(co[low], full[low]) < 1e-12
+    assert np.all(co[~low] == 0)
+    pre = _rand_c(rng, P.nc)
+    out = pre.copy()
+    s.soft(g, out=out)
+    np.testing.assert_array_equal(out[~low], pre[~low])
+    # stage entry points
+    assert rel_l2(s._inverse_wigner_trf_cmplx(c), P.inverse_dwt(cm)) < 1e-12
+    # real transforms
+    cr = c.copy()
+    s.enforce_real_symmetry(cr)
+    crz = np.where(low, cr, 0)
+    crm = np.empty_like(crz)
+    crm[perm] = crz
+    fr = s.irsoft(cr)
+    assert rel_l2(fr, P.irsoft(crm)) < 1e-12
+    rr = np.asarray(s.rsoft(fr))
+    assert rel_l2(rr[low], P.rsoft(fr)[perm][low]) < 1e-12
+    assert np.all(rr[~low] == 0)
+    # complex64 path (bw = 2^a or 3 * 2^a)
+    if bw in (8, 12, 16):
+        c32 = torch.from_numpy(c.astype(np.complex64)[None]).cuda()
+        f32 = s.isoft_many_c64(c32)
+        assert rel_l2(f32[0].cpu().numpy(), P.isoft(cm.astype(np.complex64).astype(np.complex128))) < 1e-6
+        r32 = s.soft_many_c64(torch.from_numpy(g.astype(np.complex64)[None]).cuda()).cpu().numpy()[0]
+        assert np.all(r32[~low] == 0)
+        assert rel_l2(r32[low], full[low]) < 1e-6
+    # correlation: degrees above lmax do not contribute
+    fl, gl = _rand_c(rng, bw * bw), _rand_c(rng, bw * bw)
+    lsh = np.floor(np.sqrt(np.arange(bw * bw))).astype(int)
+    ref = P.cross_correlation_ylm_cmplx(np.where(lsh <= lmax, fl, 0), gl)
+    assert rel_l2(s.cross_correlation_ylm_cmplx(fl, gl), ref) < 1e-12
+    s.dwt_engine = "mfma"
+    assert rel_l2(s.cross_correlation_ylm_cmplx(fl, gl), ref) < 1e-12
