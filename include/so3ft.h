@@ -97,6 +97,17 @@ int so3ft_rsoft_many(so3ft_handle h, const void *so3funcs_real, void *coeffs, in
 int so3ft_isoft_many_c64(so3ft_handle h, const void *coeffs, void *so3funcs, int64_t n, void *stream);
 int so3ft_soft_many_c64(so3ft_handle h, const void *so3funcs, void *coeffs, int64_t n, void *stream);
 
+/* ---- zero-allocation forward transform (no counterpart in the reference): for callers
+ * that cannot allocate device memory inside the call (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, results and semantics
+ * as so3ft_soft_many_c64, but its scratch (the fft arrays of up to 256 MiB / (8 (2bw)^3)
+ * transforms per pass, plus their coefficient sets on Risbo handles) is a static device
+ * array of 256 MiB in the library, one per GPU, held until the process ends; the call
+ * allocates nothing.  One transform must fit: bw <= 128, else SO3FT_EINVAL.  Every call on
+ * one GPU shares that GPU's array, so calls that overlap on one GPU (other streams, other
+ * threads) race and must be ordered by the caller. */
+int so3ft_soft_many_c64_noalloc(so3ft_handle h, const void *so3funcs, void *coeffs, int64_t n, void *stream);
+
 /* ---- grouped mixed-precision transforms of uneven bandwidths (no counterpart in the
  * reference, whose *_many take one bandwidth per instance).  A group fixes n bandwidths
  * (2 <= bw <= 256, bw = 2^a or 3*2^a); its transforms run as one persistent DWT launch

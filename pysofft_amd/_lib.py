@@ -41,6 +41,7 @@ SIGNATURES = {
     "so3ft_rsoft_many": (_int, [_i64, _vp, _vp, _i64, _vp]),
     "so3ft_isoft_many_c64": (_int, [_i64, _vp, _vp, _i64, _vp]),
     "so3ft_soft_many_c64": (_int, [_i64, _vp, _vp, _i64, _vp]),
+    "so3ft_soft_many_c64_noalloc": (_int, [_i64, _vp, _vp, _i64, _vp]),
     "so3ft_inverse_wigner_trf_cmplx": (_int, [_i64, _vp, _vp, _vp]),
     "so3ft_forward_wigner_trf_cmplx": (_int, [_i64, _vp, _vp, _vp]),
     "so3ft_fft": (_int, [_i64, _vp, _vp, _vp]),

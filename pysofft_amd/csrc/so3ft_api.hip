@@ -555,24 +555,68 @@ int run_inverse_c64(Plan *P, const float2 *coeff, float2 *out, int64_t n, hipStr
     return SO3FT_OK;
 }
 
-int run_forward_c64(Plan *P, const float2 *in, float2 *coeff, int64_t n, hipStream_t s)
+// ext: caller-provided scratch (the static array of the zero-allocation entry point)
+// holding ext_T transforms' fft arrays followed, for lmn handles, by ext_T coefficient
+// sets; nullptr: the plan's own scratch, allocated on demand
+int run_forward_c64(Plan *P, const float2 *in, float2 *coeff, int64_t n, hipStream_t s, float2 *ext = nullptr,
+                    int64_t ext_T = 0)
 {
     if (int rc = check_c64(P)) return rc;
     const int64_t N = P->N, N3 = N * N * N;
-    const int64_t ch = chunk_of(P, n);
-    int rc = ensure_scratch(P, (ch + 1) / 2);
-    if (rc) return rc;
-    if (P->lmn && (rc = ensure_pstage(P, (ch + 1) / 2))) return rc;
-    float2 *F = reinterpret_cast<float2 *>(P->scratch);
+    int64_t ch;
+    float2 *F, *ps;
+    if (ext) {
+        ch = std::min(ext_T, n);
+        F = ext;
+        ps = ext + ext_T * N3;
+    } else {
+        ch = chunk_of(P, n);
+        int rc = ensure_scratch(P, (ch + 1) / 2);
+        if (rc) return rc;
+        if (P->lmn && (rc = ensure_pstage(P, (ch + 1) / 2))) return rc;
+        F = reinterpret_cast<float2 *>(P->scratch);
+        ps = reinterpret_cast<float2 *>(P->pstage);
+    }
     for (int64_t i0 = 0; i0 < n; i0 += ch) {
         const int64_t T = std::min(ch, n - i0);
-        if ((rc = forward_fft_passes_c64(P, in + i0 * N3, F, T, s))) return rc;
+        if (int rc = forward_fft_passes_c64(P, in + i0 * N3, F, T, s)) return rc;
         float2 *c = coeff + i0 * P->nc;
-        float2 *ps = reinterpret_cast<float2 *>(P->pstage);
         HIPCHK(launch_dwt_forward_mixed(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX, F, N3,
                                         P->lmn ? ps : c, P->nc, (int)T, s));
         if (P->lmn) HIPCHK(launch_permute_lmn(P->lmn_perm, P->nc, P->nkeep, ps, P->nc, c, P->nc, 8, 0, (int)T, s));
     }
+    return SO3FT_OK;
+}
+
+// ---- scratch of the zero-allocation forward transform (so3ft_soft_many_c64_noalloc).
+// The array lives in the library's code object: every GPU the library is loaded on holds
+// its own copy until the process ends.  In host code `scratch` names a host-side stand-in;
+// only the device address from hipGetSymbolAddress (kept per device) is passed to kernels.
+constexpr size_t STATIC_SCRATCH_BYTES = size_t(256) << 20;
+__device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_ptr;
+
+// device address of `scratch` on device dev, resolved once per device with dev current
+int static_scratch(int dev, void **ptr)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) {
+        *ptr = it->second;
+        return SO3FT_OK;
+    }
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPCHK(hipSetDevice(cur));
+    if (e != hipSuccess || p == nullptr)
+        return fail(SO3FT_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+    *ptr = p;
     return SO3FT_OK;
 }
 
@@ -595,6 +639,10 @@ int so3ft_init(int64_t bw, int64_t lmax, int precompute_wigners, int init_ffts, 
         delete P;
         return rc;
     }
+    // resolve the zero-allocation entry point's scratch on this device now (the plan's
+    // device is current); a failure is reported by that entry point, which tries again
+    void *sp = nullptr;
+    if (static_scratch(P->device, &sp) != SO3FT_OK) g_err.clear();
     std::lock_guard<std::mutex> lk(g_mu);
     *out = g_next++;
     g_plans[*out] = P;
@@ -765,6 +813,24 @@ int so3ft_soft_many_c64(so3ft_handle h, const void *so3funcs, void *coeffs, int6
     if (n < 0) return fail(SO3FT_EINVAL, "n must be >= 0");
     return run_forward_c64(P, reinterpret_cast<const float2 *>(so3funcs), reinterpret_cast<float2 *>(coeffs), n,
                            S(stream));
+}
+
+int so3ft_soft_many_c64_noalloc(so3ft_handle h, const void *so3funcs, void *coeffs, int64_t n, void *stream)
+{
+    GETP(h);
+    if (int rc = check_ptrs(coeffs, so3funcs)) return rc;
+    if (n < 0) return fail(SO3FT_EINVAL, "n must be >= 0");
+    if (int rc = check_c64(P)) return rc;
+    const int64_t N3 = P->N * P->N * P->N;
+    const size_t per = sizeof(float2) * (size_t)(N3 + (P->lmn ? P->nc : 0));
+    if (per > STATIC_SCRATCH_BYTES)
+        return fail(SO3FT_EINVAL, "so3ft_soft_many_c64_noalloc: one transform at bw=" + std::to_string(P->bw) +
+                                      " needs " + std::to_string(per) + " bytes of scratch, beyond the static " +
+                                      std::to_string(STATIC_SCRATCH_BYTES) + " (bw <= 128)");
+    void *sp = nullptr;
+    if (int rc = static_scratch(P->device, &sp)) return rc;
+    return run_forward_c64(P, reinterpret_cast<const float2 *>(so3funcs), reinterpret_cast<float2 *>(coeffs), n,
+                           S(stream), reinterpret_cast<float2 *>(sp), (int64_t)(STATIC_SCRATCH_BYTES / per));
 }
 
 int so3ft_inverse_wigner_trf_cmplx(so3ft_handle h, const void *coeff, void *fft_array, void *stream)
