@@ -58,6 +58,11 @@ def _load():
         "or_integrate": (None, [_vp, _vp, _dp, _dp]),
         "or_enforce_real_sym": (None, [_i64, _vp]),
         "or_corr_coeffs": (None, [_i64, _vp, _vp, _vp]),
+        "or_risbo_reduced_step": (None, [_dp, _i64, _i64, _dp, _dp, _dp, C.c_int, _dp]),
+        "or_dtab_rows": (_i64, [_i64]),
+        "or_genwig_all_risbo": (None, [_i64, C.c_int, _dp]),
+        "or_genwig_all_kostelec": (None, [_i64, _dp]),
+        "or_full_big_d_risbo": (None, [_i64, C.c_double, C.c_double, C.c_double, C.c_int, _vp]),
         "or_cross_correlation_cmplx": (None, [_vp, _vp, _vp, _vp, C.c_int]),
         "or_cross_correlation_cmplx_3d": (None, [_vp, _vp, _vp, _i64, _dp, _i64, _i64, _vp, C.c_int]),
         "or_risbo_step": (None, [_dp, _i64, C.c_double, C.c_double, _dp, _dp]),
@@ -183,6 +188,45 @@ def risbo_step(dprev, l, c, s, sq):
     out = np.zeros((2 * l + 1) ** 2)
     dp = _f64(dprev) if dprev is not None and len(dprev) else np.zeros(1)
     _lib.or_risbo_step(_p(dp), l, c, s, _p(_f64(sq)), _p(out))
+    return out
+
+
+def risbo_reduced_step(d1, l, c, s, normalized=True, sq=None):
+    """wigner.f90:626-748: one step d^{l-1} -> d^l of the reduced Risbo recurrence at the
+    betas whose half-angle cosines / sines are c, s (a set closed under beta -> pi - beta,
+    reversed); d1, result: [tri][nb]."""
+    c, s = _f64(c), _f64(s)
+    nb = len(c)
+    sq = _f64(np.sqrt(np.arange(max(3, 2 * l + 2))) if sq is None else sq)
+    out = np.zeros(((l + 1) * (l + 2) // 2, nb))
+    dp = _f64(d1) if d1 is not None and np.size(d1) else np.zeros(nb)
+    _lib.or_risbo_reduced_step(_p(dp), l, nb, _p(c), _p(s), _p(sq), int(normalized), _p(out))
+    return out
+
+
+def dtab_rows(bw):
+    """rows (p, q, l), 0 <= p <= q <= l < bw, of the mnl-ordered d table"""
+    return int(_lib.or_dtab_rows(bw))
+
+
+def genwig_all_risbo(bw, normalized=True):
+    """wigner.f90:784-822: the d table by the Risbo recurrence, [mnl row][2bw]"""
+    out = np.zeros((dtab_rows(bw), 2 * bw))
+    _lib.or_genwig_all_risbo(bw, int(normalized), _p(out))
+    return out
+
+
+def genwig_all_kostelec(bw):
+    """wigner.f90:423-465: the d table by the Kostelec recurrence, [mnl row][2bw]"""
+    out = np.zeros((dtab_rows(bw), 2 * bw))
+    _lib.or_genwig_all_kostelec(bw, _p(out))
+    return out
+
+
+def full_big_d_risbo(bw, alpha, beta, gamma, normalized=True):
+    """wigner.f90:844-934: D^l_{mn}(alpha, beta, gamma) in mnl order (complex128)"""
+    out = np.zeros(total_num_coeffs(bw), np.complex128)
+    _lib.or_full_big_d_risbo(bw, alpha, beta, gamma, int(normalized), _v(out))
     return out
 
 
@@ -392,6 +436,10 @@ class RefLib:
                 "ref_wigner_dl_kostelec": (None, [_i64, _i64, _dp, C.c_int, _dp]),
                 "ref_risbo_step": (None, [_dp, _i64, C.c_double, C.c_double, _dp, _i64, _dp]),
                 "ref_full_big_d": (None, [_i64, C.c_double, C.c_double, C.c_double, _vp]),
+                "ref_full_big_d_norm": (None, [_i64, C.c_double, C.c_double, C.c_double, C.c_int, _vp]),
+                "ref_risbo_reduced_step": (None, [_dp, _i64, _i64, _dp, _dp, _dp, _i64, C.c_int, _dp]),
+                "ref_genwig_all_risbo": (None, [_i64, C.c_int, _dp]),
+                "ref_genwig_all_kostelec": (None, [_i64, _dp]),
                 "ref_enforce_real_sym_mnl": (None, [_vp, _i64]),
                 "ref_coeff_location_lmn": (_i64, [_i64, _i64, _i64]),
                 "ref_coeff_degrees_risbo": (None, [_i64, _ip]),
@@ -482,6 +530,34 @@ class RefLib:
     def full_big_d(cls, bw, alpha, beta, gamma):
         out = np.zeros(total_num_coeffs(bw), np.complex128)
         cls.lib().ref_full_big_d(bw, alpha, beta, gamma, _v(out))
+        return out
+
+    @classmethod
+    def full_big_d_norm(cls, bw, alpha, beta, gamma, normalized=True):
+        out = np.zeros(total_num_coeffs(bw), np.complex128)
+        cls.lib().ref_full_big_d_norm(bw, alpha, beta, gamma, int(normalized), _v(out))
+        return out
+
+    @classmethod
+    def risbo_reduced_step(cls, d1, l, c, s, normalized=True, sq=None):
+        c, s = _f64(c), _f64(s)
+        nb = len(c)
+        sq = _f64(np.sqrt(np.arange(max(3, 2 * l + 2))) if sq is None else sq)
+        out = np.zeros(((l + 1) * (l + 2) // 2, nb))
+        dp = _f64(d1) if d1 is not None and np.size(d1) else np.zeros(nb)
+        cls.lib().ref_risbo_reduced_step(_p(dp), l, nb, _p(c), _p(s), _p(sq), len(sq), int(normalized), _p(out))
+        return out
+
+    @classmethod
+    def genwig_all_risbo(cls, bw, normalized=True):
+        out = np.zeros((dtab_rows(bw), 2 * bw))
+        cls.lib().ref_genwig_all_risbo(bw, int(normalized), _p(out))
+        return out
+
+    @classmethod
+    def genwig_all_kostelec(cls, bw):
+        out = np.zeros((dtab_rows(bw), 2 * bw))
+        cls.lib().ref_genwig_all_kostelec(bw, _p(out))
         return out
 
     @classmethod

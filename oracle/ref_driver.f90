@@ -131,6 +131,56 @@ contains
     out = full_big_d_risbo(b, alpha, beta, gamma, .true.)
   end subroutine
 
+  !> wigner.f90:844-934 full_big_d_risbo with the normalized flag
+  subroutine ref_full_big_d_norm(bw, alpha, beta, gamma, normalized, out) bind(C, name="ref_full_big_d_norm")
+    integer(c_int64_t), value :: bw
+    real(c_double), value :: alpha, beta, gamma
+    integer(c_int), value :: normalized
+    complex(c_double_complex), intent(out) :: out((4*bw*bw*bw - bw)/3)
+    integer(c_int64_t) :: b
+    b = bw
+    out = full_big_d_risbo(b, alpha, beta, gamma, normalized /= 0)
+  end subroutine
+
+  !> wigner.f90:626-748 one step of the symmetry-reduced Risbo recurrence, degree l-1 -> l,
+  !! at nb betas (a set closed under beta -> pi - beta, reversed order); arrays C order
+  !! [tri][nb] (= Fortran (nb, tri))
+  subroutine ref_risbo_reduced_step(dl1, l, nb, cb, sb, sqrts, nsq, normalized, out) &
+      bind(C, name="ref_risbo_reduced_step")
+    integer(c_int64_t), value :: l, nb, nsq
+    real(c_double), intent(in) :: dl1(nb, max(1_c_int64_t, (l*(l + 1))/2))
+    real(c_double), intent(in) :: cb(nb), sb(nb), sqrts(nsq)
+    integer(c_int), value :: normalized
+    real(c_double), intent(out) :: out(nb, ((l + 1)*(l + 2))/2)
+    integer(c_int64_t) :: ll
+    ll = l
+    if (l == 0) then
+      out = wigner_recurrence_risbo_reduced(dl1(:, 1:0), ll, cb, sb, sqrts, normalized /= 0)
+    else
+      out = wigner_recurrence_risbo_reduced(dl1, ll, cb, sb, sqrts, normalized /= 0)
+    end if
+  end subroutine
+
+  !> wigner.f90:784-822 genwig_all_risbo: every d~^l_{m,n}(beta_k), 0 <= m <= n <= l < bw, in
+  !! mnl order, C order [mnl][2bw]
+  subroutine ref_genwig_all_risbo(bw, normalized, out) bind(C, name="ref_genwig_all_risbo")
+    integer(c_int64_t), value :: bw
+    integer(c_int), value :: normalized
+    real(c_double), intent(out) :: out(2*bw, (bw*(2 + 3*bw + bw*bw))/6)
+    integer(c_int64_t) :: b
+    b = bw
+    out = genwig_all_risbo(b, normalized /= 0)
+  end subroutine
+
+  !> wigner.f90:430-491 genwig_all_kostelec, C order [mnl][2bw]
+  subroutine ref_genwig_all_kostelec(bw, out) bind(C, name="ref_genwig_all_kostelec")
+    integer(c_int64_t), value :: bw
+    real(c_double), intent(out) :: out(2*bw, (bw*(2 + 3*bw + bw*bw))/6)
+    integer(c_int64_t) :: b
+    b = bw
+    call genwig_all_kostelec_preallocated(b, out, .true.)
+  end subroutine
+
   !> utils.f90:471-488
   subroutine ref_enforce_real_sym_mnl(coeff, bw) bind(C, name="ref_enforce_real_sym_mnl")
     integer(c_int64_t), value :: bw

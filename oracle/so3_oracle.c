@@ -766,6 +766,222 @@ void or_risbo_step(const double *dprev, i64 l, double c, double s, const double 
 }
 
 /*
+ * Symmetry-reduced Risbo recurrence, wigner.f90:626-748: d^{l-1}_{m,n} (0 <= m <= n <= l-1)
+ * -> d^l_{m,n} (0 <= m <= n <= l) at nb betas, through the half degree l - 1/2.  Arrays
+ * are [tri][nb] with tri(m, n) over the triangle of side l (input) / l + 1 (output) and
+ * the betas contiguous; the set of betas must be closed under beta -> pi - beta in
+ * reversed order (the step reads the half-degree values at pi - beta as the reversed
+ * array).  c[k] = cos(beta_k / 2), s[k] = sin(beta_k / 2); sq[k] = sqrt(k) (only sq[2] is
+ * read).  normalized: d~ = sqrt((2l+1)/2) d.
+ */
+void or_risbo_reduced_step(const double *d1, i64 l, i64 nb, const double *c, const double *s, const double *sq,
+                           int normalized, double *d)
+{
+#define T1(m, n) ((((m) * (2 * l - (m) + 1)) / 2 + (n) - (m)) * nb)            /* side l     */
+#define T2(m, n) ((((m) * (2 * (l + 1) - (m) + 1)) / 2 + (n) - (m)) * nb)      /* side l + 1 */
+    if (l == 0) {
+        for (i64 k = 0; k < nb; ++k) d[k] = normalized ? sqrt(0.5) : 1.0;
+        return;
+    }
+    if (l == 1) {
+        const double nc = sqrt(1.5);
+        for (i64 k = 0; k < nb; ++k) {
+            const double a = c[k] * c[k] - s[k] * s[k], b = sq[2] * c[k] * s[k], e = c[k] * c[k];
+            d[k] = normalized ? a * nc : a;
+            d[nb + k] = normalized ? b * nc : b;
+            d[2 * nb + k] = normalized ? e * nc : e;
+        }
+        return;
+    }
+    const i64 nh = (l * (l + 1)) / 2;
+    double *h = calloc((size_t)(nh * nb), sizeof(double));
+    /* l -> l + 1/2 (the half-degree values live on the side-l triangle) */
+    {
+        const double inv = 1.0 / (double)(2 * l - 1);
+        const i64 ll = l - 1;
+        const double nc = normalized ? inv * sqrt((double)(2 * l) / (double)(2 * l - 1)) : inv;
+        for (i64 m = 0; m <= l - 2; ++m) {
+            for (i64 k = 0; k < nb; ++k)
+                h[T1(m, m) + k] = (d1[T1(m, m) + k] * (double)(l + m) * c[k]
+                                   - d1[T1(m, m + 1) + k] * sqrt((double)((l + m) * (ll - m))) * s[k]
+                                   - d1[T1(m, m + 1) + k] * sqrt((double)((ll - m) * (l + m))) * s[k]
+                                   + d1[T1(m + 1, m + 1) + k] * (double)(ll - m) * c[k]) * nc;
+            for (i64 n = m + 1; n <= l - 2; ++n)
+                for (i64 k = 0; k < nb; ++k)
+                    h[T1(m, n) + k] = (d1[T1(m, n) + k] * sqrt((double)((l + m) * (l + n))) * c[k]
+                                       - d1[T1(m, n + 1) + k] * sqrt((double)((l + m) * (ll - n))) * s[k]
+                                       + d1[T1(m + 1, n) + k] * sqrt((double)((ll - m) * (l + n))) * s[k]
+                                       + d1[T1(m + 1, n + 1) + k] * sqrt((double)((ll - m) * (ll - n))) * c[k]) * nc;
+            for (i64 k = 0; k < nb; ++k)
+                h[T1(m, l - 1) + k] = (d1[T1(m, l - 1) + k] * sqrt((double)((l + m) * (2 * l - 1))) * c[k]
+                                       + d1[T1(m + 1, l - 1) + k] * sqrt((double)((ll - m) * (2 * l - 1))) * s[k]) * nc;
+        }
+        for (i64 k = 0; k < nb; ++k)
+            h[T1(l - 1, l - 1) + k] = d1[T1(l - 1, l - 1) + k] * (double)(2 * l - 1) * c[k] * nc;
+    }
+    /* l + 1/2 -> l + 1; hr(idx, k) = the half-degree value at pi - beta_k */
+#define HR(idx, k) h[(idx) + (nb - 1 - (k))]
+    {
+        const double inv = 1.0 / (double)(2 * l);
+        const double nc = normalized ? inv * sqrt((double)(2 * l + 1) / (double)(2 * l)) : inv;
+        const double nc2 = normalized ? 0.5 * sqrt((double)(2 * l + 1) / (double)(2 * l)) : 0.5;
+        const double lsym = (l & 1) ? -1.0 : 1.0, lpsym = -lsym;
+        for (i64 k = 0; k < nb; ++k)
+            d[T2(0, 0) + k] = (2.0 * h[T1(0, 0) + k] * c[k] - lpsym * HR(T1(0, 0), k) * s[k]
+                               + lsym * HR(T1(0, 0), k) * s[k]) * nc2;
+        for (i64 n = 1; n <= l - 1; ++n) {
+            const double nsym = (n & 1) ? -1.0 : 1.0;
+            for (i64 k = 0; k < nb; ++k)
+                d[T2(0, n) + k] = (lsym * nsym * HR(T1(0, n - 1), k) * sqrt((double)(l * (l + n))) * c[k]
+                                   - lpsym * nsym * HR(T1(0, n), k) * sqrt((double)(l * (l - n))) * s[k]
+                                   + h[T1(0, n - 1) + k] * sqrt((double)(l * (l + n))) * s[k]
+                                   + h[T1(0, n) + k] * sqrt((double)(l * (l - n))) * c[k]) * nc;
+        }
+        for (i64 k = 0; k < nb; ++k)
+            d[T2(0, l) + k] = (HR(T1(0, l - 1), k) * c[k] + h[T1(0, l - 1) + k] * s[k]) * nc2 * sqrt(2.0);
+        for (i64 m = 1; m <= l - 1; ++m) {
+            for (i64 k = 0; k < nb; ++k)
+                d[T2(m, m) + k] = (h[T1(m - 1, m - 1) + k] * (double)(l + m) * c[k]
+                                   - h[T1(m - 1, m) + k] * sqrt((double)((l + m) * (l - m))) * s[k]
+                                   - h[T1(m - 1, m) + k] * sqrt((double)((l - m) * (l + m))) * s[k]
+                                   + h[T1(m, m) + k] * (double)(l - m) * c[k]) * nc;
+            for (i64 n = m + 1; n <= l - 1; ++n)
+                for (i64 k = 0; k < nb; ++k)
+                    d[T2(m, n) + k] = (h[T1(m - 1, n - 1) + k] * sqrt((double)((l + m) * (l + n))) * c[k]
+                                       - h[T1(m - 1, n) + k] * sqrt((double)((l + m) * (l - n))) * s[k]
+                                       + h[T1(m, n - 1) + k] * sqrt((double)((l - m) * (l + n))) * s[k]
+                                       + h[T1(m, n) + k] * sqrt((double)((l - m) * (l - n))) * c[k]) * nc;
+            for (i64 k = 0; k < nb; ++k)
+                d[T2(m, l) + k] = (h[T1(m - 1, l - 1) + k] * sqrt((double)((l + m) * (2 * l))) * c[k]
+                                   + h[T1(m, l - 1) + k] * sqrt((double)((l - m) * (2 * l))) * s[k]) * nc;
+        }
+        for (i64 k = 0; k < nb; ++k)
+            d[T2(l, l) + k] = h[T1(l - 1, l - 1) + k] * (double)(2 * l) * c[k] * nc;
+    }
+#undef HR
+#undef T1
+#undef T2
+    free(h);
+}
+
+/* number of (p, q, l) entries 0 <= p <= q <= l < bw: the d table's rows */
+i64 or_dtab_rows(i64 bw) { return (bw * (2 + 3 * bw + bw * bw)) / 6; }
+
+/* first row of pair (p, q) in the mnl d table (utils.f90:683-695 minus one) */
+static i64 dtab_row(i64 p, i64 q, i64 bw)
+{
+    return (p * (3 * bw * bw + p * p + 2 - 3 * bw * (p - 2) - 3 * p)) / 6 + ((q - p) * (2 * bw + 1 - p - q)) / 2;
+}
+
+/*
+ * genwig_all_risbo, wigner.f90:784-822: d~^l_{p,q}(beta_k) for 0 <= p <= q <= l < bw at the
+ * 2bw beta samples, by the reduced Risbo recurrence, stored [mnl row][2bw] (row of (p,q,l)
+ * = dtab_row(p, q) + l - q).
+ */
+void or_genwig_all_risbo(i64 bw, int normalized, double *out)
+{
+    const i64 N = 2 * bw;
+    double *c = malloc(sizeof(double) * N), *s = malloc(sizeof(double) * N), *sq = malloc(sizeof(double) * (2 * N + 1));
+    for (i64 k = 0; k < N; ++k) {
+        const double b = (double)(2 * k + 1) * (PI / (double)(2 * N));
+        c[k] = cos(b / 2);
+        s[k] = sin(b / 2);
+    }
+    for (i64 k = 0; k <= 2 * N; ++k) sq[k] = sqrt((double)k);
+    const i64 nt = (bw * (bw + 1)) / 2;
+    double *a = calloc((size_t)(nt * N), sizeof(double)), *b2 = calloc((size_t)(nt * N), sizeof(double));
+    for (i64 l = 0; l < bw; ++l) {
+        or_risbo_reduced_step(a, l, N, c, s, sq, normalized, b2);
+        for (i64 m = 0; m <= l; ++m)
+            for (i64 n = m; n <= l; ++n) {
+                const i64 src = ((m * (2 * (l + 1) - m + 1)) / 2 + n - m) * N;
+                memcpy(out + (dtab_row(m, n, bw) + l - n) * N, b2 + src, sizeof(double) * N);
+            }
+        double *t = a; a = b2; b2 = t;
+    }
+    free(a); free(b2); free(c); free(s); free(sq);
+}
+
+/*
+ * full_big_d_risbo, wigner.f90:844-934: D^l_{m,n}(alpha, beta, gamma) =
+ * e^{-i m alpha} d^l_{m,n}(beta) e^{-i n gamma} for every (l, m, n) in mnl order, the reduced
+ * Risbo values at beta and pi - beta spread over the eight (m, n) variants with the signs
+ * of that routine; normalized: d~ and an overall 1 / (2 pi).
+ */
+void or_full_big_d_risbo(i64 bw, double alpha, double beta, double gamma, int normalized, zc *out)
+{
+    double c[2], s[2];
+    const double bb[2] = {beta, PI - beta};
+    for (int k = 0; k < 2; ++k) {
+        c[k] = cos(bb[k] / 2);
+        s[k] = sin(bb[k] / 2);
+    }
+    double *sq = malloc(sizeof(double) * (2 * bw + 1));
+    for (i64 k = 0; k <= 2 * bw; ++k) sq[k] = sqrt((double)k);
+    i64 *sl = malloc(sizeof(i64) * (2 * bw - 1) * (2 * bw - 1));
+    or_slice_table(bw, sl);
+    const i64 nt = (bw * (bw + 1)) / 2;
+    double *a = calloc((size_t)(2 * nt), sizeof(double)), *b2 = calloc((size_t)(2 * nt), sizeof(double));
+#define LOC(m, n, l) (sl[mnidx((m), (n), bw)] + (l) - ((llabs(m) > llabs(n)) ? llabs(m) : llabs(n)))
+#define EX(ma, ng) (cexp(I * ((double)(ma) * alpha)) * cexp(I * ((double)(ng) * gamma)))
+    for (i64 l = 0; l < bw; ++l) {
+        or_risbo_reduced_step(a, l, 2, c, s, sq, normalized, b2);
+        const double sl_ = (l & 1) ? -1.0 : 1.0;
+        for (i64 m = 0; m <= l; ++m) {
+            const double sm = (m & 1) ? -1.0 : 1.0;
+            for (i64 n = m; n <= l; ++n) {
+                const double sn = (n & 1) ? -1.0 : 1.0;
+                const i64 t = ((m * (2 * (l + 1) - m + 1)) / 2 + n - m) * 2;
+                const double d1 = b2[t], d2 = b2[t + 1];
+                out[LOC(m, n, l)] = EX(-m, -n) * d1;
+                if (m == 0 && n == 0) continue;
+                out[LOC(-n, -m, l)] = EX(n, m) * d1;
+                if (m != n) {
+                    out[LOC(n, m, l)] = (sm * sn) * EX(-n, -m) * d1;
+                    out[LOC(-m, -n, l)] = (sm * sn) * EX(m, n) * d1;
+                }
+                if (m == 0 || n == 0) continue;
+                out[LOC(-m, n, l)] = (sm * sl_) * EX(m, -n) * d2;
+                out[LOC(m, -n, l)] = (sn * sl_) * EX(-m, n) * d2;
+                if (m == n) continue;
+                out[LOC(-n, m, l)] = (sm * sl_) * EX(n, -m) * d2;
+                out[LOC(n, -m, l)] = (sn * sl_) * EX(-n, m) * d2;
+            }
+        }
+        double *tmp = a; a = b2; b2 = tmp;
+    }
+#undef LOC
+#undef EX
+    if (normalized) {
+        const i64 nc = or_total_num_coeffs(bw);
+        for (i64 i = 0; i < nc; ++i) out[i] *= 1.0 / (2.0 * PI);
+    }
+    free(a); free(b2); free(sq); free(sl);
+}
+
+/*
+ * genwig_all_kostelec, wigner.f90:423-465: the same table by the Kostelec recurrence
+ * (stair seeds, then or_genwig per pair), [mnl row][2bw].
+ */
+void or_genwig_all_kostelec(i64 bw, double *out)
+{
+    const i64 N = 2 * bw;
+    double *cosb = malloc(sizeof(double) * N), *sc = malloc(sizeof(double) * N), *c2 = malloc(sizeof(double) * N);
+    for (i64 k = 0; k < N; ++k) {
+        const double b = (double)(2 * k + 1) * (PI / (double)(2 * N));
+        cosb[k] = cos(b);
+        sc[k] = cos(b / 2) * sin(b / 2);
+        c2[k] = pow(cos(b / 2), 2);
+    }
+    double *seeds = malloc(sizeof(double) * ((bw * (bw + 1)) / 2) * N);
+    or_seeds(bw, N, sc, c2, seeds);
+    for (i64 p = 0; p < bw; ++p)
+        for (i64 q = p; q < bw; ++q)
+            or_genwig(p, q, bw, N, cosb, seeds + tri(p, q, bw) * N, out + dtab_row(p, q, bw) * N);
+    free(cosb); free(sc); free(c2); free(seeds);
+}
+
+/*
  * soft.py:713-764 rotate_ylm_cmplx: for every rotation (alpha,beta,gamma) and every
  * coefficient set y (LMc layout, bw^2 entries):
  *   y'^l_n = sum_m y^l_m d^l(-beta)[m][n] e^{-i m alpha} e^{-i n gamma}.

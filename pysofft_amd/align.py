@@ -11,9 +11,15 @@ with g ~ D(R) f:
      grid bandwidth up to 64) around the estimate
      (``so3ft_refine_correlation_peaks``);
   4. fallback: pairs whose refined peak, normalised by ||f|| ||g||, stays below a
-     threshold are aligned by an exhaustive search: the full correlation and peak search of
-     their degrees below bw/2 (a grid of (bw)^3 points, 8x cheaper than the full bandwidth),
-     then the refinement levels (bw, bw) and (2 bw, bw) around that peak.
+     threshold are aligned by the full FP64 correlation at bw and its peak search
+     (``Soft.correlation_peaks``, the method of the batched correlation workload), then the
+     last refinement level around that peak.  ``fallback="coarse"`` selects a cheaper
+     exhaustive search instead: the full correlation of the degrees below bw/2 (a grid of
+     (bw)^3 points, 8x cheaper), then windows at grid bandwidths bw and 2 bw.
+
+``proposer="grid"`` replaces the network by the argmax of the full correlation of the
+degrees below 4 (an 8^3 grid): the same refinement ladder and fallback without the
+learned part, the ablation that measures what the network contributes.
 
 The network is trained on the GPU by the library's own forward/backward/Adam kernels on
 pairs generated with exact Wigner-D rotations of the l <= 8 part (``training_batch``).
@@ -201,12 +207,33 @@ def training_batch(n, gen, bw=64, snr=10.0, device="cuda"):
 class Aligner:
     """Learned proposal + local refinement + full-correlation fallback."""
 
-    def __init__(self, mlp=None, levels=LEVELS, threshold=0.6, device="cuda"):
-        self.mlp = mlp if mlp is not None else AlignmentMLP()
+    def __init__(self, mlp=None, levels=LEVELS, threshold=0.6, device="cuda", fallback="full", proposer="mlp"):
+        if fallback not in ("full", "coarse"):
+            raise ValueError("fallback must be 'full' or 'coarse'")
+        if proposer not in ("mlp", "grid"):
+            raise ValueError("proposer must be 'mlp' or 'grid'")
+        self.mlp = mlp if mlp is not None else (AlignmentMLP() if proposer == "mlp" else None)
         self.levels = tuple(levels)
         self.threshold = threshold
         self.device = device
+        self.fallback = fallback
+        self.proposer = proposer
         self._soft = {}
+
+    def _plan(self, bw):
+        from .soft import Soft
+        if bw not in self._soft:
+            self._soft[bw] = Soft(bw)
+        return self._soft[bw]
+
+    @staticmethod
+    def _peak_euler(pk, bw):
+        """grid indices (i_alpha, i_beta, i_gamma) of a bandwidth-bw grid -> ZYZ angles"""
+        torch = _torch()
+        a = pk[:, 0].to(torch.float64) * (math.pi / bw)
+        b = (2 * pk[:, 1].to(torch.float64) + 1) * (math.pi / (4 * bw))
+        c = pk[:, 2].to(torch.float64) * (math.pi / bw)
+        return torch.stack([a, b, c], 1).contiguous()
 
     # ---- training -----------------------------------------------------------------
     def fit_normalisation(self, n=65536, seed=1, batch=8192):
@@ -240,10 +267,18 @@ class Aligner:
 
     # ---- inference ----------------------------------------------------------------
     def propose(self, f_lms, g_lms):
-        """(n, bw^2) pairs -> proposed ZYZ Euler angles (n, 3) float64 on the device."""
+        """(n, bw^2) pairs -> proposed ZYZ Euler angles (n, 3) float64 on the device: the
+        network's rotation, or with proposer="grid" the peak of the degrees-below-4
+        correlation."""
+        torch = _torch()
+        if self.proposer == "grid":
+            f = f_lms.to(torch.complex128)[:, :16].contiguous()
+            g = g_lms.to(torch.complex128)[:, :16].contiguous()
+            pk, _ = self._plan(4).correlation_peaks(f, g)
+            return self._peak_euler(pk, 4)
         x = features(f_lms, g_lms, self.mlp.mean, self.mlp.std)
         R, _ = self.mlp.forward(x)
-        return _matrix_to_euler_t(R.to(_torch().float64)).contiguous()
+        return _matrix_to_euler_t(R.to(torch.float64)).contiguous()
 
     def refine(self, f_lms, g_lms, euler, levels=None):
         """Window refinement through `levels`; returns (euler (n, 3), |C| of the last level)."""
@@ -264,9 +299,8 @@ class Aligner:
 
     def align(self, f_lms, g_lms):
         """Rotations (ZYZ Euler, (n, 3)) aligning g to f, and the mask of pairs that fell back
-        to the full correlation."""
+        to the exhaustive search."""
         torch = _torch()
-        from .soft import Soft
         f = f_lms.to(torch.complex128).contiguous()
         g = g_lms.to(torch.complex128).contiguous()
         n, nb = f.shape
@@ -276,20 +310,19 @@ class Aligner:
         norm = torch.sqrt((f.abs() ** 2).sum(1) * (g.abs() ** 2).sum(1))
         bad = score / norm < self.threshold
         if bool(bad.any()):
-            fb = max(bw // 2, 2)
-            if fb not in self._soft:
-                self._soft[fb] = Soft(fb)
-            s = self._soft[fb]
             idx = torch.nonzero(bad).flatten()
             fi, gi = f[idx].contiguous(), g[idx].contiguous()
-            pk, _ = s.correlation_peaks(fi[:, :fb * fb].contiguous(), gi[:, :fb * fb].contiguous())
-            N = 2 * fb
-            a = pk[:, 0].to(torch.float64) * (math.pi / fb)
-            b = (2 * pk[:, 1].to(torch.float64) + 1) * (math.pi / (2 * N))
-            c = pk[:, 2].to(torch.float64) * (math.pi / fb)
-            # the coarse peak is within half a step (pi / (2 fb)) of the optimum: a window of
-            # +-2 steps at grid bandwidth bw covers it, then one at 2 bw
-            ef, _ = self.refine(fi, gi, torch.stack([a, b, c], 1), levels=((bw, bw), (2 * bw, bw)))
+            if self.fallback == "full":
+                # the full FP64 correlation at bw and its peak (soft.f08:2242-2420), then
+                # the last refinement level around that grid point
+                pk, _ = self._plan(bw).correlation_peaks(fi, gi)
+                ef, _ = self.refine(fi, gi, self._peak_euler(pk, bw), levels=self.levels[-1:])
+            else:
+                fb = max(bw // 2, 2)
+                pk, _ = self._plan(fb).correlation_peaks(fi[:, :fb * fb].contiguous(), gi[:, :fb * fb].contiguous())
+                # the coarse peak is within half a step (pi / (2 fb)) of the optimum: a window
+                # of +-2 steps at grid bandwidth bw covers it, then one at 2 bw
+                ef, _ = self.refine(fi, gi, self._peak_euler(pk, fb), levels=((bw, bw), (2 * bw, bw)))
             e = e.clone()
             e[idx] = ef
         return e, bad

@@ -68,6 +68,15 @@ def main():
             f += c[i] * (ea[n + bw - 1][:, None, None] * ea[m + bw - 1][None, :, None] * dv[None, None, :])
         out[f"isoft_c_{bw}"] = c
         out[f"isoft_f_{bw}"] = f / (2 * np.pi)
+    # the symmetry-reduced Risbo recurrence (wigner.f90:626-748) through genwig_all_risbo
+    # (wigner.f90:784-822, mnl rows x 2bw betas) and the full Wigner-D of full_big_d_risbo
+    # (wigner.f90:844-934); the Kostelec table (wigner.f90:423-465) for comparison
+    out["genwig_all_risbo_9"] = R.genwig_all_risbo(9, True)
+    out["genwig_all_risbo_raw_5"] = R.genwig_all_risbo(5, False)
+    out["genwig_all_kostelec_9"] = R.genwig_all_kostelec(9)
+    out["full_big_d_angles"] = np.array([0.3, 1.1, 2.5])
+    out["full_big_d_6"] = R.full_big_d_norm(6, 0.3, 1.1, 2.5, True)
+    out["full_big_d_raw_4"] = R.full_big_d_norm(4, 0.3, 1.1, 2.5, False)
     path = os.path.join(os.path.dirname(__file__), "ref_vectors.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes")

@@ -76,6 +76,52 @@ def test_risbo_step_vs_reference_vectors(golden):
     np.testing.assert_array_equal(d, golden["risbo_l11"])
 
 
+def test_risbo_reduced_tables_vs_reference_vectors(golden):
+    """The symmetry-reduced Risbo recurrence (wigner.f90:626-748) through the mnl d table of
+    genwig_all_risbo (wigner.f90:784-822), normalized and not, and the Kostelec table of
+    genwig_all_kostelec (wigner.f90:423-465): bit-equal to the reference build's output."""
+    np.testing.assert_array_equal(O.genwig_all_risbo(9, True), golden["genwig_all_risbo_9"])
+    np.testing.assert_array_equal(O.genwig_all_risbo(5, False), golden["genwig_all_risbo_raw_5"])
+    np.testing.assert_array_equal(O.genwig_all_kostelec(9), golden["genwig_all_kostelec_9"])
+
+
+def _dtab_index(bw):
+    """(p, q, l) of every row of the mnl d table (utils.f90:683-695 order)"""
+    return [(p, q, l) for p in range(bw) for q in range(p, bw) for l in range(q, bw)]
+
+
+def test_risbo_table_vs_jacobi_and_kostelec():
+    """Independent of the reference: every entry of the Risbo table equals the normalised
+    Jacobi-polynomial d~^l_{p,q}(beta_k) (docs/defs/wigner.md:11-14) and the Kostelec table
+    to 1e-13; the unnormalised table is the normalised one / sqrt((2l+1)/2)."""
+    bw = 7
+    t = O.genwig_all_risbo(bw, True)
+    b = O.beta_samples(2 * bw)
+    for i, (p, q, l) in enumerate(_dtab_index(bw)):
+        np.testing.assert_allclose(t[i], brute.wigner_d_norm(l, p, q, b), rtol=0, atol=1e-13)
+    np.testing.assert_allclose(t, O.genwig_all_kostelec(bw), rtol=0, atol=1e-13)
+    raw = O.genwig_all_risbo(bw, False)
+    ls = np.array([l for (_, _, l) in _dtab_index(bw)], float)
+    np.testing.assert_allclose(raw * np.sqrt((2 * ls + 1) / 2)[:, None], t, rtol=0, atol=1e-14)
+
+
+def test_full_big_d_vs_reference_vectors_and_closed_form(golden):
+    """full_big_d_risbo (wigner.f90:844-934): equal to the reference build's vectors, and to
+    the closed form e^{-i m alpha} d^l_{mn}(beta) e^{-i n gamma} (/(2 pi) normalised) with the
+    routine's own convention: a factor -1 on the entries with m n < 0 and m + n odd."""
+    al, be, ga = golden["full_big_d_angles"]
+    x6 = O.full_big_d_risbo(6, al, be, ga, True)
+    np.testing.assert_allclose(x6, golden["full_big_d_6"], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(O.full_big_d_risbo(4, al, be, ga, False), golden["full_big_d_raw_4"], rtol=0,
+                               atol=1e-15)
+    deg = O.coeff_degrees(6)
+    ref = np.array([np.exp(-1j * m * al) * brute.wigner_d_norm(l, m, n, np.array([be]))[0] * np.exp(-1j * n * ga)
+                    for l, m, n in deg]) / (2 * np.pi)
+    flip = (deg[:, 1] * deg[:, 2] < 0) & ((deg[:, 1] + deg[:, 2]) % 2 == 1)
+    ref[flip] *= -1
+    np.testing.assert_allclose(x6, ref, rtol=0, atol=1e-14)
+
+
 @pytest.mark.parametrize("bw", [1, 4, 6])
 def test_enforce_real_sym_vs_reference_vectors(golden, bw):
     c = golden[f"sym_in_{bw}"].copy()
@@ -192,6 +238,27 @@ def test_correlation_argmax_recovers_rotation():
     assert rel_l2(P.cross_correlation_ylm_cmplx(f, g), P.isoft(O.corr_coeffs(bw, f, g))) < 1e-14
     # plant a mistake in the oracle's input: conj(g) instead of g must not pass
     assert rel_l2(P.cross_correlation_ylm_cmplx(f, np.conj(g)), P.isoft(O.corr_coeffs(bw, f, g))) > 1e-3
+
+
+@pytest.mark.skipif(not RefLib.available(), reason="reference build oracle/_ref not present")
+def test_live_reference_build_agrees_risbo():
+    """The reduced Risbo step at arbitrary symmetric beta sets and the tables at more
+    bandwidths, live against the reference build."""
+    rng = np.random.default_rng(5)
+    b = np.sort(rng.uniform(0.05, np.pi / 2 - 0.05, 5))
+    b = np.concatenate([b, np.pi - b[::-1]])
+    c, sn = np.cos(b / 2), np.sin(b / 2)
+    d = dr = None
+    for l in range(14):
+        d = O.risbo_reduced_step(d, l, c, sn, True)
+        dr = RefLib.risbo_reduced_step(dr, l, c, sn, True)
+        np.testing.assert_array_equal(d, dr)
+    for bw in (2, 3, 11, 16):
+        np.testing.assert_array_equal(O.genwig_all_risbo(bw), RefLib.genwig_all_risbo(bw))
+        np.testing.assert_array_equal(O.genwig_all_kostelec(bw), RefLib.genwig_all_kostelec(bw))
+    for bw in (1, 2, 5, 9):
+        np.testing.assert_allclose(O.full_big_d_risbo(bw, 1.9, 0.4, 5.1), RefLib.full_big_d_norm(bw, 1.9, 0.4, 5.1),
+                                   rtol=0, atol=1e-15)
 
 
 @pytest.mark.skipif(not RefLib.available(), reason="reference build oracle/_ref not present")
