@@ -71,7 +71,10 @@ int so3ft_get_fftw_flags(so3ft_handle h, int32_t *flags);
  * vector ALUs (lane = beta) and the forward on v_mfma_f64_16x16x4f64 with lane = beta
  * recurrence chains; SO3FT_DWT_MFMA runs both on v_mfma_f64_16x16x4f64 with 16-degree
  * x 64-beta tiles of d per order pair.  All are FP64 throughout and give the same
- * results to rounding.  Kept across so3ft_reset. */
+ * results to rounding.  Kept across so3ft_reset.  Handles with a d table (made with
+ * precompute_wigners, or with recurrence_type = Risbo: the table is filled once at creation
+ * by the handle's recurrence, soft.f08:194-219) read d from it on the default engine's
+ * kernels whatever the engine; the complex64 entry points compute d on the fly. */
 enum { SO3FT_DWT_VALU = 0, SO3FT_DWT_MFMA = 1 };
 int so3ft_set_dwt_engine(so3ft_handle h, int engine);
 int so3ft_get_dwt_engine(so3ft_handle h, int *engine);
@@ -252,6 +255,33 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
  * reference of the FFT passes).  T transforms per launch. */
 int so3ft_bench_kernel(so3ft_handle h, int which, int reps, void *a, void *b, int64_t T, void *stream,
                        float *ms_per_launch);
+
+/* ---- Wigner-d utilities of the reference's make_wigner module (pysofft/__init__.py:12-13
+ * exports it as `pysofft.wigner`).  Arrays are DEVICE float64, betas contiguous.
+ * so3ft_wigner_mn_recurrence_risbo: wigner.f90:529-585, one step l-1 -> l of Risbo's
+ * recurrence for the full small-d matrix at nbeta angles: d_prev (nbeta, (2l-1)^2) ->
+ * d_out (nbeta, (2l+1)^2), rows [i][j] row-major per beta; cos_b, sin_b: cos/sin of the
+ * half angles (the reference's rotate_ylm passes cos(b/2), sin(-b/2)); sqrts[k] = sqrt(k),
+ * nsqrt >= max(3, 2l+1).  d_prev is not read for l < 2. */
+int so3ft_wigner_mn_recurrence_risbo(const void *d_prev, int64_t l, const void *cos_b, const void *sin_b,
+                                     const void *sqrts, int64_t nsqrt, int64_t nbeta, void *d_out, void *stream);
+/* wigner.f90:626-748 wigner_recurrence_risbo_reduced: one step l-1 -> l of the symmetry-reduced
+ * recurrence, d^l_{m,n} for 0 <= m <= n <= l at nbeta betas that are closed under
+ * beta -> pi - beta in reversed order (the step reads pi - beta as the reversed array):
+ * dl_prev (l(l+1)/2, nbeta) -> dl_out ((l+1)(l+2)/2, nbeta), rows tri(m, n) = m(2s-m+1)/2 + n-m
+ * on the triangle of side s; normalized: d~ = sqrt((2l+1)/2) d; sqrts only read at l = 1. */
+int so3ft_wigner_recurrence_risbo_reduced(const void *dl_prev, int64_t l, const void *cos_b, const void *sin_b,
+                                          const void *sqrts, int64_t nsqrt, int64_t nbeta, int normalized,
+                                          void *dl_out, void *stream);
+/* wigner.f90:784-822 genwig_all_risbo (recurrence_type 1) / :423-465 genwig_all_kostelec (0):
+ * every normalised d~^l_{m,n}(beta_k), 0 <= m <= n <= l < bw, at the 2bw beta samples, into out
+ * (bw(2 + 3bw + bw^2)/6 rows in mnl order, utils.f90:683-695, x 2bw) */
+int so3ft_genwig_all(int64_t bw, int64_t recurrence_type, void *out, void *stream);
+/* wigner.f90:844-934 full_big_d_risbo: D^l_{mn}(alpha, beta, gamma) for every (l, m, n) in mnl
+ * order into out (complex128, NC(bw)), with that routine's signs; normalized: d~ and / (2 pi).
+ * Synchronises the stream. */
+int so3ft_full_big_d_risbo(int64_t bw, double alpha, double beta, double gamma, int normalized, void *out,
+                           void *stream);
 
 /* ---- utils: utils.f90:471-505 enforce_real_sym_mnl(_many), in place on device */
 int so3ft_enforce_real_sym_mnl(int64_t bw, void *coeffs, int64_t n, void *stream);

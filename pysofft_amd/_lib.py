@@ -58,6 +58,10 @@ SIGNATURES = {
     "so3ft_cross_correlation_ylm_real_3d": (_int, [_i64, _vp, _vp, _vp, _P(_dbl), _i64, _i64, _i64, _vp]),
     "so3ft_inverse_wigner_trf_corr_cmplx": (_int, [_i64, _vp, _vp, _vp, _vp]),
     "so3ft_inverse_wigner_trf_corr_real": (_int, [_i64, _vp, _vp, _vp, _vp]),
+    "so3ft_wigner_mn_recurrence_risbo": (_int, [_vp, _i64, _vp, _vp, _vp, _i64, _i64, _vp, _vp]),
+    "so3ft_wigner_recurrence_risbo_reduced": (_int, [_vp, _i64, _vp, _vp, _vp, _i64, _i64, _int, _vp, _vp]),
+    "so3ft_genwig_all": (_int, [_i64, _i64, _vp, _vp]),
+    "so3ft_full_big_d_risbo": (_int, [_i64, _dbl, _dbl, _dbl, _int, _vp, _vp]),
     "so3ft_enforce_real_sym_mnl": (_int, [_i64, _vp, _i64, _vp]),
     "so3ft_enforce_real_sym_lmn": (_int, [_i64, _vp, _i64, _vp]),
     "so3ft_rotate_ylm_cmplx": (_int, [_i64, _vp, _i64, _vp, _i64, _int, _vp, _vp]),

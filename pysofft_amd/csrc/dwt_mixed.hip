@@ -650,13 +650,13 @@ __device__ __forceinline__ d4_t mma64(double a, double b, d4_t c)
 
 constexpr int F64_DS = 66;   // padded row of the FP64 A tile (doubles)
 
-template <int TB>
+template <int TB, bool TAB>
 __device__ __forceinline__ void fwd_m64_item(const double2 *__restrict__ F, int64_t f_ts,
                                              double2 *__restrict__ coeff, int64_t c_ts, int T,
                                              const double *__restrict__ seeds, const double *__restrict__ rc,
                                              const int64_t *__restrict__ rc_off, const int64_t *__restrict__ slice,
                                              int bw, int N, int p, int q, int t0, int real, double (*Dt)[F64_DS],
-                                             double2 (*rs)[64])
+                                             double2 (*rs)[64], const double *__restrict__ dtab)
 {
     static_assert(TB == 1 || TB == 2, "one transform pair per wave");
     const int lane = threadIdx.x & 63;
@@ -706,14 +706,22 @@ __device__ __forceinline__ void fwd_m64_item(const double2 *__restrict__ F, int6
             const double2 cw = rs[nc + c][lane];
             const double cb = cw.x, wt = cw.y;
             asm volatile("" ::: "memory");
-            double2 st = rs[c][lane];
-            double dcur = st.x, dprev = st.y;
+            if (TAB) {   // d~ from the table rows of this pair (zero past the last degree / beta)
+                const int beta = 64 * c + lane;
+                const double *tr = dtab + dtab_row(p, q, bw) * N + (beta < N ? beta : 0);
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {   // the chain is zero past the last degree
-                Dt[k][lane] = wt * dcur;
-                rec_step(rcp, l0 + k, cb, dcur, dprev);
+                for (int k = 0; k < 16; ++k)
+                    Dt[k][lane] = (l0 + k < L && beta < N) ? wt * tr[(int64_t)(l0 + k) * N] : 0.0;
+            } else {
+                double2 st = rs[c][lane];
+                double dcur = st.x, dprev = st.y;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {   // the chain is zero past the last degree
+                    Dt[k][lane] = wt * dcur;
+                    rec_step(rcp, l0 + k, cb, dcur, dprev);
+                }
+                rs[c][lane] = make_double2(dcur, dprev);
             }
-            rs[c][lane] = make_double2(dcur, dprev);
             asm volatile("" ::: "memory");
 #pragma unroll
             for (int s4 = 0; s4 < 16; ++s4) {
@@ -742,7 +750,7 @@ __device__ __forceinline__ void fwd_m64_item(const double2 *__restrict__ F, int6
     }
 }
 
-template <int TB>
+template <int TB, bool TAB>
 __global__ __launch_bounds__(64) void dwt_forward_m64(const double2 *__restrict__ F, int64_t f_ts,
                                                       double2 *__restrict__ coeff, int64_t c_ts, int T,
                                                       const double *__restrict__ w, const double *__restrict__ cosb,
@@ -751,7 +759,7 @@ __global__ __launch_bounds__(64) void dwt_forward_m64(const double2 *__restrict_
                                                       const int64_t *__restrict__ rc_off,
                                                       const int64_t *__restrict__ slice,
                                                       const int32_t *__restrict__ pairs, int bw, int N, int npair,
-                                                      int nunit, int real)
+                                                      int nunit, int real, const double *__restrict__ dtab)
 {
     // dynamic LDS: Dt [16][F64_DS] double | rs [2 nc][64] double2 (chain states, then (cos beta, w))
     extern __shared__ double smx[];
@@ -761,8 +769,8 @@ __global__ __launch_bounds__(64) void dwt_forward_m64(const double2 *__restrict_
     if (unit >= nunit) return;
     const int pq = pairs[unit % npair];
     fwd_stage_sampling(rs, cosb, w, N);
-    fwd_m64_item<TB>(F, f_ts, coeff, c_ts, T, seeds, rc, rc_off, slice, bw, N, pq & 0xffff, pq >> 16,
-                     (unit / npair) * TB, real, Dt, rs);
+    fwd_m64_item<TB, TAB>(F, f_ts, coeff, c_ts, T, seeds, rc, rc_off, slice, bw, N, pq & 0xffff, pq >> 16,
+                          (unit / npair) * TB, real, Dt, rs, dtab);
 }
 
 // FP64 forward, one block of nc = N / 64 waves per (pair, 2 transforms): wave c owns beta
@@ -773,12 +781,12 @@ __global__ __launch_bounds__(64) void dwt_forward_m64(const double2 *__restrict_
 // to its own LDS tile, runs its 16 K-steps, and leaves its chunk's partial sums in that
 // tile; after a barrier the block adds the partials in chunk order and writes the
 // tile's coefficients.
-template <int TB>
+template <int TB, bool TAB>
 __global__ __launch_bounds__(512) void dwt_forward_m64w(
     const double2 *__restrict__ F, int64_t f_ts, double2 *__restrict__ coeff, int64_t c_ts, int T,
     const double *__restrict__ w, const double *__restrict__ cosb, const double *__restrict__ seeds,
     const double *__restrict__ rc, const int64_t *__restrict__ rc_off, const int64_t *__restrict__ slice,
-    const int32_t *__restrict__ pairs, int bw, int N, int npair, int nunit, int real)
+    const int32_t *__restrict__ pairs, int bw, int N, int npair, int nunit, int real, const double *__restrict__ dtab)
 {
     static_assert(TB == 1 || TB == 2, "one transform pair per wave");
     constexpr int NV = 2 * 4;   // partials per lane: (re, im) x result item
@@ -813,7 +821,8 @@ __global__ __launch_bounds__(512) void dwt_forward_m64w(
     const int beta = 64 * wv + lane;
     const bool ok = beta < N;
     const double cb = ok ? ((gptr<double>)cosb)[beta] : 0.0, wt = ok ? ((gptr<double>)w)[beta] : 0.0;
-    double dcur = ok ? ((gptr<double>)seeds)[tri * N + beta] : 0.0, dprev = 0.0;
+    double dcur = (!TAB && ok) ? ((gptr<double>)seeds)[tri * N + beta] : 0.0, dprev = 0.0;
+    const double *trow = TAB ? dtab + dtab_row(p, q, bw) * N + (ok ? beta : 0) : nullptr;
     double2 bv[16];   // K-step s: the sample at beta 64 c + 4 s + k (clamped; zero A rows past N)
 #pragma unroll
     for (int s4 = 0; s4 < 16; ++s4) {
@@ -828,10 +837,16 @@ __global__ __launch_bounds__(512) void dwt_forward_m64w(
 
 #pragma unroll 1
     for (int l0 = 0; l0 < L; l0 += 16) {
+        if (TAB) {
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {   // the chain is zero past the last degree
-            Dt[k][lane] = wt * dcur;
-            rec_step(rcp, l0 + k, cb, dcur, dprev);
+            for (int k = 0; k < 16; ++k)
+                Dt[k][lane] = (l0 + k < L && ok) ? wt * trow[(int64_t)(l0 + k) * N] : 0.0;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {   // the chain is zero past the last degree
+                Dt[k][lane] = wt * dcur;
+                rec_step(rcp, l0 + k, cb, dcur, dprev);
+            }
         }
         d4_t are[2], aim[2];
 #pragma unroll
@@ -921,7 +936,25 @@ hipError_t launch_dwt_forward_mixed(const DeviceTables &tb, int bw, int N, int n
     return hipGetLastError();
 }
 
-// FP64 forward on FP64 MFMA (modes DWT_COMPLEX and DWT_REAL)
+// FP64 forward on FP64 MFMA (modes DWT_COMPLEX and DWT_REAL); with the plan's d table
+// (tb.dtab) the A tiles are read from it instead of the recurrence
+template <int TB, bool TAB>
+static void launch_m64(const DeviceTables &tb, int bw, int N, int npair, const double2 *F, int64_t f_ts,
+                       double2 *coeff, int64_t c_ts, int T, int nunit, int real, hipStream_t s)
+{
+    const int nc = (N + 63) / 64;
+    if (nc >= 2 && nc <= 8) {   // one wave per 64-beta chunk: fft-array samples read once
+        const size_t ldsw = sizeof(double) * (size_t)nc * 16 * F64_DS;
+        hipLaunchKernelGGL((dwt_forward_m64w<TB, TAB>), dim3(nunit), dim3(64 * nc), ldsw, s, F, f_ts, coeff, c_ts, T,
+                           tb.w, tb.cosb, tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit, real,
+                           tb.dtab);
+        return;
+    }
+    const size_t lds = sizeof(double) * (16 * F64_DS + 4 * 64 * ((N + 63) / 64));
+    hipLaunchKernelGGL((dwt_forward_m64<TB, TAB>), dim3(nunit), dim3(64), lds, s, F, f_ts, coeff, c_ts, T, tb.w,
+                       tb.cosb, tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit, real, tb.dtab);
+}
+
 hipError_t launch_dwt_forward_m64(const DeviceTables &tb, int bw, int N, int npair, int mode, const double2 *F,
                                   int64_t f_ts, double2 *coeff, int64_t c_ts, int T, hipStream_t s)
 {
@@ -930,24 +963,13 @@ hipError_t launch_dwt_forward_m64(const DeviceTables &tb, int bw, int N, int npa
     if (npair <= 0 || T <= 0) return hipSuccess;
     const int TB = T >= 2 ? 2 : 1;
     const int nunit = npair * ((T + TB - 1) / TB);
-    const int nc = (N + 63) / 64;
-    if (nc >= 2 && nc <= 8) {   // one wave per 64-beta chunk: fft-array samples read once
-        const size_t ldsw = sizeof(double) * (size_t)nc * 16 * F64_DS;
-        if (TB == 2)
-            hipLaunchKernelGGL((dwt_forward_m64w<2>), dim3(nunit), dim3(64 * nc), ldsw, s, F, f_ts, coeff, c_ts, T,
-                               tb.w, tb.cosb, tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit, real);
-        else
-            hipLaunchKernelGGL((dwt_forward_m64w<1>), dim3(nunit), dim3(64 * nc), ldsw, s, F, f_ts, coeff, c_ts, T,
-                               tb.w, tb.cosb, tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit, real);
-        return hipGetLastError();
+    if (tb.dtab) {
+        if (TB == 2) launch_m64<2, true>(tb, bw, N, npair, F, f_ts, coeff, c_ts, T, nunit, real, s);
+        else launch_m64<1, true>(tb, bw, N, npair, F, f_ts, coeff, c_ts, T, nunit, real, s);
+    } else {
+        if (TB == 2) launch_m64<2, false>(tb, bw, N, npair, F, f_ts, coeff, c_ts, T, nunit, real, s);
+        else launch_m64<1, false>(tb, bw, N, npair, F, f_ts, coeff, c_ts, T, nunit, real, s);
     }
-    const size_t lds = sizeof(double) * (16 * F64_DS + 4 * 64 * ((N + 63) / 64));
-    if (TB == 2)
-        hipLaunchKernelGGL((dwt_forward_m64<2>), dim3(nunit), dim3(64), lds, s, F, f_ts, coeff, c_ts, T, tb.w, tb.cosb,
-                           tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit, real);
-    else
-        hipLaunchKernelGGL((dwt_forward_m64<1>), dim3(nunit), dim3(64), lds, s, F, f_ts, coeff, c_ts, T, tb.w, tb.cosb,
-                           tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit, real);
     return hipGetLastError();
 }
 

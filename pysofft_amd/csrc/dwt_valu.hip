@@ -132,15 +132,17 @@ __device__ __forceinline__ bool decode_unit(int nch, int nunit, int &unit, int &
 }
 
 // Inverse.  Block = one wave, lanes = 64 betas of one chunk; TB transforms per wave.
-// Work unit = (pair, transform group), pairs fastest.
-template <int TB>
+// Work unit = (pair, transform group), pairs fastest.  TAB: d~ is read from the plan's
+// d table (rows of the pair's degrees, the lane's beta) instead of the recurrence.
+template <int TB, bool TAB>
 __global__ __launch_bounds__(64) void dwt_inverse_valu(const double2 *__restrict__ coeff, int64_t c_ts,
                                                        double2 *__restrict__ F, int64_t f_ts, int T,
                                                        const double *__restrict__ cosb,
                                                        const double *__restrict__ seeds,
                                                        const int64_t *__restrict__ slice,
                                                        const int32_t *__restrict__ pairs, int bw, int N, int mode,
-                                                       int npair, int nch, int nunit)
+                                                       int npair, int nch, int nunit,
+                                                       const double *__restrict__ dtab)
 {
     // recurrence constants; the last step of a tail reads up to entry 3 L + 2 <= 770
     __shared__ double cst[3 * 258];
@@ -170,16 +172,26 @@ __global__ __launch_bounds__(64) void dwt_inverse_valu(const double2 *__restrict
         for (int t = 0; t < TB; ++t) dst[t] = src[t][j];
     };
 
-    for (int i = lane; i < L - 1; i += 64) {
-        double a, x, b;
-        rec_consts_v(q + i, p, q, a, x, b);
-        cst[3 * i] = a; cst[3 * i + 1] = x; cst[3 * i + 2] = b;
-    }
+    if (!TAB)
+        for (int i = lane; i < L - 1; i += 64) {
+            double a, x, b;
+            rec_consts_v(q + i, p, q, a, x, b);
+            cst[3 * i] = a; cst[3 * i + 1] = x; cst[3 * i + 2] = b;
+        }
     const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
-    double dcur = (beta < N) ? seeds[tri * N + beta] : 0.0;
-    const double cb = (beta < N) ? cosb[beta] : 0.0;
+    double dcur = (!TAB && beta < N) ? seeds[tri * N + beta] : 0.0;
+    const double cb = (!TAB && beta < N) ? cosb[beta] : 0.0;
     double dprev = 0.0;
     const double sq = (q & 1) ? -1.0 : 1.0;  // (-1)^l for l = q + j, j even
+    // table rows of this pair at the lane's beta (lanes past N read row entry 0 and use 0)
+    const double *trow = TAB ? dtab + dtab_row(p, q, bw) * N + (beta < N ? beta : 0) : nullptr;
+    const double tsc = beta < N ? 1.0 : 0.0;
+    double dring[3][2];   // table values of the three buffered steps
+    auto load_d = [&](double (&dd)[2], int j0) {
+        if (!TAB) return;
+        dd[0] = j0 < L ? tsc * trow[(int64_t)j0 * N] : 0.0;
+        dd[1] = j0 + 1 < L ? tsc * trow[(int64_t)(j0 + 1) * N] : 0.0;
+    };
 
     double acc[TB][16];
 #pragma unroll
@@ -191,12 +203,22 @@ __global__ __launch_bounds__(64) void dwt_inverse_valu(const double2 *__restrict
     load(buf[0], 0);
     load(buf[1], 2);
     load(buf[2], 4);
+    load_d(dring[0], 0);
+    load_d(dring[1], 2);
+    load_d(dring[2], 4);
     __syncthreads();
 
     // one step = degrees j0, j0+1 from buf[u]; the recurrence always advances twice
     // (past the last degree it reads in-bounds LDS beyond the constants and the
     // result is never used)
     auto step = [&](int u, int j0, bool tail) {
+        if (TAB) {
+            const double d0 = dring[u][0], d1 = dring[u][1];
+            const double da0 = sq * d0, da1 = -sq * d1;
+#pragma unroll
+            for (int t = 0; t < TB; ++t) fma_block<true>(acc[t], buf[u][t], d0, d1, da0, da1);
+            return;
+        }
         const double d0 = dcur;
         {
             const double ca = cst[3 * j0], cx = cst[3 * j0 + 1], cbb = cst[3 * j0 + 2];
@@ -219,10 +241,13 @@ __global__ __launch_bounds__(64) void dwt_inverse_valu(const double2 *__restrict
     for (; j6 + 6 <= L; j6 += 6) {
         step(0, j6, false);
         load(buf[0], j6 + 6);
+        load_d(dring[0], j6 + 6);
         step(1, j6 + 2, false);
         load(buf[1], j6 + 8);
+        load_d(dring[1], j6 + 8);
         step(2, j6 + 4, false);
         load(buf[2], j6 + 10);
+        load_d(dring[2], j6 + 10);
     }
     if (j6 < L) step(0, j6, true);
     if (j6 + 2 < L) step(1, j6 + 2, true);
@@ -479,8 +504,14 @@ hipError_t launch_dwt_inverse_valu(const DeviceTables &tb, int bw, int N, int np
 #define SO3_INV(TBV)                                                                                              \
     do {                                                                                                          \
         const int nunit = npair * ((T + TBV - 1) / TBV);                                                          \
-        hipLaunchKernelGGL((dwt_inverse_valu<TBV>), dim3(grid_of(nunit, nch)), dim3(64), 0, s, coeff, c_ts, F,   \
-                           f_ts, T, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, mode, npair, nch, nunit);       \
+        if (tb.dtab)                                                                                              \
+            hipLaunchKernelGGL((dwt_inverse_valu<TBV, true>), dim3(grid_of(nunit, nch)), dim3(64), 0, s, coeff,  \
+                               c_ts, F, f_ts, T, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, mode, npair, nch,  \
+                               nunit, tb.dtab);                                                                   \
+        else                                                                                                      \
+            hipLaunchKernelGGL((dwt_inverse_valu<TBV, false>), dim3(grid_of(nunit, nch)), dim3(64), 0, s, coeff, \
+                               c_ts, F, f_ts, T, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, mode, npair, nch,  \
+                               nunit, nullptr);                                                                   \
     } while (0)
     switch (TBv) {
     case 1: SO3_INV(1); break;

@@ -75,7 +75,7 @@ Plan *get(so3ft_handle h)
 void free_plan_device(Plan *P)
 {
     hipFree(P->tb.cosb); hipFree(P->tb.w); hipFree(P->tb.seeds); hipFree(P->tb.slice);
-    hipFree(P->tb.pairs); hipFree(P->tb.tw); hipFree(P->tb.rc); hipFree(P->tb.rc_off);
+    hipFree(P->tb.pairs); hipFree(P->tb.tw); hipFree(P->tb.rc); hipFree(P->tb.rc_off); hipFree(P->tb.dtab);
     hipFree(P->scratch); hipFree(P->red); hipFree(P->cstage);
     P->tb = DeviceTables();
     P->scratch = nullptr; P->scratch_T = 0; P->red = nullptr;
@@ -280,6 +280,34 @@ int build_plan(Plan *P, int64_t bw, int64_t lmax, int precompute, int init_ffts,
     }
     HIPCHK(hipMalloc((void **)&P->red, sizeof(double) * (2 * 1024 + 2)));
     HIPCHK(dwt_set_attributes((int)bw, (int)N));
+    // The d table: precompute_wigners (soft.f08:194-219) fills it once with the handle's
+    // recurrence (genwig_all_kostelec / genwig_all_risbo); a Risbo handle without it, whose
+    // reference transforms rerun the Risbo recurrence on every call (soft.f08:676-824),
+    // fills it the same way at creation (the values depend only on bw).  The FP64 DWTs then
+    // read d from the table.
+    if (precompute || rt == 1) {
+        const size_t bytes = sizeof(double) * (size_t)dtab_entries((int)bw);
+        if (hipMalloc((void **)&P->tb.dtab, bytes) != hipSuccess) {
+            hipGetLastError();
+            P->tb.dtab = nullptr;
+            return fail(SO3FT_ENOMEM, "Wigner d table allocation of " + std::to_string(bytes) + " bytes failed");
+        }
+        if (rt == 0) {
+            HIPCHK(fill_dtab_kostelec(P->tb, (int)bw, P->tb.dtab, nullptr));
+        } else {
+            double *work = nullptr;
+            if (hipMalloc((void **)&work, sizeof(double) * (size_t)dtab_work_entries((int)bw)) != hipSuccess) {
+                hipGetLastError();
+                return fail(SO3FT_ENOMEM, "Risbo recurrence scratch allocation failed");
+            }
+            const hipError_t e = fill_dtab_risbo((int)bw, P->tb.dtab, work, nullptr);
+            const hipError_t es = hipStreamSynchronize(nullptr);
+            hipFree(work);
+            HIPCHK(e);
+            HIPCHK(es);
+        }
+        HIPCHK(hipStreamSynchronize(nullptr));
+    }
     return SO3FT_OK;
 }
 
@@ -384,13 +412,14 @@ int dwt_inverse(Plan *P, int mode, const double2 *c, int64_t c_ts, const double2
     const int bw = (int)P->bw, N = (int)P->N, np = (int)P->npair;
     // the MFMA engine forms correlation coefficients in its kernel; the other paths (and
     // handles that drop degrees above lmax) stage them first
-    if (mode == DWT_CORR && (P->dwt_engine != SO3FT_DWT_MFMA || P->lcap < P->bw - 1)) {
+    const bool mfma = P->dwt_engine == SO3FT_DWT_MFMA && !P->tb.dtab;   // table plans read d on the VALU path
+    if (mode == DWT_CORR && (!mfma || P->lcap < P->bw - 1)) {
         if (int rc = ensure_cstage(P, T)) return rc;
         HIPCHK(launch_corr_coeffs(P->tb, bw, c, aux, c_ts, 0, 1, 0, nullptr, P->cstage, P->nc, (int)P->lcap, (int)T,
                                   s));
         c = P->cstage; c_ts = P->nc; mode = DWT_COMPLEX;
     }
-    if (P->dwt_engine == SO3FT_DWT_MFMA) {
+    if (mfma) {
         HIPCHK(launch_dwt_inverse(P->tb, bw, N, np, mode, c, c_ts, aux, F, f_ts, (int)T, s));
         return SO3FT_OK;
     }
@@ -401,7 +430,7 @@ int dwt_inverse(Plan *P, int mode, const double2 *c, int64_t c_ts, const double2
 int dwt_forward(Plan *P, int mode, const double2 *F, int64_t f_ts, double2 *c, int64_t c_ts, int64_t T, hipStream_t s)
 {
     const int bw = (int)P->bw, N = (int)P->N, np = (int)P->npair;
-    if (P->dwt_engine == SO3FT_DWT_MFMA)
+    if (P->dwt_engine == SO3FT_DWT_MFMA && !P->tb.dtab)
         HIPCHK(launch_dwt_forward(P->tb, bw, N, np, mode, F, f_ts, c, c_ts, (int)T, s));
     else if (mode == DWT_COMPLEX || mode == DWT_REAL)   // lane-beta FP64 MFMA forward (FP64 throughout)
         HIPCHK(launch_dwt_forward_m64(P->tb, bw, N, np, mode, F, f_ts, c, c_ts, (int)T, s));

@@ -58,7 +58,28 @@ struct DeviceTables {
     double2 *tw = nullptr;     // [N]    exp(-2 pi i k / N)
     double *rc = nullptr;      // [sum over pairs of (bw - q - 1 + REC_PAD)][3] recurrence constants (a, x, b)
     int64_t *rc_off = nullptr; // [ntri]  first entry of pair tri(p, q) in rc
+    double *dtab = nullptr;    // [mnl row][N] d~^l_{p,q}(beta_k), 0 <= p <= q <= l < bw (null: on the fly)
 };
+
+// first row of pair (p, q) in the d table: rows in mnl order, pair-major then degree
+// (pysofft/utils.f90:683-695, zero-based); row of (p, q, l) = dtab_row(p, q) + l - q
+__host__ __device__ inline int64_t dtab_row(int64_t p, int64_t q, int64_t bw)
+{
+    return (p * (3 * bw * bw + p * p + 2 - 3 * bw * (p - 2) - 3 * p)) / 6 + ((q - p) * (2 * bw + 1 - p - q)) / 2;
+}
+// doubles of a plan's d table, and of the scratch its Risbo fill needs
+int64_t dtab_entries(int bw);
+int64_t dtab_work_entries(int bw);
+// the d table by the reduced Risbo recurrence (wigner.f90:784-822) / by the Kostelec
+// recurrence from the plan's seeds and constants (wigner.f90:423-465)
+hipError_t fill_dtab_risbo(int bw, double *tab, double *work, hipStream_t s);
+hipError_t fill_dtab_kostelec(const DeviceTables &tb, int bw, double *tab, hipStream_t s);
+// one step l-1 -> l of the reduced Risbo recurrence at nb betas (closed under beta -> pi -
+// beta, reversed): d1 [l(l+1)/2][nb] -> d [(l+1)(l+2)/2][nb], h scratch [l(l+1)/2][nb];
+// c, s: half-angle cos / sin; sq2 = sqrt(2) as the caller's table holds it; tab (nb = 2bw):
+// also write degree l into a plan table
+hipError_t launch_risbo_reduced_step(const double *d1, double *h, double *d, int l, int nb, const double *c,
+                                     const double *s, double sq2, int normalized, double *tab, int bw, hipStream_t st);
 
 // recurrence constants of pair (p, q) for the mixed-precision kernels: entry i = (a, x, b)
 // of the step from degree q + i to q + i + 1, d^{l+1} = a (cos(beta) - x) d^l - b d^{l-1},
