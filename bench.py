@@ -190,6 +190,46 @@ def ragged_batch(args, dev, timed):
             "tuned_config": cfg}
 
 
+def precomputed_wigners(args, dev, timed):
+    """FP64 isoft_many + soft_many of 16 transforms with d read from the plan's precomputed
+    table (precompute_wigners=True, soft.f08:194-219) against d computed on the fly, at
+    bw = 64 and 128; the table's size and build time (plan creation)."""
+    import torch
+    from pysofft_amd import Soft
+    out = {}
+    for bw in (64, 128):
+        B = 16
+        N = 2 * bw
+        g = torch.Generator(device="cuda").manual_seed(bw)
+        c = torch.view_as_complex(torch.randn(B, nc(bw), 2, generator=g, dtype=torch.float64, device=dev))
+        f = torch.empty((B, N, N, N), dtype=torch.complex128, device=dev)
+        r = torch.empty_like(c)
+        res = {}
+        for name, pre in (("on_the_fly", False), ("table", True)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            s = Soft(bw, precompute_wigners=pre)
+            torch.cuda.synchronize()
+            t_init = time.perf_counter() - t0
+
+            def step():
+                s.isoft_many(c, out=f)
+                s.soft_many(f, out=r)
+
+            step()
+            k = 5
+            ms, _ = timed(step, k)
+            res[name] = {"ms_per_step": ms / k, "plan_init_s": t_init}
+            del s
+        rows = bw * (2 + 3 * bw + bw * bw) // 6
+        res["table_bytes"] = 8 * rows * N
+        res["table_over_on_the_fly"] = res["on_the_fly"]["ms_per_step"] / res["table"]["ms_per_step"]
+        out[str(bw)] = res
+        del c, f, r
+    return {"workload": "FP64 isoft_many + soft_many, 16 complex transforms, d from the precomputed table vs on "
+                        "the fly (Kostelec recurrence)", "by_bw": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,6 +247,7 @@ def main():
     ap.add_argument("--train-pairs", type=int, default=8_192_000)
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-bandwidth grouped batch")
     ap.add_argument("--ragged-n", type=int, default=256)
+    ap.add_argument("--no-precompute", action="store_true", help="skip the precomputed-Wigner comparison")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -339,6 +380,10 @@ def main():
         roof["share_of_step"] = per_step[dom] / sum(per_step.values())
     del f64, r64
 
+    precomp = None
+    if not args.no_precompute:
+        precomp = precomputed_wigners(args, dev, timed)
+
     ragged = None
     if not args.no_ragged:
         ragged = ragged_batch(args, dev, timed)
@@ -400,7 +445,8 @@ def main():
                    "bw": bw, "batch": B,
                    "relL2_vs_fp64_max": {"isoft": err_isoft, "soft": err_soft, "round_trip": err_rt},
                    "fp64_pairs_per_s": fp64_value, "mixed_over_fp64": value / fp64_value, "wall_s": wall},
-        "roofline": roof, "kernels": kernels, "ragged_batch": ragged, "alignment": align, "cpu_baseline": cpu,
+        "roofline": roof, "kernels": kernels, "ragged_batch": ragged, "alignment": align,
+        "precomputed_wigners": precomp, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

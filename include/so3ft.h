@@ -95,8 +95,9 @@ int so3ft_rsoft_many(so3ft_handle h, const void *so3funcs_real, void *coeffs, in
  * only): complex64 coefficients (n, NC) and complex64 SO(3) functions (n, 2bw, 2bw,
  * 2bw); the Wigner-d recurrence, quadrature weights and FFT butterflies run in FP64,
  * the DWT contraction on the matrix cores with d split into FP32 hi + lo parts (hi
- * products in FP32, lo products in bfloat16) and FP64 accumulation across 16-degree blocks.  Needs bw = 2^a or 3*2^a (else
- * SO3FT_EUNSUPPORTED).  Same semantics as so3ft_isoft_many / so3ft_soft_many. */
+ * products in FP32, lo products in bfloat16) and FP64 accumulation across 16-degree blocks.  Any bw (FFT
+ * lengths 2^a or 3*2^a run the register FFT, others an LDS Stockham pass).  Same semantics as
+ * so3ft_isoft_many / so3ft_soft_many. */
 int so3ft_isoft_many_c64(so3ft_handle h, const void *coeffs, void *so3funcs, int64_t n, void *stream);
 int so3ft_soft_many_c64(so3ft_handle h, const void *so3funcs, void *coeffs, int64_t n, void *stream);
 
@@ -113,7 +114,7 @@ int so3ft_soft_many_c64_noalloc(so3ft_handle h, const void *so3funcs, void *coef
 
 /* ---- grouped mixed-precision transforms of uneven bandwidths (no counterpart in the
  * reference, whose *_many take one bandwidth per instance).  A group fixes n bandwidths
- * (2 <= bw <= 256, bw = 2^a or 3*2^a); its transforms run as one persistent DWT launch
+ * (2 <= bw <= 256); its transforms run as one persistent DWT launch
  * over the ragged work of all bandwidths plus per-bandwidth FFT passes (parallel graph
  * branches), captured into a hipGraph per (input, output) pointer pair on first use and
  * replayed after.  Arrays are PACKED: transform i's complex64 coefficients (NC(bw_i),

@@ -546,6 +546,7 @@ static hipError_t fwd_typed(const DeviceTables &tb, int bw, int N, int npair, in
 hipError_t launch_dwt_inverse(const DeviceTables &tb, int bw, int N, int npair, int mode, const double2 *coeff,
                               int64_t c_ts, const double2 *aux, double2 *F, int64_t f_ts, int T, hipStream_t s)
 {
+    if (bw > 256) return hipErrorInvalidValue;   // the recurrence constants' LDS holds bw <= 256
     if (npair <= 0 || T <= 0) return hipSuccess;
     return inv_typed<0, double2, double2>(tb, bw, N, npair, mode, coeff, c_ts, aux, F, f_ts, T, s);
 }
@@ -553,6 +554,7 @@ hipError_t launch_dwt_inverse(const DeviceTables &tb, int bw, int N, int npair, 
 hipError_t launch_dwt_forward(const DeviceTables &tb, int bw, int N, int npair, int mode, const double2 *F,
                               int64_t f_ts, double2 *coeff, int64_t c_ts, int T, hipStream_t s)
 {
+    if (bw > 256) return hipErrorInvalidValue;
     if (npair <= 0 || T <= 0) return hipSuccess;
     return fwd_typed<0, double2, double2>(tb, bw, N, npair, mode, F, f_ts, coeff, c_ts, T, s);
 }

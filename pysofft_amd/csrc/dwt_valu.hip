@@ -144,8 +144,9 @@ __global__ __launch_bounds__(64) void dwt_inverse_valu(const double2 *__restrict
                                                        int npair, int nch, int nunit,
                                                        const double *__restrict__ dtab)
 {
-    // recurrence constants; the last step of a tail reads up to entry 3 L + 2 <= 770
-    __shared__ double cst[3 * 258];
+    // recurrence constants (dynamic LDS, 3 (bw + 2) doubles); the last step of a tail
+    // reads up to entry 3 L + 2
+    extern __shared__ double cst[];
     int unit, chunk;
     if (!decode_unit(nch, nunit, unit, chunk)) return;
     const int lane = threadIdx.x;
@@ -501,6 +502,7 @@ hipError_t launch_dwt_inverse_valu(const DeviceTables &tb, int bw, int N, int np
     int TBv = T >= 4 ? 4 : (T >= 2 ? 2 : 1);
     if (valu_tb_env()) TBv = valu_tb_env();
     const int nch = (N + 63) / 64;
+    const size_t cst_bytes = sizeof(double) * 3 * (size_t)(bw + 2);
 #define SO3_INV(TBV)                                                                                              \
     do {                                                                                                          \
         const int nunit = npair * ((T + TBV - 1) / TBV);                                                          \
@@ -509,9 +511,9 @@ hipError_t launch_dwt_inverse_valu(const DeviceTables &tb, int bw, int N, int np
                                c_ts, F, f_ts, T, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, mode, npair, nch,  \
                                nunit, tb.dtab);                                                                   \
         else                                                                                                      \
-            hipLaunchKernelGGL((dwt_inverse_valu<TBV, false>), dim3(grid_of(nunit, nch)), dim3(64), 0, s, coeff, \
-                               c_ts, F, f_ts, T, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, mode, npair, nch,  \
-                               nunit, nullptr);                                                                   \
+            hipLaunchKernelGGL((dwt_inverse_valu<TBV, false>), dim3(grid_of(nunit, nch)), dim3(64), cst_bytes, s, \
+                               coeff, c_ts, F, f_ts, T, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, mode, npair, \
+                               nch, nunit, nullptr);                                                              \
     } while (0)
     switch (TBv) {
     case 1: SO3_INV(1); break;
@@ -525,7 +527,7 @@ hipError_t launch_dwt_inverse_valu(const DeviceTables &tb, int bw, int N, int np
 hipError_t launch_dwt_forward_valu(const DeviceTables &tb, int bw, int N, int npair, int mode, const double2 *F,
                                    int64_t f_ts, double2 *coeff, int64_t c_ts, int T, hipStream_t s)
 {
-    if (mode == DWT_CORR) return hipErrorInvalidValue;
+    if (mode == DWT_CORR || bw > 256) return hipErrorInvalidValue;   // LDS sized for bw <= 256
     // TB = 4 needs 238 VGPRs (2 waves per SIMD) and measured slower than TB = 2
     int TBv = T >= 8 ? 2 : 1;
     if (valu_tb_env()) TBv = valu_tb_env();

@@ -22,6 +22,8 @@ struct FftK {
     int rad[16];
     int64_t in_t, in_o, in_r, out_t, out_o, out_r;
     int mask_rows, mask_outer, in_real, out_real, sign;
+    int in_c64, out_c64;   // complex64 storage of the input / output (FP64 arithmetic)
+    int col0, ncols;       // beta column window [col0, col0 + ncols)
     double scale;
 };
 
@@ -46,7 +48,8 @@ __global__ __launch_bounds__(FFT_THREADS) void fft_axis_kernel(const void *__res
     double2 *tw = sm;
     double2 *X0 = tw + N;
     double2 *X1 = X0 + N * C;
-    const int c0 = blockIdx.x * C;
+    const int c0 = k.col0 + blockIdx.x * C;
+    const int cend = k.col0 + k.ncols;
     const int64_t o = blockIdx.y, t = blockIdx.z;
 
     if (k.mask_outer && !order_ok((int)o, N, k.bw, k.lmax)) return;  // consumer masks this line
@@ -60,10 +63,16 @@ __global__ __launch_bounds__(FFT_THREADS) void fft_axis_kernel(const void *__res
     for (int idx = tid; idx < N * C; idx += FFT_THREADS) {
         const int r = idx / C, c = idx - r * C;
         double2 v = make_double2(0.0, 0.0);
-        if (c0 + c < N && (!k.mask_rows || order_ok(r, N, k.bw, k.lmax))) {
+        if (c0 + c < cend && (!k.mask_rows || order_ok(r, N, k.bw, k.lmax))) {
             const int64_t off = ib + r * k.in_r + c;
-            if (k.in_real) v.x = reinterpret_cast<const double *>(in)[off];
-            else v = reinterpret_cast<const double2 *>(in)[off];
+            if (k.in_real) {
+                v.x = reinterpret_cast<const double *>(in)[off];
+            } else if (k.in_c64) {
+                const float2 f = reinterpret_cast<const float2 *>(in)[off];
+                v = make_double2((double)f.x, (double)f.y);
+            } else {
+                v = reinterpret_cast<const double2 *>(in)[off];
+            }
         }
         X0[idx] = v;
     }
@@ -125,10 +134,11 @@ __global__ __launch_bounds__(FFT_THREADS) void fft_axis_kernel(const void *__res
     const int64_t ob = t * k.out_t + o * k.out_o + c0;
     for (int idx = tid; idx < N * C; idx += FFT_THREADS) {
         const int r = idx / C, c = idx - r * C;
-        if (c0 + c >= N) continue;
+        if (c0 + c >= cend) continue;
         const double2 v = A[idx];
         const int64_t off = ob + r * k.out_r + c;
         if (k.out_real) reinterpret_cast<double *>(out)[off] = v.x * k.scale;
+        else if (k.out_c64) reinterpret_cast<float2 *>(out)[off] = make_float2((float)(v.x * k.scale), (float)(v.y * k.scale));
         else reinterpret_cast<double2 *>(out)[off] = make_double2(v.x * k.scale, v.y * k.scale);
     }
 }
@@ -141,23 +151,36 @@ static int fft_cols(int N)
     return C;
 }
 
-hipError_t launch_fft_axis(const FftDesc &fd, const double2 *tw, int bw, int lmax, const void *in, void *out,
-                           const FftAxis &ax, int T, hipStream_t s)
+// radices of N: 4s, then 2s, then odd factors
+static int radices(int N, int *rad)
 {
-    if (fft_reg_supported(fd.N) && !getenv("SO3FT_FFT_STOCKHAM"))
-        return launch_fft_reg(fd.N, tw, bw, lmax, in, out, ax, T, 0, 0, 0, fd.N, s);
+    int nr = 0, n = N;
+    while (n % 4 == 0) { rad[nr++] = 4; n /= 4; }
+    while (n % 2 == 0) { rad[nr++] = 2; n /= 2; }
+    for (int p = 3; n > 1; p += 2)
+        while (n % p == 0) { rad[nr++] = p; n /= p; }
+    return nr;
+}
+
+hipError_t launch_fft_stockham(int N, const double2 *tw, int bw, int lmax, const void *in, void *out,
+                               const FftAxis &ax, int T, int in_c64, int out_c64, int col0, int ncols, hipStream_t s)
+{
     FftK k;
-    k.N = fd.N;
-    k.C = fft_cols(fd.N);
-    k.nrad = fd.nrad;
-    for (int i = 0; i < 16; ++i) k.rad[i] = (i < fd.nrad) ? fd.rad[i] : 1;
+    k.N = N;
+    k.C = fft_cols(N);
+    int rad[32];
+    k.nrad = radices(N, rad);
+    if (k.nrad > 16) return hipErrorInvalidValue;
+    for (int i = 0; i < 16; ++i) k.rad[i] = (i < k.nrad) ? rad[i] : 1;
     k.bw = bw; k.lmax = lmax;
     k.in_t = ax.in_t; k.in_o = ax.in_o; k.in_r = ax.in_r;
     k.out_t = ax.out_t; k.out_o = ax.out_o; k.out_r = ax.out_r;
     k.mask_rows = ax.mask_rows; k.mask_outer = ax.mask_outer;
     k.in_real = ax.in_real; k.out_real = ax.out_real;
+    k.in_c64 = in_c64; k.out_c64 = out_c64;
+    k.col0 = col0; k.ncols = ncols;
     k.sign = ax.sign; k.scale = ax.scale;
-    const int bytes = (int)sizeof(double2) * (fd.N + 2 * fd.N * k.C);
+    const int bytes = (int)sizeof(double2) * (N + 2 * N * k.C);
     static bool attr = false;
     if (!attr) {
         hipError_t e = hipFuncSetAttribute((const void *)fft_axis_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -165,9 +188,17 @@ hipError_t launch_fft_axis(const FftDesc &fd, const double2 *tw, int bw, int lma
         if (e != hipSuccess) return e;
         attr = true;
     }
-    dim3 grid((fd.N + k.C - 1) / k.C, fd.N, T), block(FFT_THREADS);
+    dim3 grid((ncols + k.C - 1) / k.C, N, T), block(FFT_THREADS);
     hipLaunchKernelGGL(fft_axis_kernel, grid, block, bytes, s, in, out, tw, k);
     return hipGetLastError();
+}
+
+hipError_t launch_fft_axis(const FftDesc &fd, const double2 *tw, int bw, int lmax, const void *in, void *out,
+                           const FftAxis &ax, int T, hipStream_t s)
+{
+    if (fft_reg_supported(fd.N) && !getenv("SO3FT_FFT_STOCKHAM"))
+        return launch_fft_reg(fd.N, tw, bw, lmax, in, out, ax, T, 0, 0, 0, fd.N, s);
+    return launch_fft_stockham(fd.N, tw, bw, lmax, in, out, ax, T, 0, 0, 0, fd.N, s);
 }
 
 // ---------------------------------------------------------------------------

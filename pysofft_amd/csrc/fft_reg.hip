@@ -449,6 +449,8 @@ static hipError_t launch_reg_typed(int N, const double2 *tw, int bw, int lmax, c
 hipError_t launch_fft_reg(int N, const double2 *tw, int bw, int lmax, const void *in, void *out,
                           const FftAxis &ax, int T, int in_c64, int out_c64, int col0, int ncols, hipStream_t s)
 {
+    // lengths without a register-kernel factorisation: the LDS Stockham kernel
+    if (!fft_reg_supported(N)) return launch_fft_stockham(N, tw, bw, lmax, in, out, ax, T, in_c64, out_c64, col0, ncols, s);
     if (!in_c64 && !out_c64) return launch_reg_typed<double2, double2>(N, tw, bw, lmax, in, out, ax, T, col0, ncols, s);
     if (in_c64 && out_c64) return launch_reg_typed<float2, float2>(N, tw, bw, lmax, in, out, ax, T, col0, ncols, s);
     if (in_c64) return launch_reg_typed<float2, double2>(N, tw, bw, lmax, in, out, ax, T, col0, ncols, s);

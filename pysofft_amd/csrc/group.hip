@@ -89,8 +89,8 @@ int check_bws(const int64_t *bws, int64_t n)
 {
     if (!bws || n < 1 || n > (1 << 20)) return set_error(SO3FT_EINVAL, "need 1 <= n <= 2^20 bandwidths");
     for (int64_t i = 0; i < n; ++i)
-        if (bws[i] < 2 || bws[i] > 256 || !fft_reg_supported((int)(2 * bws[i])))
-            return set_error(SO3FT_EUNSUPPORTED, "grouped transforms need 2 <= bw <= 256 with bw = 2^a or 3*2^a, got " +
+        if (bws[i] < 2 || bws[i] > 256)
+            return set_error(SO3FT_EUNSUPPORTED, "grouped transforms need 2 <= bw <= 256, got " +
                                                      std::to_string(bws[i]));
     return SO3FT_OK;
 }

@@ -174,7 +174,7 @@ void rec_consts_host(int64_t l, int64_t p, int64_t q, double &a, double &x, doub
 
 int build_plan(Plan *P, int64_t bw, int64_t lmax, int precompute, int init_ffts, int64_t rt, int32_t flags)
 {
-    if (bw < 1 || bw > 256) return fail(SO3FT_EINVAL, "bw must satisfy 1 <= bw <= 256, got " + std::to_string(bw));
+    if (bw < 1 || bw > 512) return fail(SO3FT_EINVAL, "bw must satisfy 1 <= bw <= 512, got " + std::to_string(bw));
     if (lmax < 0 || lmax >= bw) return fail(SO3FT_EINVAL, "lmax must satisfy 0 <= lmax < bw");
     if (rt != 0 && rt != 1) return fail(SO3FT_EINVAL, "unknown recurrence type " + std::to_string(rt));
     const int64_t N = 2 * bw;
@@ -285,8 +285,11 @@ int build_plan(Plan *P, int64_t bw, int64_t lmax, int precompute, int init_ffts,
     // reference transforms rerun the Risbo recurrence on every call (soft.f08:676-824),
     // fills it the same way at creation (the values depend only on bw).  The FP64 DWTs then
     // read d from the table.
-    if (precompute || rt == 1) {
-        const size_t bytes = sizeof(double) * (size_t)dtab_entries((int)bw);
+    // (A Risbo handle's table is capped at 24 GiB, bw <= 256: beyond it d comes from the
+    // Kostelec recurrence, equal to the Risbo values to rounding.)
+    const size_t tab_bytes = sizeof(double) * (size_t)dtab_entries((int)bw);
+    if (precompute || (rt == 1 && tab_bytes <= (size_t(24) << 30))) {
+        const size_t bytes = tab_bytes;
         if (hipMalloc((void **)&P->tb.dtab, bytes) != hipSuccess) {
             hipGetLastError();
             P->tb.dtab = nullptr;
@@ -412,7 +415,9 @@ int dwt_inverse(Plan *P, int mode, const double2 *c, int64_t c_ts, const double2
     const int bw = (int)P->bw, N = (int)P->N, np = (int)P->npair;
     // the MFMA engine forms correlation coefficients in its kernel; the other paths (and
     // handles that drop degrees above lmax) stage them first
-    const bool mfma = P->dwt_engine == SO3FT_DWT_MFMA && !P->tb.dtab;   // table plans read d on the VALU path
+    // table plans read d on the default engine's kernels; the MFMA engine's LDS holds the
+    // recurrence constants of bw <= 256
+    const bool mfma = P->dwt_engine == SO3FT_DWT_MFMA && !P->tb.dtab && P->bw <= 256;
     if (mode == DWT_CORR && (!mfma || P->lcap < P->bw - 1)) {
         if (int rc = ensure_cstage(P, T)) return rc;
         HIPCHK(launch_corr_coeffs(P->tb, bw, c, aux, c_ts, 0, 1, 0, nullptr, P->cstage, P->nc, (int)P->lcap, (int)T,
@@ -430,7 +435,7 @@ int dwt_inverse(Plan *P, int mode, const double2 *c, int64_t c_ts, const double2
 int dwt_forward(Plan *P, int mode, const double2 *F, int64_t f_ts, double2 *c, int64_t c_ts, int64_t T, hipStream_t s)
 {
     const int bw = (int)P->bw, N = (int)P->N, np = (int)P->npair;
-    if (P->dwt_engine == SO3FT_DWT_MFMA && !P->tb.dtab)
+    if (P->dwt_engine == SO3FT_DWT_MFMA && !P->tb.dtab && bw <= 256)
         HIPCHK(launch_dwt_forward(P->tb, bw, N, np, mode, F, f_ts, c, c_ts, (int)T, s));
     else if (mode == DWT_COMPLEX || mode == DWT_REAL)   // lane-beta FP64 MFMA forward (FP64 throughout)
         HIPCHK(launch_dwt_forward_m64(P->tb, bw, N, np, mode, F, f_ts, c, c_ts, (int)T, s));
@@ -555,8 +560,7 @@ int forward_fft_passes_c64(Plan *P, const float2 *in, float2 *F, int64_t T, hipS
 
 int check_c64(Plan *P)
 {
-    if (!fft_reg_supported((int)P->N))
-        return fail(SO3FT_EUNSUPPORTED, "complex64 transforms need bw = 2^a or 3*2^a (2 <= bw <= 256)");
+    (void)P;   // every bandwidth: lengths without a register FFT take the Stockham pass
     return SO3FT_OK;
 }
 
@@ -1177,7 +1181,6 @@ int so3ft_bench_kernel(so3ft_handle h, int which, int reps, void *a, void *b, in
         }
     };
     if (which < 0 || (which > 9 && which != 12 && which != 13)) return fail(SO3FT_EINVAL, "unknown kernel id");
-    if (which >= 8 && !fft_reg_supported((int)N)) return fail(SO3FT_EUNSUPPORTED, "no complex64 FFT for this bw");
     HIPCHK(launch());  // warm-up
     HIPCHK(hipEventRecord(e0, s));
     for (int i = 0; i < reps; ++i) HIPCHK(launch());
@@ -1193,7 +1196,7 @@ int so3ft_bench_kernel(so3ft_handle h, int which, int reps, void *a, void *b, in
 
 int so3ft_enforce_real_sym_mnl(int64_t bw, void *coeffs, int64_t n, void *stream)
 {
-    if (bw < 1 || bw > 256 || !coeffs || n < 0) return fail(SO3FT_EINVAL, "bad argument");
+    if (bw < 1 || bw > 512 || !coeffs || n < 0) return fail(SO3FT_EINVAL, "bad argument");
     // slice table for this bw (cached per bw)
     static std::mutex mu;
     static std::map<std::pair<int, int64_t>, int64_t *> cache;
@@ -1219,7 +1222,7 @@ int so3ft_enforce_real_sym_mnl(int64_t bw, void *coeffs, int64_t n, void *stream
 
 int so3ft_enforce_real_sym_lmn(int64_t bw, void *coeffs, int64_t n, void *stream)
 {
-    if (bw < 1 || bw > 256 || !coeffs || n < 0) return fail(SO3FT_EINVAL, "bad argument");
+    if (bw < 1 || bw > 512 || !coeffs || n < 0) return fail(SO3FT_EINVAL, "bad argument");
     if (n == 0) return SO3FT_OK;
     // reorder to mnl in a work buffer, apply the lmn routine's rule, reorder back
     static std::mutex mu;

@@ -159,10 +159,13 @@ hipError_t launch_fft_axis(const FftDesc &fd, const double2 *tw, int bw, int lma
                            void *out, const FftAxis &ax, int T, hipStream_t s);
 
 bool fft_reg_supported(int N);
+// the LDS Stockham pass for any N (mixed radices), complex64 or complex128 storage
+hipError_t launch_fft_stockham(int N, const double2 *tw, int bw, int lmax, const void *in, void *out,
+                               const FftAxis &ax, int T, int in_c64, int out_c64, int col0, int ncols, hipStream_t s);
 // plain device copy of nvec 16-byte vectors (16-byte aligned pointers): bandwidth reference
 hipError_t launch_copy16(const void *a, void *b, int64_t nvec, hipStream_t s);
-// register two-step pass for power-of-two N; in_c64/out_c64 select complex64 storage;
-// only beta columns [col0, col0+ncols) are processed
+// register two-step pass for N = 2^a or 3 * 2^a (other N: the Stockham pass); in_c64 /
+// out_c64 select complex64 storage; only beta columns [col0, col0+ncols) are processed
 hipError_t launch_fft_reg(int N, const double2 *tw, int bw, int lmax, const void *in, void *out,
                           const FftAxis &ax, int T, int in_c64, int out_c64, int col0, int ncols, hipStream_t s);
 

@@ -106,11 +106,35 @@ def test_grouped_graph_replays_match_eager():
         assert torch.equal(c2, ce), r
 
 
+def test_grouped_any_bandwidth_vs_fp64_and_single_path():
+    """Bandwidths that are not 2^a or 3 * 2^a (FFT lengths 10, 80, 110 take the Stockham
+    pass) in one group with a register-FFT bandwidth: equal to the single-bandwidth path
+    bit for bit and within 1e-7 of the FP64 path."""
+    from pysofft_amd import Soft
+    from pysofft_amd.grouped import SoftGroup
+    bws = [32, 40, 5, 55, 40]
+    g = SoftGroup(bws)
+    c = _coeffs(g, 7)
+    f = g.isoft(c)
+    ref = _single_path(bws, g.coeff_views(c), True)
+    for i, (a, b) in enumerate(zip(g.grid_views(f), ref)):
+        assert torch.equal(a, b), (i, bws[i])
+    c2 = g.soft(f)
+    for i, (cv, fv, rv) in enumerate(zip(g.coeff_views(c), g.grid_views(f), g.coeff_views(c2))):
+        s = Soft(bws[i])
+        f64 = s.isoft(cv.to(torch.complex128))
+        assert ((fv.to(torch.complex128) - f64).norm() / f64.norm()).item() <= TOL, i
+        r64 = s.soft(fv.to(torch.complex128))
+        assert ((rv.to(torch.complex128) - r64).norm() / r64.norm()).item() <= TOL, i
+
+
 def test_grouped_rejects_bad_inputs():
     from pysofft_amd import _lib
     from pysofft_amd.grouped import SoftGroup
     with pytest.raises(_lib.So3ftError):
-        SoftGroup([32, 40])
+        SoftGroup([32, 300])
+    with pytest.raises(_lib.So3ftError):
+        SoftGroup([1, 8])
     g = SoftGroup([4, 8])
     with pytest.raises(ValueError):
         g.isoft(torch.zeros(5, dtype=torch.complex64, device="cuda"))

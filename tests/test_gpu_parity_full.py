@@ -271,3 +271,56 @@ def test_risbo_lmax_sums_degrees_up_to_lmax(bw, lmax):
     assert rel_l2(s.cross_correlation_ylm_cmplx(fl, gl), ref) < 1e-12
     s.dwt_engine = "mfma"
     assert rel_l2(s.cross_correlation_ylm_cmplx(fl, gl), ref) < 1e-12
+
+
+def test_bw512_real_round_trip_and_samples():
+    """bw=512 (the largest bandwidth the reference publishes, docs/speed.md:20-23): real FP64
+    transforms on the 1024^3 grid (FFT length 1024: the Stockham pass).  irsoft then rsoft of
+    Hermitian coefficients scaled by (2l+1)^-1/2 returns them to relative L2 <= 1e-10 (the
+    Kostelec recurrence's own error grows to ~1e-11 at these degrees, docs/accuracy.md), and
+    grid samples of the l <= 2 part equal direct sums with the Jacobi-formula d."""
+    from oracle import brute
+    bw = 512
+    s = _soft(bw)
+    nc = s.n_coeffs
+    rng = np.random.default_rng(512)
+    ls = O.coeff_degrees(bw)[:, 0]
+    c = (rng.standard_normal(nc) + 1j * rng.standard_normal(nc)) / np.sqrt(2 * ls + 1)
+    O.enforce_real_sym(c, bw)
+    ct = torch.from_numpy(c).cuda()
+    f = s.irsoft(ct)
+    assert f.dtype == torch.float64 and tuple(f.shape) == (1024, 1024, 1024)
+    c2 = s.rsoft(f)
+    del f
+    assert _dev_rel(c2, ct) <= 1e-10
+    del c2
+    sel = np.nonzero(ls <= 2)[0]
+    lo = np.zeros(nc, np.complex128)
+    lo[sel] = c[sel]
+    fl = s.irsoft(torch.from_numpy(lo).cuda())
+    deg = O.coeff_degrees(bw)
+    a, b, _ = brute.grid_angles(bw)
+    for ig, ia, ib in [(0, 0, 0), (17, 700, 311), (1023, 511, 3), (256, 900, 1000)]:
+        v = 0j
+        for i in sel:
+            l, m, n = deg[i]
+            v += lo[i] * np.exp(-1j * m * a[ia]) * brute.wigner_d_norm(l, m, n, b[ib:ib + 1])[0] * \
+                np.exp(-1j * n * a[ig]) / (2 * np.pi)
+        got = fl[ig, ia, ib].item()
+        assert abs(got - v.real) <= 1e-12 * max(1.0, abs(v)), (ig, ia, ib, got, v)
+
+
+def test_bw512_mixed_vs_fp64():
+    """complex64 isoft / soft at bw=512 (one transform) against the FP64 path on the same
+    inputs: relative L2 <= 1e-7."""
+    bw = 512
+    s = _soft(bw)
+    g = torch.Generator(device="cuda").manual_seed(5120)
+    c32 = torch.view_as_complex(torch.randn(1, s.n_coeffs, 2, generator=g, device="cuda")).contiguous()
+    f32 = s.isoft_many_c64(c32)
+    f64 = s.isoft_many(c32.to(torch.complex128))
+    assert _dev_rel(f32[0], f64[0]) <= 1e-7
+    del f64
+    r32 = s.soft_many_c64(f32)
+    r64 = s.soft_many(f32.to(torch.complex128))
+    assert _dev_rel(r32[0], r64[0]) <= 1e-7

@@ -303,7 +303,7 @@ def test_invalid_inputs_raise_not_abort():
         s.isoft(np.zeros(10, np.complex128))
     from pysofft_amd._lib import So3ftError
     with pytest.raises(So3ftError):
-        _soft(300)
+        _soft(513)
 
 
 @pytest.mark.parametrize("bw", [1, 2, 5, 16, 33])
@@ -380,11 +380,24 @@ def test_mixed_c64_large_bw_vs_fp64(bw):
     assert f32.shape == (n, N, N, N)
 
 
-def test_mixed_c64_rejects_unsupported_bw():
-    from pysofft_amd import _lib
-    s = _soft(5)
-    with pytest.raises(_lib.So3ftError):
-        s.isoft_many_c64(np.zeros((1, s.n_coeffs), np.complex64))
+@pytest.mark.parametrize("bw", [5, 7, 20, 40])
+def test_mixed_c64_any_bw_vs_oracle(bw):
+    """complex64 transforms at bandwidths that are not 2^a or 3 * 2^a (FFT lengths 10, 14,
+    40, 80: the Stockham pass with complex64 storage) against the FP64 oracle:
+    per-transform relative L2 <= 1e-7."""
+    rng = np.random.default_rng(800 + bw)
+    s = _soft(bw)
+    P = O.OraclePlan(bw)
+    n = 3
+    c = _rand_c(rng, (n, P.nc)).astype(np.complex64)
+    f = np.asarray(s.isoft_many_c64(c))
+    fo = P.isoft_many(c.astype(np.complex128))
+    g = _rand_c(rng, (n,) + tuple(P.grid_shape)).astype(np.complex64)
+    co = P.soft_many(g.astype(np.complex128))
+    cg = np.asarray(s.soft_many_c64(g))
+    for i in range(n):
+        assert rel_l2(f[i], fo[i]) <= 1e-7, (bw, i, rel_l2(f[i], fo[i]))
+        assert rel_l2(cg[i], co[i]) <= 1e-7, (bw, i, rel_l2(cg[i], co[i]))
 
 
 @pytest.mark.parametrize("engine", ["valu", "mfma"])

@@ -90,9 +90,9 @@ def test_group_layout_host():
         g += (2 * bws[i]) ** 3
     assert (ct, gt) == (c, g)
     L = _lib.lib()
-    b = np.array([32, 40], np.int64)
+    b = np.array([32, 300], np.int64)
     assert L.so3ft_group_layout(b.ctypes.data, 2, None, None, None, None) == -5
-    assert b"3*2^a" in L.so3ft_last_error()
+    assert b"2 <= bw <= 256" in L.so3ft_last_error()
     b = np.array([1], np.int64)
     assert L.so3ft_group_layout(b.ctypes.data, 1, None, None, None, None) == -5
     assert L.so3ft_group_layout(b.ctypes.data, 0, None, None, None, None) == -1
