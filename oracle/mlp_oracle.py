@@ -6,6 +6,11 @@ layer linear; the 6 outputs (a, b) -> rotation R = [x y z] by Gram-Schmidt
 (x = a/|a|, y = (b - (x.b) x)/|.|, z = x cross y); loss = mean ||R - T||_F^2.
 Only tests load this module.  Its gradients are pinned by finite differences in
 tests/test_oracle_pins.py.
+
+``forward_bf16`` / ``gradients_bf16`` round to bfloat16 (nearest even) where the device
+network stores bfloat16 operands (inputs, weights, hidden activations after GELU, and the
+backpropagated dZ), everything else FP64, so that they differ from the device only by
+its FP32 accumulation.
 """
 import numpy as np
 from scipy.special import erf
@@ -94,3 +99,47 @@ def gradients(params, dims, X, T):
             break
         g = (g @ W) * gelu_grad(zs[i - 1])
     return np.concatenate([np.concatenate([gw.ravel(), gb]) for gw, gb in grads])
+
+
+def bf16_round(a):
+    """float values rounded to bfloat16 (round to nearest even), returned as float64"""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward_bf16(params, dims, X):
+    layers = unpack(params, dims)
+    a = bf16_round(X)
+    zs, acts = [], [a]
+    for i, (W, b) in enumerate(layers):
+        z = a @ bf16_round(W).T + b
+        zs.append(z)
+        a = bf16_round(gelu(z)) if i + 1 < len(layers) else z
+        acts.append(a)
+    return a, zs, acts
+
+
+def gradients_bf16(params, dims, X, T):
+    """gradients with the device's bfloat16 rounding points: dW = bf16(dZ)^T bf16(A),
+    db = sum of the unrounded dZ, dA = bf16(dZ) bf16(W)"""
+    layers = unpack(params, dims)
+    out, zs, acts = forward_bf16(params, dims, X)
+    g = grad_out(out, T)
+    grads = [None] * len(layers)
+    for i in range(len(layers) - 1, -1, -1):
+        W, _ = layers[i]
+        gb = bf16_round(g)
+        grads[i] = (gb.T @ acts[i], g.sum(0))
+        if i == 0:
+            break
+        g = (gb @ bf16_round(W)) * gelu_grad(zs[i - 1])
+    return np.concatenate([np.concatenate([gw.ravel(), gbias]) for gw, gbias in grads])
+
+
+def adam_step(w, g, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """one Adam update (bias-corrected), FP64: returns (w, m, v)"""
+    m = b1 * m + (1 - b1) * g
+    v = b2 * v + (1 - b2) * g * g
+    c1, c2 = 1 - b1 ** step, 1 - b2 ** step
+    return w - lr * (m / c1) / (np.sqrt(v / c2) + eps), m, v

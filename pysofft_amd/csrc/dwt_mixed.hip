@@ -800,8 +800,21 @@ __global__ __launch_bounds__(512) void dwt_forward_m64w(
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double (*Dt)[F64_DS] = reinterpret_cast<double (*)[F64_DS]>(smx) + 16 * wv;
     auto red = [&](int c, int v) -> double * { return smx + (int64_t)c * 16 * F64_DS + v * 64; };
-    const int pq = pairs[unit % npair], p = pq & 0xffff, q = pq >> 16;
-    const int t0 = (unit / npair) * TB;
+    // on the fly: pairs fastest.  TAB: the transform groups of one pair are consecutive
+    // blocks of one XCD (block = 8 slot + xcd), so its table rows come from L2 after the
+    // first read
+    int pidx, grp;
+    if (TAB) {
+        const int ng = (T + TB - 1) / TB, slot = unit >> 3;
+        pidx = (slot / ng) * 8 + (unit & 7);
+        grp = slot % ng;
+        if (pidx >= npair) return;
+    } else {
+        pidx = unit % npair;
+        grp = unit / npair;
+    }
+    const int pq = pairs[pidx], p = pq & 0xffff, q = pq >> 16;
+    const int t0 = grp * TB;
     const int L = bw - q;
     const int nt = min(TB, T - t0);
     const int j = lane & 15, kk = lane >> 4;
@@ -945,8 +958,10 @@ static void launch_m64(const DeviceTables &tb, int bw, int N, int npair, const d
     const int nc = (N + 63) / 64;
     if (nc >= 2 && nc <= 8) {   // one wave per 64-beta chunk: fft-array samples read once
         const size_t ldsw = sizeof(double) * (size_t)nc * 16 * F64_DS;
-        hipLaunchKernelGGL((dwt_forward_m64w<TB, TAB>), dim3(nunit), dim3(64 * nc), ldsw, s, F, f_ts, coeff, c_ts, T,
-                           tb.w, tb.cosb, tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit, real,
+        // table order: 8 pairs x transform groups per run of slots (see the kernel)
+        const int nb = TAB ? (npair + 7) / 8 * 8 * ((T + TB - 1) / TB) : nunit;
+        hipLaunchKernelGGL((dwt_forward_m64w<TB, TAB>), dim3(nb), dim3(64 * nc), ldsw, s, F, f_ts, coeff, c_ts, T,
+                           tb.w, tb.cosb, tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nb, real,
                            tb.dtab);
         return;
     }

@@ -150,11 +150,24 @@ __global__ __launch_bounds__(64) void dwt_inverse_valu(const double2 *__restrict
     int unit, chunk;
     if (!decode_unit(nch, nunit, unit, chunk)) return;
     const int lane = threadIdx.x;
-    const int pq = pairs[unit % npair];
+    // on the fly: pairs fastest.  TAB: the transform groups of one pair are consecutive
+    // units of one XCD (unit = 8 slot + xcd), so the pair's table rows are read from HBM
+    // once and then from that XCD's L2.
+    int pidx, grp;
+    if (TAB) {
+        const int ng = (T + TB - 1) / TB, slot = unit >> 3;
+        pidx = (slot / ng) * 8 + (unit & 7);
+        grp = slot % ng;
+        if (pidx >= npair) return;
+    } else {
+        pidx = unit % npair;
+        grp = unit / npair;
+    }
+    const int pq = pairs[pidx];
     const int p = pq & 0xffff, q = pq >> 16;
     const int L = bw - q;
     const int beta = chunk * 64 + lane;
-    const int t0 = (unit / npair) * TB;
+    const int t0 = grp * TB;
     const int nt = min(TB, T - t0);
 
     // the coefficient this lane holds for the row broadcasts: variant k & 7, degree
@@ -505,11 +518,13 @@ hipError_t launch_dwt_inverse_valu(const DeviceTables &tb, int bw, int N, int np
     const size_t cst_bytes = sizeof(double) * 3 * (size_t)(bw + 2);
 #define SO3_INV(TBV)                                                                                              \
     do {                                                                                                          \
-        const int nunit = npair * ((T + TBV - 1) / TBV);                                                          \
+        const int ngr = (T + TBV - 1) / TBV;                                                                      \
+        const int nunit = npair * ngr;                                                                            \
+        const int nunit_t = (npair + 7) / 8 * 8 * ngr;   /* table order: 8 pairs x groups per slot run */       \
         if (tb.dtab)                                                                                              \
-            hipLaunchKernelGGL((dwt_inverse_valu<TBV, true>), dim3(grid_of(nunit, nch)), dim3(64), 0, s, coeff,  \
+            hipLaunchKernelGGL((dwt_inverse_valu<TBV, true>), dim3(grid_of(nunit_t, nch)), dim3(64), 0, s, coeff, \
                                c_ts, F, f_ts, T, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, mode, npair, nch,  \
-                               nunit, tb.dtab);                                                                   \
+                               nunit_t, tb.dtab);                                                                 \
         else                                                                                                      \
             hipLaunchKernelGGL((dwt_inverse_valu<TBV, false>), dim3(grid_of(nunit, nch)), dim3(64), cst_bytes, s, \
                                coeff, c_ts, F, f_ts, T, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, mode, npair, \

@@ -42,6 +42,11 @@ def _random_rotations(rng, n):
 
 
 def test_mlp_forward_and_gradients_vs_oracle():
+    """The network's forward pass, loss and per-layer gradients against the FP64 oracle that
+    rounds to bfloat16 where the device does (oracle/mlp_oracle.py, itself pinned by finite
+    differences): relative L2 <= 1e-2 (what is left is FP32 accumulation and the rounding
+    of values it moves across a bfloat16 boundary); the unrounded oracle is within 3e-2.
+    Then one Adam step (lr > 0) against a numpy Adam on the device's own gradients."""
     from oracle import mlp_oracle as M
     from pysofft_amd.align import AlignmentMLP, init_params
     rng = np.random.default_rng(1)
@@ -53,22 +58,33 @@ def test_mlp_forward_and_gradients_vs_oracle():
     T = _random_rotations(rng, n)
     Xt = torch.from_numpy(X).cuda()
     R, out = net.forward(Xt)
-    o_ref, _, _ = M.forward(p, dims, X)
-    assert rel_l2(out.cpu().numpy(), o_ref) < 3e-2
-    assert rel_l2(R.cpu().numpy(), M.gram_schmidt(o_ref)) < 3e-2
-    loss = net.train_step(Xt, torch.from_numpy(T.astype(np.float32)).cuda(), 0.0)
-    assert abs(loss - M.loss(p, dims, X, T.reshape(n, 9))) < 3e-2 * abs(M.loss(p, dims, X, T.reshape(n, 9)))
+    o_ref, _, _ = M.forward_bf16(p, dims, X)
+    assert rel_l2(out.cpu().numpy(), o_ref) < 1e-2
+    assert rel_l2(R.cpu().numpy(), M.gram_schmidt(o_ref)) < 1e-2
+    assert rel_l2(out.cpu().numpy(), M.forward(p, dims, X)[0]) < 3e-2
+    Tt = torch.from_numpy(T.astype(np.float32)).cuda()
+    loss = net.train_step(Xt, Tt, 0.0)
+    l_ref = np.mean(np.sum((M.gram_schmidt(o_ref) - T) ** 2, axis=(1, 2)))
+    assert abs(loss - l_ref) < 1e-2 * abs(l_ref)
     g = net.get_params(grads=True)
-    g_ref = M.gradients(p, dims, X, T.reshape(n, 9))
+    g_ref = M.gradients_bf16(p, dims, X, T.reshape(n, 9))
     o = 0
     for i in range(len(dims) - 1):
         nw, nbias = dims[i] * dims[i + 1], dims[i + 1]
-        assert rel_l2(g[o:o + nw], g_ref[o:o + nw]) < 5e-2, ("W", i)
+        assert rel_l2(g[o:o + nw], g_ref[o:o + nw]) < 1e-2, ("W", i, rel_l2(g[o:o + nw], g_ref[o:o + nw]))
         o += nw
-        assert rel_l2(g[o:o + nbias], g_ref[o:o + nbias]) < 5e-2, ("b", i)
+        assert rel_l2(g[o:o + nbias], g_ref[o:o + nbias]) < 1e-2, ("b", i, rel_l2(g[o:o + nbias], g_ref[o:o + nbias]))
         o += nbias
     # lr = 0 leaves the weights unchanged
     np.testing.assert_array_equal(net.get_params(), p)
+    # one Adam step (the first: zero moments, bias corrections 1 - 0.9, 1 - 0.999)
+    lr = 1e-3
+    net.train_step(Xt, Tt, lr)
+    g1 = net.get_params(grads=True).astype(np.float64)
+    w1 = net.get_params().astype(np.float64)
+    w_ref, _, _ = M.adam_step(p.astype(np.float64), g1, 0.0, 0.0, 1, lr)
+    np.testing.assert_allclose(w1, w_ref, rtol=0, atol=1e-6 * lr + 2e-7 * np.abs(w_ref).max())
+    assert np.abs(w1 - p).max() > 0.5 * lr   # the step moved the weights
 
 
 def test_features_vs_numpy():
