@@ -382,16 +382,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dw
 }
 
 // ---------------------------------------------------------------------------
-// forward, one wave per (pair, TB transforms): lane = beta within a 64-beta chunk; the
-// lane carries one recurrence chain per chunk (beta = 64 c + lane), its state kept in
-// LDS between tiles.  Degrees go in 16-degree tiles; for each (tile, chunk) the lanes
-// advance their chain by 16 degrees, writing w(beta) d (hi, lo) to an LDS tile
-// [degree][beta], and the wave runs 16 K-steps of v_mfma_f32_16x16x4f32 with A = that
-// tile (M = degree, K = beta) and B = the fft-array samples (K = beta, N = 8 variants x
-// 2 transforms), one MFMA for the real and one for the imaginary parts.  FP32
-// accumulation over 16 betas, FP64 across them and across chunks: a tile's
-// coefficients are complete after its last chunk, so there is no cross-wave reduction
-// and no degree is recomputed.
+// forward, one wave per (pair, TB transforms): v_mfma_f32_16x16x4f32 with A = w(beta) d
+// tiles [degree][beta] in LDS (M = degree, K = beta) and B = the fft-array samples (K =
+// beta, N = 8 variants x 2 transforms), one MFMA for the real and one for the imaginary
+// parts.  FP32 accumulation over a few K-steps, FP64 across them and across chunks: a
+// tile's coefficients are complete after its last chunk, so there is no cross-wave
+// reduction and no degree is recomputed.
 //
 // B loads: K-step s of lane (column j, k) takes beta 8 (s >> 1) + 2 k + (s & 1) of the
 // chunk, so one 16-byte load gives a lane both samples (re, im, re, im) of two K-steps
@@ -399,18 +395,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dw
 // transform pair.  The columns are sample-contiguous in beta (reversed for the variants
 // at pi - beta), so the fft array must be 16-byte aligned with an even transform stride.
 // ---------------------------------------------------------------------------
-constexpr int FT_DS = 66;   // padded row of the forward tile (float2 entries)
+__device__ __forceinline__ void fwd_stage_sampling(double2 (*rs)[64], const double *cosb, const double *w, int N);
+constexpr int FT32_DS = 65;   // padded row of the 32-degree hi tile (floats)
+constexpr int FT32_TD = 32;   // degrees per tile
 
 template <int TB>
-__device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64_t f_ts,
-                                             float2 *__restrict__ coeff, int64_t c_ts, int T,
-                                             const double *__restrict__ seeds, const double *__restrict__ rc,
-                                             const int64_t *__restrict__ rc_off, const int64_t *__restrict__ slice,
-                                             int bw, int N, int p, int q, int t0, float2 (*Dt)[FT_DS],
-                                             double2 (*rs)[64])
+__device__ __forceinline__ void fwd_mix_item32(const float2 *__restrict__ F, int64_t f_ts,
+                                               float2 *__restrict__ coeff, int64_t c_ts, int T,
+                                               const double *__restrict__ seeds, const double *__restrict__ rc,
+                                               const int64_t *__restrict__ rc_off,
+                                               const int64_t *__restrict__ slice, int bw, int N, int p, int q,
+                                               int t0, float (*Dt)[FT32_DS], double2 (*rs)[64])
 {
     constexpr int TG = (TB + 1) / 2;   // transform pairs: column j = variant j & 7 of transform 2 g + (j >> 3)
-    constexpr int KS = 4;              // accumulators per (pair, part): K-step s uses s % KS
+    constexpr int KS = 2;              // accumulators per (pair, part, M-tile): K-step s uses s % KS
     const int lane = threadIdx.x & 63;
     const int L = bw - q;
     const int nc = (N + 63) >> 6;
@@ -424,9 +422,6 @@ __device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64
     for (int g = 0; g < TG; ++g)
         src[g] = F + (int64_t)max(0, min(t0 + 2 * g + tt, T - 1)) * f_ts + colj;
 
-    // chain states (d^l, d^{l-1}) per chunk; rows nc .. 2 nc - 1 of rs hold (cos beta, w),
-    // staged by the caller, so that the only vector loads in the loop are the B operands
-    // (vector loads complete in order)
     const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
     const const_dptr rcp = (const_dptr)rc + 3 * ((const_i64ptr)rc_off)[tri];
     for (int c = 0; c < nc; ++c) {
@@ -435,11 +430,7 @@ __device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64
     }
     __syncthreads();   // rs (one wave: orders the LDS writes before the reads)
 
-    // B operands of K-steps 2h, 2h+1 of chunk c for transform pair g: the samples at
-    // betas b0 and b0 + 1, b0 = 64 c + 8 h + 2 k (clamped into the column; rows past N
-    // meet zero A rows), as loaded: (re, im) of b0 then b0 + 1, or the reverse order for
-    // the variants at pi - beta.  They are picked apart where the K-steps use them, so
-    // that nothing waits for a load before then.
+    // B operands of K-steps 2h, 2h+1 of chunk c for transform pair g, as in the inverse
     float4 raw[8][TG];
     auto load_b = [&](int c, int h) {
         const int b0 = min(64 * c + 8 * h + 2 * kk, N - 2);
@@ -451,101 +442,75 @@ __device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64
     for (int h = 0; h < 8; ++h) load_b(0, h);
 
 #pragma unroll 1
-    for (int l0 = 0; l0 < L; l0 += 16) {
-        double accr[TG][4], acci[TG][4];
+    for (int l0 = 0; l0 < L; l0 += FT32_TD) {
+        double accr[TG][2][4], acci[TG][2][4];
 #pragma unroll
         for (int g = 0; g < TG; ++g)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) accr[g][r] = acci[g][r] = 0.0;
+            for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) accr[g][mt][r] = acci[g][mt][r] = 0.0;
 #pragma unroll 1
         for (int c = 0; c < nc; ++c) {
-            // raw holds this chunk's B operands, loaded during the previous chunk's K-steps
             const int cn = c + 1 == nc ? 0 : c + 1;   // chunk whose operands the K-steps reload
-            // w d (hi, lo) for degrees l0 .. l0 + 15 at this lane's beta of chunk c
             const double2 cw = rs[nc + c][lane];
             const double cb = cw.x, wt = cw.y;
             asm volatile("" ::: "memory");
             double2 st = rs[c][lane];
             double dcur = st.x, dprev = st.y;
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {   // the chain is zero past the last degree
-                float hi, lo;
-                split(wt * dcur, hi, lo);
-                Dt[k][lane] = make_float2(hi, lo);
+            for (int k = 0; k < FT32_TD; ++k) {   // the chain is zero past the last degree
+                Dt[k][lane] = (float)(wt * dcur);
                 rec_step(rcp, l0 + k, cb, dcur, dprev);
             }
             rs[c][lane] = make_double2(dcur, dprev);
             asm volatile("" ::: "memory");
-            f4_t ar[TG][KS], ai[TG][KS];
-#pragma unroll
-            for (int g = 0; g < TG; ++g)
-#pragma unroll
-                for (int u = 0; u < KS; ++u) ar[g][u] = ai[g][u] = f4_t{0.f, 0.f, 0.f, 0.f};
-            // lo products of K-steps 4 m .. 4 m + 3 (slot s4 & 3, for A and B alike) in one
-            // bfloat16 MFMA per (transform pair, part); slots packed in pairs (word
-            // (s4 >> 1) & 1) from the values of the even K-step (pa, pr, pi) and the odd one
-            uint32_t alo[2], blr[TG][2], bli[TG][2];
-            float pa, pr[TG], pi[TG];
+            f4_t ar[TG][2][KS], ai[TG][2][KS];
+            // FP32 sums of at most 4 K-steps (16 products) per accumulator, then FP64: the
+            // accumulators are flushed after each half of the chunk's K-steps
 #pragma unroll
             for (int s4 = 0; s4 < 16; ++s4) {
-                const float2 a = Dt[j][8 * (s4 >> 1) + 2 * kk + (s4 & 1)];
+                if ((s4 & 7) == 0) {
+#pragma unroll
+                    for (int g = 0; g < TG; ++g)
+#pragma unroll
+                        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                            for (int u = 0; u < KS; ++u) ar[g][mt][u] = ai[g][mt][u] = f4_t{0.f, 0.f, 0.f, 0.f};
+                }
+                const int col = 8 * (s4 >> 1) + 2 * kk + (s4 & 1);   // beta of this K-step in the chunk
+                const float a0 = Dt[j][col], a1 = Dt[16 + j][col];
                 const int u = s4 % KS;
                 const bool second = (s4 & 1) != vr.rev;   // this K-step's sample is the load's second
-                float bre[TG], bim[TG];
 #pragma unroll
                 for (int g = 0; g < TG; ++g) {
                     const float4 v = raw[s4 >> 1][g];
-                    bre[g] = second ? v.z : v.x;
-                    bim[g] = second ? v.w : v.y;
+                    const float bre = second ? v.z : v.x, bim = second ? v.w : v.y;
+                    ar[g][0][u] = mma32(a0, bre, ar[g][0][u]);
+                    ai[g][0][u] = mma32(a0, bim, ai[g][0][u]);
+                    ar[g][1][u] = mma32(a1, bre, ar[g][1][u]);
+                    ai[g][1][u] = mma32(a1, bim, ai[g][1][u]);
                 }
-#pragma unroll
-                for (int g = 0; g < TG; ++g) {
-                    ar[g][u] = mma32(a.x, bre[g], ar[g][u]);
-                    ai[g][u] = mma32(a.x, bim[g], ai[g][u]);
-                }
-                if (s4 & 1) {
-                    const int w2 = (s4 >> 1) & 1;
-                    alo[w2] = bf16x2_trunc(pa, a.y);
-#pragma unroll
-                    for (int g = 0; g < TG; ++g) {
-                        blr[g][w2] = bf16x2_trunc(pr[g], bre[g]);
-                        bli[g][w2] = bf16x2_trunc(pi[g], bim[g]);
-                    }
-                } else {
-                    pa = a.y;
-#pragma unroll
-                    for (int g = 0; g < TG; ++g) {
-                        pr[g] = bre[g];
-                        pi[g] = bim[g];
-                    }
-                }
-                if ((s4 & 3) == 3) {
-                    const bf4_t la = bf4_of(alo[0], alo[1]);
-#pragma unroll
-                    for (int g = 0; g < TG; ++g) {
-                        ar[g][1] = mma_bf16(la, bf4_of(blr[g][0], blr[g][1]), ar[g][1]);
-                        ai[g][1] = mma_bf16(la, bf4_of(bli[g][0], bli[g][1]), ai[g][1]);
-                    }
-                }
-                // K-steps 2h, 2h+1 consumed: reload them for chunk cn (chunk 0 of the next
-                // tile after the last chunk; once more, unused, after the last tile), so
-                // that the loads have a whole chunk of work to arrive in
                 if (s4 & 1) {
                     load_b(cn, s4 >> 1);
                     __builtin_amdgcn_sched_barrier(0);   // issued here, not after the last K-step
                 }
+                if ((s4 & 7) == 7) {
+#pragma unroll
+                    for (int g = 0; g < TG; ++g)
+#pragma unroll
+                        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                            for (int u = 0; u < KS; ++u)
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) {
+                                    accr[g][mt][r] += (double)ar[g][mt][u][r];
+                                    acci[g][mt][r] += (double)ai[g][mt][u][r];
+                                }
+                }
             }
-#pragma unroll
-            for (int g = 0; g < TG; ++g)
-#pragma unroll
-                for (int u = 0; u < KS; ++u)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        accr[g][r] += (double)ar[g][u][r];
-                        acci[g][r] += (double)ai[g][u][r];
-                    }
         }
-        // item r of lane l: degree row 4(l>>4) + r, column j = l & 15
+        // item r of lane l in M-tile mt: degree row 16 mt + 4(l>>4) + r, column j = l & 15
         if (vr.valid) {
             const int64_t o = slice_m(slice, vr.m, vr.n, bw);
 #pragma unroll
@@ -554,24 +519,58 @@ __device__ __forceinline__ void fwd_mix_item(const float2 *__restrict__ F, int64
                 if (t >= nt) continue;
                 float2 *ct = coeff + (int64_t)(t0 + t) * c_ts + o;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int li = l0 + 4 * kk + r;
-                    if (li < L) {
-                        const double sg = vr.sign * ((vr.lsign && ((q + li) & 1)) ? -1.0 : 1.0);
-                        ct[li] = make_float2((float)(sg * accr[g][r]), (float)(sg * acci[g][r]));
+                for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int li = l0 + 16 * mt + 4 * kk + r;
+                        if (li < L) {
+                            const double sg = vr.sign * ((vr.lsign && ((q + li) & 1)) ? -1.0 : 1.0);
+                            ct[li] = make_float2((float)(sg * accr[g][mt][r]), (float)(sg * acci[g][mt][r]));
+                        }
                     }
-                }
             }
         }
     }
 }
 
-// LDS of the forward item: Dt [16][FT_DS] float2 | rs [2 nc][64] double2 (chain states,
-// then (cos beta, w))
-__device__ __forceinline__ void fwd_lds_carve(double *smx, float2 (*&Dt)[FT_DS], double2 (*&rs)[64])
+template <int TB>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void dwt_forward_mix32(
+    const float2 *__restrict__ F, int64_t f_ts, float2 *__restrict__ coeff, int64_t c_ts, int T,
+    const double *__restrict__ w, const double *__restrict__ cosb, const double *__restrict__ seeds,
+    const double *__restrict__ rc, const int64_t *__restrict__ rc_off, const int64_t *__restrict__ slice,
+    const int32_t *__restrict__ pairs, int bw, int N, int npair, int nunit)
 {
-    Dt = reinterpret_cast<float2 (*)[FT_DS]>(smx);
-    rs = reinterpret_cast<double2 (*)[64]>(smx + 16 * FT_DS);
+    // dynamic LDS: Dt [32][FT32_DS] float | rs [2 nc][64] double2
+    extern __shared__ double smx[];
+    float (*Dt)[FT32_DS] = reinterpret_cast<float (*)[FT32_DS]>(smx);
+    double2 (*rs)[64] = reinterpret_cast<double2 (*)[64]>(smx + (FT32_TD * FT32_DS + 1) / 2);
+    const int unit = blockIdx.x;
+    if (unit >= nunit) return;
+    const int pq = pairs[unit % npair];
+    fwd_stage_sampling(rs, cosb, w, N);
+    fwd_mix_item32<TB>(F, f_ts, coeff, c_ts, T, seeds, rc, rc_off, slice, bw, N, pq & 0xffff, pq >> 16,
+                       (unit / npair) * TB, Dt, rs);
+}
+
+// grouped forward over transforms of several bandwidths (see dwt_inverse_mix_grouped);
+// dynamic LDS sized for the largest bandwidth of the group
+template <int TB>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void dwt_forward_mix32_grouped(
+    const float2 *__restrict__ F, float2 *__restrict__ coeff, const MixSegment *__restrict__ segs,
+    const int4 *__restrict__ items, int nitem)
+{
+    extern __shared__ double smx[];
+    float (*Dt)[FT32_DS] = reinterpret_cast<float (*)[FT32_DS]>(smx);
+    double2 (*rs)[64] = reinterpret_cast<double2 (*)[64]>(smx + (FT32_TD * FT32_DS + 1) / 2);
+    for (int it = blockIdx.x; it < nitem; it += gridDim.x) {
+        const int4 wk = items[it];
+        if (wk.x < 0) continue;
+        const MixSegment sg = segs[wk.x];
+        __syncthreads();  // the previous item's LDS reads are done
+        fwd_stage_sampling(rs, sg.cosb, sg.w, sg.N);
+        fwd_mix_item32<TB>(F + sg.goff, (int64_t)sg.N * sg.N * sg.N, coeff + sg.coff, sg.nc, sg.T, sg.seeds,
+                           sg.rc, sg.rc_off, sg.slice, sg.bw, sg.N, wk.y & 0xffff, wk.y >> 16, wk.z, Dt, rs);
+    }
 }
 
 // (cos beta, w(beta)) rows of the chain state table
@@ -582,52 +581,6 @@ __device__ __forceinline__ void fwd_stage_sampling(double2 (*rs)[64], const doub
         const int beta = 64 * c + lane;
         rs[nc + c][lane] = beta < N ? make_double2(((gptr<double>)cosb)[beta], ((gptr<double>)w)[beta])
                                     : make_double2(0.0, 0.0);
-    }
-}
-
-template <int TB>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void dwt_forward_mix(const float2 *__restrict__ F, int64_t f_ts,
-                                                      float2 *__restrict__ coeff, int64_t c_ts, int T,
-                                                      const double *__restrict__ w, const double *__restrict__ cosb,
-                                                      const double *__restrict__ seeds,
-                                                      const double *__restrict__ rc,
-                                                      const int64_t *__restrict__ rc_off,
-                                                      const int64_t *__restrict__ slice,
-                                                      const int32_t *__restrict__ pairs, int bw, int N, int npair,
-                                                      int nunit)
-{
-    extern __shared__ double smx[];
-    float2 (*Dt)[FT_DS];
-    double2 (*rs)[64];
-    fwd_lds_carve(smx, Dt, rs);
-    const int unit = blockIdx.x;
-    if (unit >= nunit) return;
-    const int pq = pairs[unit % npair];
-    fwd_stage_sampling(rs, cosb, w, N);
-    fwd_mix_item<TB>(F, f_ts, coeff, c_ts, T, seeds, rc, rc_off, slice, bw, N, pq & 0xffff, pq >> 16,
-                     (unit / npair) * TB, Dt, rs);
-}
-
-// grouped forward over transforms of several bandwidths (see dwt_inverse_mix_grouped);
-// dynamic LDS sized for the largest bandwidth of the group
-template <int TB>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void dwt_forward_mix_grouped(const float2 *__restrict__ F,
-                                                              float2 *__restrict__ coeff,
-                                                              const MixSegment *__restrict__ segs,
-                                                              const int4 *__restrict__ items, int nitem)
-{
-    extern __shared__ double smx[];
-    for (int it = blockIdx.x; it < nitem; it += gridDim.x) {
-        const int4 wk = items[it];
-        if (wk.x < 0) continue;
-        const MixSegment sg = segs[wk.x];
-        __syncthreads();  // the previous item's LDS reads are done
-        float2 (*Dt)[FT_DS];
-        double2 (*rs)[64];
-        fwd_lds_carve(smx, Dt, rs);
-        fwd_stage_sampling(rs, sg.cosb, sg.w, sg.N);
-        fwd_mix_item<TB>(F + sg.goff, (int64_t)sg.N * sg.N * sg.N, coeff + sg.coff, sg.nc, sg.T, sg.seeds, sg.rc,
-                         sg.rc_off, sg.slice, sg.bw, sg.N, wk.y & 0xffff, wk.y >> 16, wk.z, Dt, rs);
     }
 }
 
@@ -899,10 +852,10 @@ __global__ __launch_bounds__(512) void dwt_forward_m64w(
 
 static int grid_m(int nunit, int nch) { return (nunit + 7) / 8 * 8 * nch; }
 
-// dynamic LDS bytes of dwt_forward_mix (A tile of 16 degrees, chunk states and sampling)
-static size_t fwd_lds(int N)
+// dynamic LDS bytes of dwt_forward_mix32 (32-degree tile and chunk states)
+static size_t fwd32_lds(int N)
 {
-    return sizeof(double) * (16 * FT_DS + 4 * 64 * ((N + 63) / 64));
+    return sizeof(double) * ((FT32_TD * FT32_DS + 1) / 2 + 4 * 64 * ((N + 63) / 64));
 }
 
 // the 16-byte fft-array accesses of both directions
@@ -938,8 +891,8 @@ hipError_t launch_dwt_forward_mixed(const DeviceTables &tb, int bw, int N, int n
 #define SO3_MFWD(TBV)                                                                                          \
     do {                                                                                                       \
         const int nunit = npair * ((T + TBV - 1) / TBV);                                                       \
-        hipLaunchKernelGGL((dwt_forward_mix<TBV>), dim3(nunit), dim3(64), fwd_lds(N), s, F, f_ts, coeff, c_ts, T,      \
-                           tb.w, tb.cosb, tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit);                 \
+        hipLaunchKernelGGL((dwt_forward_mix32<TBV>), dim3(nunit), dim3(64), fwd32_lds(N), s, F, f_ts, coeff, c_ts, \
+                           T, tb.w, tb.cosb, tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit);  \
     } while (0)
     // four transforms per wave from bw = 96 (more MFMA work per recurrence step), else two
     if (bw >= 96 && T >= 4) SO3_MFWD(4);
@@ -1009,13 +962,13 @@ hipError_t launch_dwt_forward_mixed_grouped(int TB, const float2 *F, float2 *coe
 {
     if (nitem <= 0) return hipSuccess;
     if (grid <= 0 || !fwd_aligned(F, 0)) return hipErrorInvalidValue;
-    const size_t lds = fwd_lds(2 * max_bw);
+    const size_t lds = fwd32_lds(2 * max_bw);
     if (TB == 4)
-        hipLaunchKernelGGL((dwt_forward_mix_grouped<4>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
+        hipLaunchKernelGGL((dwt_forward_mix32_grouped<4>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
     else if (TB == 2)
-        hipLaunchKernelGGL((dwt_forward_mix_grouped<2>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
+        hipLaunchKernelGGL((dwt_forward_mix32_grouped<2>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
     else if (TB == 1)
-        hipLaunchKernelGGL((dwt_forward_mix_grouped<1>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
+        hipLaunchKernelGGL((dwt_forward_mix32_grouped<1>), dim3(grid), dim3(64), lds, s, F, coeff, segs, items, nitem);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -1031,10 +984,10 @@ int mixed_grouped_occupancy(int inverse, int TB, int max_bw)
         if (TB == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_inverse_mix_grouped<2>, 64, lds);
         else if (TB == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_inverse_mix_grouped<1>, 64, lds);
     } else {
-        const size_t lds = fwd_lds(2 * max_bw);
-        if (TB == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<4>, 64, lds);
-        else if (TB == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<2>, 64, lds);
-        else if (TB == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix_grouped<1>, 64, lds);
+        const size_t lds = fwd32_lds(2 * max_bw);
+        if (TB == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix32_grouped<4>, 64, lds);
+        else if (TB == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix32_grouped<2>, 64, lds);
+        else if (TB == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dwt_forward_mix32_grouped<1>, 64, lds);
     }
     if (e != hipSuccess) {
         (void)hipGetLastError();

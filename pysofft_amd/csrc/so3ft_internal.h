@@ -84,8 +84,8 @@ hipError_t launch_risbo_reduced_step(const double *d1, double *h, double *d, int
 // recurrence constants of pair (p, q) for the mixed-precision kernels: entry i = (a, x, b)
 // of the step from degree q + i to q + i + 1, d^{l+1} = a (cos(beta) - x) d^l - b d^{l-1},
 // for 0 <= i < L - 1 (L = bw - q); the next REC_PAD entries are zero, so a chain run past
-// the last degree stays at zero with no per-degree test
-constexpr int REC_PAD = 16;
+// the last degree stays at zero with no per-degree test (up to a 32-degree tile past it)
+constexpr int REC_PAD = 32;
 
 struct FftDesc {
     int N = 0;
