@@ -48,15 +48,6 @@ __device__ __forceinline__ void rec_step(P rcp, int i, double cb, double &dcur, 
     dcur = dn;
 }
 
-// the same step with entry i given as (a, -a x, b)
-__device__ __forceinline__ void rec_step_ax(const double *c, int i, double cb, double &dcur, double &dprev)
-{
-    const double t = fma(c[3 * i], cb, c[3 * i + 1]);
-    const double dn = fma(t, dcur, -c[3 * i + 2] * dprev);
-    dprev = dcur;
-    dcur = dn;
-}
-
 // plan tables live in global memory; the grouped kernels read their pointers from the
 // segment table, so the address space is restated for the compiler (no flat accesses)
 template <class T> using gptr = const __attribute__((address_space(1))) T *;
@@ -112,12 +103,11 @@ constexpr int INV_DS = 72;  // padded row of the inverse hi tile (floats)
 constexpr int INV_LS = 80;  // padded row of the inverse lo table (4 bfloat16 per beta)
 constexpr int INV_TD = 8;   // degrees per tile: two K-steps of 4
 
-// dynamic LDS of the inverse: Dh [2][INV_TD][INV_DS] float | Dlo [4][INV_LS][2] uint32 |
-// cst [3 (bw - 1 + REC_PAD)] double
-constexpr int INV_LDS_D = (2 * INV_TD * INV_DS + 4 * INV_LS * 2) / 2;   // doubles before cst
-static size_t inv_lds(int bw)
+// dynamic LDS of the inverse: Dh [2][INV_TD][INV_DS] float | Dlo [4][INV_LS][2] uint32
+constexpr int INV_LDS_D = (2 * INV_TD * INV_DS + 4 * INV_LS * 2) / 2;   // doubles
+static size_t inv_lds(int)
 {
-    return sizeof(double) * (INV_LDS_D + 3 * (bw - 1 + REC_PAD));
+    return sizeof(double) * INV_LDS_D;
 }
 typedef float (*InvHi)[INV_TD][INV_DS];
 typedef uint32_t (*InvLo)[INV_LS][2];
@@ -126,7 +116,7 @@ typedef uint32_t (*InvLo)[INV_LS][2];
 // 64 chunk .. 64 chunk + 63.  MFMA columns j = variant j & 7 of transform t0 + (j >> 3),
 // real and imaginary parts in separate MFMAs; degrees go in tiles of 8, K-step s of a
 // tile taking degree l0 + 2 k + s in lane row k, so that a lane's coefficients of both
-// K-steps are adjacent in memory.  rc / rc_off: recurrence constants; cst / Dh / Dlo: the
+// K-steps are adjacent in memory.  rc / rc_off: recurrence constants; Dh / Dlo: the
 // block's LDS.
 template <int TB>
 __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, int64_t c_ts,
@@ -134,7 +124,7 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
                                              const double *__restrict__ cosb, const double *__restrict__ seeds,
                                              const double *__restrict__ rc, const int64_t *__restrict__ rc_off,
                                              const int64_t *__restrict__ slice, int bw, int N, int p, int q, int t0,
-                                             int chunk, double *cst, InvHi Dh, InvLo Dlo)
+                                             int chunk, InvHi Dh, InvLo Dlo)
 {
     static_assert(TB == 1 || TB == 2, "one transform pair per wave");
     const int lane = threadIdx.x;
@@ -155,15 +145,9 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
     const float2 *src = coeff + (int64_t)(t0 + min(tt, nt - 1)) * c_ts + offj;
 
     const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
-    // the pair's recurrence constants, staged in LDS (a read there has a short latency)
+    // the pair's recurrence constants: scalar loads (the same in every lane), each one
+    // operand of a step's instructions
     const const_dptr rcp = (const_dptr)rc + 3 * ((const_i64ptr)rc_off)[tri];
-    // as (a, -a x, b), so that a step's factor a (cos(beta) - x) is one fma
-    for (int i = lane; i < L - 1 + REC_PAD; i += 64) {
-        const double ra = rcp[3 * i];
-        cst[3 * i] = ra;
-        cst[3 * i + 1] = -ra * rcp[3 * i + 1];
-        cst[3 * i + 2] = rcp[3 * i + 2];
-    }
     double dcur = (beta < N) ? ((gptr<double>)seeds)[tri * N + beta] : 0.0;
     const double cb = (beta < N) ? ((gptr<double>)cosb)[beta] : 0.0;
     double dprev = 0.0;
@@ -176,8 +160,6 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
 #pragma unroll
         for (int r = 0; r < 4; ++r) dr[g][r] = di[g][r] = 0.0;
     }
-    __syncthreads();   // cst
-
     // coefficients of degrees l0 + 2k and l0 + 2k + 1 (re, im, re, im), each clamped into
     // the slice (past-the-end degrees enter with a zero sign); no branch, so that the
     // ring's loads stay in flight across tiles (a wait after a branch merge is for all)
@@ -197,10 +179,10 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
             float hi0, lo0, hi1, lo1;
             split(dcur, hi0, lo0);
             Dh[buf][2 * k][lane] = hi0;
-            rec_step_ax(cst, l0 + 2 * k, cb, dcur, dprev);
+            rec_step(rcp, l0 + 2 * k, cb, dcur, dprev);
             split(dcur, hi1, lo1);
             Dh[buf][2 * k + 1][lane] = hi1;
-            rec_step_ax(cst, l0 + 2 * k + 1, cb, dcur, dprev);
+            rec_step(rcp, l0 + 2 * k + 1, cb, dcur, dprev);
             Dlo[k][lane][h] = bf16x2_trunc(lo0, lo1);
         }
     };
@@ -244,11 +226,13 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
             }
         }
         // per K-step the hi products: eight independent accumulators between dependent
-        // MFMAs
+        // MFMAs; the first K-step of a flush block starts them from zero (an inline
+        // constant operand, no register clearing)
+        constexpr f4_t zero4 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-            ar[g] = mma32(a[0][g], br0, ar[g]);
-            ai[g] = mma32(a[0][g], bi0, ai[g]);
+            ar[g] = mma32(a[0][g], br0, h == 0 ? zero4 : ar[g]);
+            ai[g] = mma32(a[0][g], bi0, h == 0 ? zero4 : ai[g]);
         }
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
@@ -276,7 +260,8 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
     };
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
-    // FP32 sums of (at most) 16 degrees into the FP64 totals
+    // FP32 sums of (at most) 16 degrees into the FP64 totals (the next block's first
+    // MFMAs restart the FP32 sums)
     auto flush = [&]() {
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
@@ -285,7 +270,6 @@ __device__ __forceinline__ void inv_mix_item(const float2 *__restrict__ coeff, i
                 dr[g][r] += (double)ar[g][r];
                 di[g][r] += (double)ai[g][r];
             }
-            ar[g] = ai[g] = f4_t{0.f, 0.f, 0.f, 0.f};
         }
     };
 
@@ -346,17 +330,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dw
     const int64_t *__restrict__ rc_off, const int64_t *__restrict__ slice, const int32_t *__restrict__ pairs, int bw,
     int N, int npair, int nch, int nunit)
 {
-    extern __shared__ double smx[];   // hi tiles | lo table | recurrence constants
+    extern __shared__ double smx[];   // hi tiles | lo table
     const InvHi Dh = reinterpret_cast<InvHi>(smx);
     const InvLo Dlo = reinterpret_cast<InvLo>(smx + INV_TD * INV_DS);
-    double *cst = smx + INV_LDS_D;
     const int id = blockIdx.x;
     const int chunk = (id >> 3) % nch;
     const int unit = ((id >> 3) / nch) * 8 + (id & 7);  // chunks of a unit share an XCD
     if (unit >= nunit) return;
     const int pq = pairs[unit % npair];
     inv_mix_item<TB>(coeff, c_ts, F, f_ts, T, cosb, seeds, rc, rc_off, slice, bw, N, pq & 0xffff, pq >> 16,
-                     (unit / npair) * TB, chunk, cst, Dh, Dlo);
+                     (unit / npair) * TB, chunk, Dh, Dlo);
 }
 
 // grouped launch over transforms of several bandwidths: a persistent grid walks the
@@ -366,18 +349,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dw
     const float2 *__restrict__ coeff, float2 *__restrict__ F, const MixSegment *__restrict__ segs,
     const int4 *__restrict__ items, int nitem)
 {
-    extern __shared__ double smx[];   // sized for the group's largest bandwidth
+    extern __shared__ double smx[];
     const InvHi Dh = reinterpret_cast<InvHi>(smx);
     const InvLo Dlo = reinterpret_cast<InvLo>(smx + INV_TD * INV_DS);
-    double *cst = smx + INV_LDS_D;
     for (int it = blockIdx.x; it < nitem; it += gridDim.x) {
         const int4 w = items[it];
         if (w.x < 0) continue;
         const MixSegment sg = segs[w.x];
         __syncthreads();  // the previous item's LDS reads are done
         inv_mix_item<TB>(coeff + sg.coff, sg.nc, F + sg.goff, (int64_t)sg.N * sg.N * sg.N, sg.T, sg.cosb, sg.seeds,
-                         sg.rc, sg.rc_off, sg.slice, sg.bw, sg.N, w.y & 0xffff, w.y >> 16, w.z, w.w, cst, Dh,
-                         Dlo);
+                         sg.rc, sg.rc_off, sg.slice, sg.bw, sg.N, w.y & 0xffff, w.y >> 16, w.z, w.w, Dh, Dlo);
     }
 }
 
@@ -470,14 +451,10 @@ __device__ __forceinline__ void fwd_mix_item32(const float2 *__restrict__ F, int
             // accumulators are flushed after each half of the chunk's K-steps
 #pragma unroll
             for (int s4 = 0; s4 < 16; ++s4) {
-                if ((s4 & 7) == 0) {
-#pragma unroll
-                    for (int g = 0; g < TG; ++g)
-#pragma unroll
-                        for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-                            for (int u = 0; u < KS; ++u) ar[g][mt][u] = ai[g][mt][u] = f4_t{0.f, 0.f, 0.f, 0.f};
-                }
+                // the first K-step of each accumulator in a half starts it from zero (an
+                // inline constant operand, no register clearing)
+                const bool first = (s4 & 7) < KS;
+                constexpr f4_t zero4 = {0.f, 0.f, 0.f, 0.f};
                 const int col = 8 * (s4 >> 1) + 2 * kk + (s4 & 1);   // beta of this K-step in the chunk
                 const float a0 = Dt[j][col], a1 = Dt[16 + j][col];
                 const int u = s4 % KS;
@@ -486,10 +463,10 @@ __device__ __forceinline__ void fwd_mix_item32(const float2 *__restrict__ F, int
                 for (int g = 0; g < TG; ++g) {
                     const float4 v = raw[s4 >> 1][g];
                     const float bre = second ? v.z : v.x, bim = second ? v.w : v.y;
-                    ar[g][0][u] = mma32(a0, bre, ar[g][0][u]);
-                    ai[g][0][u] = mma32(a0, bim, ai[g][0][u]);
-                    ar[g][1][u] = mma32(a1, bre, ar[g][1][u]);
-                    ai[g][1][u] = mma32(a1, bim, ai[g][1][u]);
+                    ar[g][0][u] = mma32(a0, bre, first ? zero4 : ar[g][0][u]);
+                    ai[g][0][u] = mma32(a0, bim, first ? zero4 : ai[g][0][u]);
+                    ar[g][1][u] = mma32(a1, bre, first ? zero4 : ar[g][1][u]);
+                    ai[g][1][u] = mma32(a1, bim, first ? zero4 : ai[g][1][u]);
                 }
                 if (s4 & 1) {
                     load_b(cn, s4 >> 1);
