@@ -871,8 +871,9 @@ hipError_t launch_dwt_forward_mixed(const DeviceTables &tb, int bw, int N, int n
         hipLaunchKernelGGL((dwt_forward_mix32<TBV>), dim3(nunit), dim3(64), fwd32_lds(N), s, F, f_ts, coeff, c_ts, \
                            T, tb.w, tb.cosb, tb.seeds, tb.rc, tb.rc_off, tb.slice, tb.pairs, bw, N, npair, nunit);  \
     } while (0)
-    // four transforms per wave from bw = 96 (more MFMA work per recurrence step), else two
-    if (bw >= 96 && T >= 4) SO3_MFWD(4);
+    // four transforms per wave from bw = 64 (more MFMA work per recurrence step; measured
+    // 8% faster than two at bw = 64, batch 16), else two
+    if (bw >= 64 && T >= 4) SO3_MFWD(4);
     else if (T >= 2) SO3_MFWD(2);
     else SO3_MFWD(1);
 #undef SO3_MFWD

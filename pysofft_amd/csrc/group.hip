@@ -67,7 +67,13 @@ struct Group {
     std::vector<hipStream_t> side;
     hipEvent_t ev_fork = nullptr;
     std::vector<hipEvent_t> ev_join;
-    std::map<std::pair<const void *, void *>, hipGraphExec_t> ginv, gfwd;
+    // captured graphs per (input, output) pair, each with an event recorded after its last
+    // launch (an evicted graph is updated only once that launch has finished)
+    struct GExec {
+        hipGraphExec_t ex = nullptr;
+        hipEvent_t done = nullptr;
+    };
+    std::map<std::pair<const void *, void *>, GExec> ginv, gfwd;
 };
 
 std::mutex g_mu;
@@ -213,9 +219,9 @@ int set_items(Group &G)
     for (int k = 0; k < (int)G.segs.size(); ++k) {
         if (int rc = build_items(G, true, G.cfg.inv_tb, 1, v, k)) return rc;
         if (int rc = upload_items(v, G.inv_seg[k])) return rc;
-        // a segment's own forward launch takes four transforms per wave from bw = 96 (as the
+        // a segment's own forward launch takes four transforms per wave from bw = 64 (as the
         // single-bandwidth path), else two
-        const int tb = G.segs[k].bw >= 96 ? 4 : 2;
+        const int tb = G.segs[k].bw >= 64 ? 4 : 2;
         G.fwd_seg_tb[k] = tb;
         if (int rc = build_items(G, false, tb, 1, v, k)) return rc;
         if (int rc = upload_items(v, G.fwd_seg[k])) return rc;
@@ -225,8 +231,11 @@ int set_items(Group &G)
 
 void drop_graphs(Group &G)
 {
-    for (auto &kv : G.ginv) hipGraphExecDestroy(kv.second);
-    for (auto &kv : G.gfwd) hipGraphExecDestroy(kv.second);
+    for (auto *c : {&G.ginv, &G.gfwd})
+        for (auto &kv : *c) {
+            if (kv.second.ex) hipGraphExecDestroy(kv.second.ex);
+            if (kv.second.done) hipEventDestroy(kv.second.done);
+        }
     G.ginv.clear();
     G.gfwd.clear();
 }
@@ -365,18 +374,42 @@ int enqueue(Group &G, bool inverse, const void *in, void *out, hipStream_t s, bo
     return launch_fwd_dwt(G, G.cfg, G.fwd, fout, s);
 }
 
-int capture(Group &G, bool inverse, const void *in, void *out, hipGraphExec_t *exec)
+// capture the launch sequence for (in, out) and make it executable: into *exec by
+// instantiation, or, given a graph already instantiated for other buffers (reuse), by
+// updating that one's node arguments in place (the same launch sequence with other pointers
+// has the same topology), which avoids a new instantiation; reuse is consumed either way
+int capture(Group &G, bool inverse, const void *in, void *out, hipGraphExec_t *exec,
+            hipGraphExec_t reuse = nullptr)
 {
-    if (int rc = ensure_streams(G)) return rc;
-    GCHK(hipStreamBeginCapture(G.cap, hipStreamCaptureModeThreadLocal));
+    if (int rc = ensure_streams(G)) {
+        if (reuse) hipGraphExecDestroy(reuse);
+        return rc;
+    }
+    const hipError_t eb = hipStreamBeginCapture(G.cap, hipStreamCaptureModeThreadLocal);
+    if (eb != hipSuccess) {
+        if (reuse) hipGraphExecDestroy(reuse);
+        return set_error(SO3FT_EHIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(eb));
+    }
     const int rc = enqueue(G, inverse, in, out, G.cap, true);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(G.cap, &graph);
-    if (rc) {
+    if (rc || e != hipSuccess) {
         if (e == hipSuccess && graph) hipGraphDestroy(graph);
-        return rc;
+        if (reuse) hipGraphExecDestroy(reuse);
+        if (rc) return rc;
+        return set_error(SO3FT_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     }
-    if (e != hipSuccess) return set_error(SO3FT_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    if (reuse) {
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateSuccess;
+        if (hipGraphExecUpdate(reuse, graph, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+            hipGraphDestroy(graph);
+            *exec = reuse;
+            return SO3FT_OK;
+        }
+        (void)hipGetLastError();   // not updatable: instantiate a new one
+        hipGraphExecDestroy(reuse);
+    }
     const hipError_t ei = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
     hipGraphDestroy(graph);
     if (ei != hipSuccess) return set_error(SO3FT_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
@@ -500,12 +533,30 @@ int run(int64_t h, bool inverse, const void *in, void *out, void *stream)
     const auto key = std::make_pair(in, out);
     auto itg = cache.find(key);
     if (itg == cache.end()) {
+        // at most 8 graphs per direction; past that, the evicted graph is updated for the new
+        // buffers instead of instantiating another (callers passing fresh buffers every call
+        // keep the replay without an instantiation per call)
+        hipGraphExec_t evicted = nullptr;
+        hipEvent_t done = nullptr;
         if (cache.size() >= 8) {
-            hipGraphExecDestroy(cache.begin()->second);
+            evicted = cache.begin()->second.ex;
+            done = cache.begin()->second.done;
             cache.erase(cache.begin());
+            const hipError_t ew = hipEventSynchronize(done);   // its last launch has finished
+            if (ew != hipSuccess) {
+                hipGraphExecDestroy(evicted);
+                hipEventDestroy(done);
+                return set_error(SO3FT_EHIP, std::string("hipEventSynchronize: ") + hipGetErrorString(ew));
+            }
+        } else {
+            GCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
         }
         hipGraphExec_t ex = nullptr;
         int &sc = G->sched[inverse ? 1 : 0];
+        if (sc < 0 && evicted) {   // not reached: a full cache implies a chosen schedule
+            hipGraphExecDestroy(evicted);
+            evicted = nullptr;
+        }
         if (sc < 0) {
             // time both schedules' graphs on the caller's buffers (the output is rewritten by
             // the launch below) and keep the faster
@@ -544,16 +595,22 @@ int run(int64_t h, bool inverse, const void *in, void *out, void *stream)
             hipEventDestroy(e1);
             if (rc) {
                 if (ex) hipGraphExecDestroy(ex);
+                hipEventDestroy(done);
                 sc = -1;
                 return rc;
             }
             sc = G->sched_ms[inverse ? 1 : 0][1] < G->sched_ms[inverse ? 1 : 0][0] ? 1 : 0;
-        } else if (int rc = capture(*G, inverse, in, out, &ex)) {
+        } else if (int rc = capture(*G, inverse, in, out, &ex, evicted)) {
+            hipEventDestroy(done);
             return rc;
         }
-        itg = cache.emplace(key, ex).first;
+        Group::GExec ge;
+        ge.ex = ex;
+        ge.done = done;
+        itg = cache.emplace(key, ge).first;
     }
-    GCHK(hipGraphLaunch(itg->second, s));
+    GCHK(hipGraphLaunch(itg->second.ex, s));
+    GCHK(hipEventRecord(itg->second.done, s));
     return SO3FT_OK;
 }
 

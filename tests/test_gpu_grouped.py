@@ -106,6 +106,35 @@ def test_grouped_graph_replays_match_eager():
         assert torch.equal(c2, ce), r
 
 
+def test_grouped_graph_fresh_buffers_every_call():
+    """Fresh output buffers on every call (out=None, the tensors kept alive so that every
+    address is new): past eight cached graphs per direction the evicted graph is updated for
+    the new buffers; every result equals the eager launches."""
+    from pysofft_amd.grouped import SoftGroup
+    g = SoftGroup(BWS_SMALL)
+    c = _coeffs(g, 4)
+    g.set_graphs(False)
+    fe = g.isoft(c).clone()
+    ce = g.soft(fe).clone()
+    g.set_graphs(True)
+    keep = []
+    for r in range(12):
+        f = g.isoft(c)
+        c2 = g.soft(f)
+        keep.append((f, c2))
+        torch.cuda.synchronize()
+        assert torch.equal(f, fe), r
+        assert torch.equal(c2, ce), r
+    # the earliest buffers again: their graphs were evicted, so these calls update or capture anew
+    for f, c2 in keep[:3]:
+        f.zero_()
+        c2.fill_(1.0)
+        g.isoft(c, out=f)
+        g.soft(f, out=c2)
+        torch.cuda.synchronize()
+        assert torch.equal(f, fe) and torch.equal(c2, ce)
+
+
 def test_grouped_any_bandwidth_vs_fp64_and_single_path():
     """Bandwidths that are not 2^a or 3 * 2^a (FFT lengths 10, 80, 110 take the Stockham
     pass) in one group with a register-FFT bandwidth: equal to the single-bandwidth path

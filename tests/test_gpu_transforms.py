@@ -332,11 +332,12 @@ def test_dwt_engines_agree_with_oracle(bw, n):
 def test_mixed_c64_vs_oracle(bw):
     """complex64 transforms (FP32 MFMA contraction, FP64 recurrence and FFTs) against the
     FP64 oracle on the same complex64-rounded inputs: per-transform relative L2 <= 1e-7
-    for isoft, soft and the round trip."""
+    for isoft, soft and the round trip.  Five transforms: at bw = 64 the forward runs four
+    per wave and a one-transform tail."""
     rng = np.random.default_rng(500 + bw)
     s = _soft(bw)
     P = O.OraclePlan(bw)
-    n = 3
+    n = 5
     c = _rand_c(rng, (n, P.nc)).astype(np.complex64)
     f = np.asarray(s.isoft_many_c64(c))
     assert f.dtype == np.complex64
@@ -356,7 +357,7 @@ def test_mixed_c64_large_bw_vs_fp64(bw, n):
     """complex64 transforms at bandwidths beyond the oracle's reach (FFT lengths 192, 256,
     512) against the FP64 path, itself pinned to the oracle at smaller bandwidths and to
     direct sums at bw = 96 and 128: relative L2 <= 1e-7 per transform for isoft and soft.
-    n = 5 runs the forward with four transforms per wave (bw >= 96) and its one-transform
+    n = 5 runs the forward with four transforms per wave (bw >= 64) and its one-transform
     tail; n = 2 and 1 the two- and one-transform waves."""
     N = 2 * bw
     s = _soft(bw)
