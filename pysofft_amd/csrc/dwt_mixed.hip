@@ -1,9 +1,10 @@
 // Mixed-precision discrete Wigner transforms: complex64 coefficients and fft arrays,
-// FP64 Kostelec recurrence (pysofft/wigner.f90:227-260), contraction on the matrix cores
-// with d split into FP32 hi + lo parts: the hi products on v_mfma_f32_16x16x4f32 (exact
-// FP32 operands), the lo products, four K-steps at a time, on v_mfma_f32_16x16x16_bf16
-// (they are below 2^-24 of the result, so bfloat16 operands suffice); FP32 accumulation
-// over at most 16 terms and FP64 accumulation across those blocks.  Work unit: an order pair
+// FP64 Kostelec recurrence (pysofft/wigner.f90:227-260), contraction on the matrix cores.
+// The inverse splits d into FP32 hi + lo parts: the hi products on v_mfma_f32_16x16x4f32
+// (exact FP32 operands), the lo products, four K-steps at a time, on
+// v_mfma_f32_16x16x16_bf16 (they are below 2^-24 of the result, so bfloat16 operands
+// suffice); the forward uses the FP32 value of w d.  FP32 accumulation over at most 16
+// terms and FP64 accumulation across those blocks.  Work unit: an order pair
 // 0 <= p <= q with its up to 8 symmetric variants (pysofft/soft.f08:534-631, 937-1023);
 // the MFMA N dimension is 16 columns = 8 variants x 2 transforms, real and imaginary
 // parts in separate MFMAs, so a lane always loads or stores whole complex values.
@@ -18,9 +19,9 @@
 //   them back as the A operands of four 16-beta tiles (M = beta, K = degree); B = the
 //   coefficients (K = degree, N = columns).
 // forward: one wave per (pair, TB transforms), lane = beta within 64-beta chunks; for
-//   each 16-degree tile and chunk the lanes advance their recurrence and store
-//   w(beta) d (hi, lo) in LDS as [degree][beta]; A = that tile (M = degree, K = beta),
-//   B = the fft-array samples (K = beta, N = columns).
+//   each 32-degree tile and chunk the lanes advance their recurrence and store
+//   w(beta) d in LDS as [degree][beta]; A = that tile's two 16-degree halves (M = degree,
+//   K = beta), B = the fft-array samples (K = beta, N = columns).
 #include "so3ft_internal.h"
 
 #include <type_traits>
