@@ -21,6 +21,8 @@ struct FftRegK {
     int mask_rows, mask_outer, in_real, out_real, sign;
     int col0, ncols;   // beta column window [col0, col0+ncols) (slab processing)
     double scale;
+    double *pv;        // peak-search partials of the fused pass (fft_reg_lean<..., true>)
+    int64_t *pi;
 };
 
 __device__ __forceinline__ bool order_ok2(int i, int N, int bw, int lmax)
@@ -267,7 +269,11 @@ hipError_t launch_copy16(const void *a, void *b, int64_t nvec, hipStream_t s)
 // share a CU (the kernel above holds two).  Measured faster for complex128 passes (11% at
 // N = 256, 31% at N = 128) and complex64 passes up to N = 192 (10% at N = 128, 2% at
 // N = 192); the prefetching kernel stays ahead for complex64 at N = 256.
-template <int R1, int R2, class TI, class TO>
+//
+// AMAX: the pass's output is not stored; each block leaves the largest |out|^2 of its tile
+// and that element's C-order index within the transform (the smallest on ties) in
+// k.pv / k.pi [transform][block], for a peak search that never writes the grid.
+template <int R1, int R2, class TI, class TO, bool AMAX = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void fft_reg_lean(
     const void *__restrict__ in, void *__restrict__ out, const double2 *__restrict__ twg, FftRegK k)
 {
@@ -323,12 +329,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
         for (int k1 = 0; k1 < R1; ++k1) Yd[(k1 * R2 + grp) * C + c] = v[k1].y;
     }
     __syncthreads();
+    double bv = -1.0;
+    int64_t bi = INT64_MAX;
     if (grp < R1) {
         const int k1 = grp;
 #pragma unroll
         for (int n2 = 0; n2 < R2; ++n2) v[n2] = make_double2(re[n2], Yd[(k1 * R2 + n2) * C + c]);
         Dft<R2>::run(v, tw, N / R2, k.sign);
-        if (colok) {
+        if (AMAX && colok) {
+            const int64_t ib = o * k.out_o + cb + c + (int64_t)k1 * k.out_r;
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) {
+                const double2 x = make_double2(v[k2].x * k.scale, v[k2].y * k.scale);
+                amax_better(bv, bi, x.x * x.x + x.y * x.y, ib + (int64_t)(R1 * k2) * k.out_r);
+            }
+        } else if (colok) {
             const int64_t ob = t * k.out_t + o * k.out_o + cb + c + (int64_t)k1 * k.out_r;
             const uint32_t ostep = (uint32_t)(R1 * k.out_r);
             TO *po = reinterpret_cast<TO *>(out) + ob;
@@ -340,6 +355,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
                 if (k.out_real) pr[off] = (typename Scalar<TO>::type)x.x;
                 else st<TO>(po, off, x);
             }
+        }
+    }
+    if constexpr (AMAX) {
+        __shared__ double sv[16];
+        __shared__ int64_t si[16];
+        amax_block_reduce(bv, bi, sv, si);
+        if (threadIdx.x == 0) {
+            const int64_t nb = (int64_t)gridDim.x * gridDim.y;
+            k.pv[t * nb + (int64_t)blockIdx.y * gridDim.x + blockIdx.x] = bv;
+            k.pi[t * nb + (int64_t)blockIdx.y * gridDim.x + blockIdx.x] = bi;
         }
     }
 }
@@ -372,6 +397,22 @@ bool fft_reg_supported(int N)
     return split_pow2(N, a, b);
 }
 
+static FftRegK make_k(int N, int R1, int bw, int lmax, const FftAxis &ax, int col0, int ncols)
+{
+    FftRegK k{};
+    k.N = N;
+    k.C = (R1 >= 24) ? 8 : 16;
+    if (k.C > N) k.C = N;
+    k.bw = bw; k.lmax = lmax;
+    k.in_t = ax.in_t; k.in_o = ax.in_o; k.in_r = ax.in_r;
+    k.out_t = ax.out_t; k.out_o = ax.out_o; k.out_r = ax.out_r;
+    k.mask_rows = ax.mask_rows; k.mask_outer = ax.mask_outer;
+    k.in_real = ax.in_real; k.out_real = ax.out_real; k.sign = ax.sign; k.scale = ax.scale;
+    k.col0 = col0; k.ncols = ncols;
+    k.pv = nullptr; k.pi = nullptr;
+    return k;
+}
+
 // the one-tile kernel exists for N <= 256 only (larger lengths would spill)
 template <int R1, int R2, class TI, class TO>
 static void launch_lean(dim3 grid, dim3 block, int bytes, hipStream_t s, const void *in, void *out,
@@ -386,16 +427,7 @@ static hipError_t launch_reg_typed(int N, const double2 *tw, int bw, int lmax, c
 {
     int R1 = 0, R2 = 0;
     if (!split_pow2(N, R1, R2)) return hipErrorInvalidValue;
-    FftRegK k;
-    k.N = N;
-    k.C = (R1 >= 24) ? 8 : 16;
-    if (k.C > N) k.C = N;
-    k.bw = bw; k.lmax = lmax;
-    k.in_t = ax.in_t; k.in_o = ax.in_o; k.in_r = ax.in_r;
-    k.out_t = ax.out_t; k.out_o = ax.out_o; k.out_r = ax.out_r;
-    k.mask_rows = ax.mask_rows; k.mask_outer = ax.mask_outer;
-    k.in_real = ax.in_real; k.out_real = ax.out_real; k.sign = ax.sign; k.scale = ax.scale;
-    k.col0 = col0; k.ncols = ncols;
+    const FftRegK k = make_k(N, R1, bw, lmax, ax, col0, ncols);
     const int threads = (R1 > R2 ? R1 : R2) * k.C;
     const int bytes = (int)sizeof(double2) * (N + N * k.C);
     // complex64 input: each block walks up to 8 column tiles (prefetching the next; a
@@ -455,6 +487,53 @@ hipError_t launch_fft_reg(int N, const double2 *tw, int bw, int lmax, const void
     if (in_c64 && out_c64) return launch_reg_typed<float2, float2>(N, tw, bw, lmax, in, out, ax, T, col0, ncols, s);
     if (in_c64) return launch_reg_typed<float2, double2>(N, tw, bw, lmax, in, out, ax, T, col0, ncols, s);
     return launch_reg_typed<double2, float2>(N, tw, bw, lmax, in, out, ax, T, col0, ncols, s);
+}
+
+// the fused pass exists where the one-tile complex128 kernel does (N = 2^a or 3 * 2^a up to
+// 256), for whole lines (no outer mask) into a complex128 result
+int fft_reg_argmax_partials(int N)
+{
+    int R1 = 0, R2 = 0;
+    if (!split_pow2(N, R1, R2) || N > 256) return 0;
+    const int C = std::min((R1 >= 24) ? 8 : 16, N);
+    return (N + C - 1) / C * N;
+}
+
+hipError_t launch_fft_reg_argmax(int N, const double2 *tw, int bw, int lmax, const double2 *in, const FftAxis &ax,
+                                 int T, double *pv, int64_t *pi, int *nblk, hipStream_t s)
+{
+    int R1 = 0, R2 = 0;
+    if (!split_pow2(N, R1, R2) || N > 256 || ax.in_real || ax.out_real || ax.mask_outer || !pv || !pi)
+        return hipErrorNotSupported;
+    FftRegK k = make_k(N, R1, bw, lmax, ax, 0, N);
+    k.pv = pv;
+    k.pi = pi;
+    const int ntile = (N + k.C - 1) / k.C;
+    *nblk = ntile * N;
+    const dim3 grid(ntile, N, T), block((R1 > R2 ? R1 : R2) * k.C);
+    const int bytes = (int)(sizeof(double2) * N + sizeof(double) * N * k.C);
+#define SO3_AMAX(A, B)                                                                                       \
+    do {                                                                                                    \
+        if constexpr (A * B <= 256)                                                                         \
+            hipLaunchKernelGGL((fft_reg_lean<A, B, double2, double2, true>), grid, block, bytes, s, in, nullptr, tw, k); \
+    } while (0)
+    switch (N) {
+    case 4: SO3_AMAX(2, 2); break;
+    case 6: SO3_AMAX(3, 2); break;
+    case 8: SO3_AMAX(4, 2); break;
+    case 12: SO3_AMAX(4, 3); break;
+    case 16: SO3_AMAX(4, 4); break;
+    case 24: SO3_AMAX(6, 4); break;
+    case 32: SO3_AMAX(8, 4); break;
+    case 48: SO3_AMAX(8, 6); break;
+    case 64: SO3_AMAX(8, 8); break;
+    case 96: SO3_AMAX(12, 8); break;
+    case 128: SO3_AMAX(16, 8); break;
+    case 192: SO3_AMAX(16, 12); break;
+    default: SO3_AMAX(16, 16); break;
+    }
+#undef SO3_AMAX
+    return hipGetLastError();
 }
 
 }  // namespace so3

@@ -348,17 +348,31 @@ int check_ptrs(const void *a, const void *b)
 }
 
 // F (i_n,i_m,beta) -> SO(3) function: FFT along m (in place), then along n with 1/(2 pi)
-int inverse_fft_passes(Plan *P, double2 *F, void *out, int out_real, int64_t T, hipStream_t s)
+// the inverse transform's two passes: along m (alpha) in place, then along n (gamma)
+// with the 1/(2 pi) of the inverse (soft.f08:1896-1906)
+static FftAxis inverse_alpha_axis(int64_t N)
 {
-    const int64_t N = P->N, N2 = N * N, N3 = N2 * N;
+    const int64_t N2 = N * N, N3 = N2 * N;
     FftAxis a{};
     a.in_t = N3; a.in_o = N2; a.in_r = N; a.out_t = N3; a.out_o = N2; a.out_r = N;
     a.mask_rows = 1; a.mask_outer = 1; a.sign = -1; a.scale = 1.0;
-    HIPCHK(launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, F, F, a, (int)T, s));
+    return a;
+}
+
+static FftAxis inverse_gamma_axis(int64_t N, int out_real)
+{
+    const int64_t N2 = N * N, N3 = N2 * N;
     FftAxis b{};
     b.in_t = N3; b.in_o = N; b.in_r = N2; b.out_t = N3; b.out_o = N; b.out_r = N2;
     b.mask_rows = 1; b.mask_outer = 0; b.out_real = out_real; b.sign = -1;
     b.scale = 1.0 / (2.0 * 3.14159265358979323846);
+    return b;
+}
+
+int inverse_fft_passes(Plan *P, double2 *F, void *out, int out_real, int64_t T, hipStream_t s)
+{
+    const FftAxis a = inverse_alpha_axis(P->N), b = inverse_gamma_axis(P->N, out_real);
+    HIPCHK(launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, F, F, a, (int)T, s));
     HIPCHK(launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, F, out, b, (int)T, s));
     return SO3FT_OK;
 }
@@ -1019,7 +1033,10 @@ int so3ft_cross_correlation_argmax_many(so3ft_handle h, const void *f_lms, const
     const int64_t ch = chunk_of(P, n);
     int rc = ensure_scratch(P, ch);
     if (rc) return rc;
-    const int64_t need = ch * argmax_partials(N3);
+    // the gamma pass fused with the peak search where that shape exists: the correlation
+    // grids are never written (one grid write and one grid read less per pair)
+    const int nfused = fft_reg_supported((int)P->N) ? fft_reg_argmax_partials((int)P->N) : 0;
+    const int64_t need = ch * std::max<int64_t>(argmax_partials(N3), nfused);
     if (P->amax_cap < need) {
         hipFree(P->amax_v); hipFree(P->amax_i);
         P->amax_v = nullptr; P->amax_i = nullptr; P->amax_cap = 0;
@@ -1036,9 +1053,19 @@ int so3ft_cross_correlation_argmax_many(so3ft_handle h, const void *f_lms, const
         const double2 *f = reinterpret_cast<const double2 *>(f_lms) + i0 * nb;
         const double2 *g = reinterpret_cast<const double2 *>(g_lms) + i0 * nb;
         if ((rc = dwt_inverse(P, DWT_CORR, f, nb, g, P->scratch, N3, T, s))) return rc;
-        if ((rc = inverse_fft_passes(P, P->scratch, P->scratch, 0, T, s))) return rc;
-        HIPCHK(launch_argmax_abs(P->scratch, N3, (int)T, P->amax_v, P->amax_i, reinterpret_cast<int64_t *>(idx) + i0,
-                                 val ? reinterpret_cast<double *>(val) + i0 : nullptr, s));
+        int64_t *ix = reinterpret_cast<int64_t *>(idx) + i0;
+        double *vx = val ? reinterpret_cast<double *>(val) + i0 : nullptr;
+        if (nfused > 0) {
+            const FftAxis a = inverse_alpha_axis(P->N), b = inverse_gamma_axis(P->N, 0);
+            HIPCHK(launch_fft_axis(P->fft, P->tb.tw, (int)P->bw, (int)P->lmax, P->scratch, P->scratch, a, (int)T, s));
+            int nblk = 0;
+            HIPCHK(launch_fft_reg_argmax((int)P->N, P->tb.tw, (int)P->bw, (int)P->lmax, P->scratch, b, (int)T,
+                                         P->amax_v, P->amax_i, &nblk, s));
+            HIPCHK(launch_argmax_final(P->amax_v, P->amax_i, nblk, (int)T, ix, vx, s));
+        } else {
+            if ((rc = inverse_fft_passes(P, P->scratch, P->scratch, 0, T, s))) return rc;
+            HIPCHK(launch_argmax_abs(P->scratch, N3, (int)T, P->amax_v, P->amax_i, ix, vx, s));
+        }
     }
     return SO3FT_OK;
 }

@@ -159,6 +159,42 @@ hipError_t launch_fft_axis(const FftDesc &fd, const double2 *tw, int bw, int lma
                            void *out, const FftAxis &ax, int T, hipStream_t s);
 
 bool fft_reg_supported(int N);
+// (value, index) of a peak search: the larger value wins, the smaller index breaks ties
+__device__ __forceinline__ void amax_better(double &v, int64_t &i, double v2, int64_t i2)
+{
+    if (v2 > v || (v2 == v && i2 < i)) {
+        v = v2;
+        i = i2;
+    }
+}
+// block-wide reduction of every thread's (v, i) into thread 0's (blocks of up to 1024
+// threads; sv / si hold one entry per wave; every thread of the block calls it)
+__device__ __forceinline__ void amax_block_reduce(double &v, int64_t &i, double *sv, int64_t *si)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double v2 = __shfl_down(v, o, 64);
+        const int64_t i2 = __shfl_down(i, o, 64);
+        amax_better(v, i, v2, i2);
+    }
+    if (lane == 0) { sv[wid] = v; si[wid] = i; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)((blockDim.x + 63) >> 6); ++w) amax_better(v, i, sv[w], si[w]);
+    }
+}
+// per transform, the best of nblk partial (|z|^2, index) pairs: idx[t], and sqrt of the value
+// into val[t] (val may be null)
+hipError_t launch_argmax_final(const double *pv, const int64_t *pi, int nblk, int T, int64_t *idx, double *val,
+                               hipStream_t s);
+// FP64 pass along one axis (complex128 in, complex128 not written) fused with the peak
+// search of |out|^2: each block leaves its best (|z|^2, C-order index within the transform)
+// in pv / pi [T][*nblk].  hipErrorNotSupported where no fused shape exists (run the pass and
+// launch_argmax_abs instead).  Partials per transform: fft_reg_argmax_partials(N).
+hipError_t launch_fft_reg_argmax(int N, const double2 *tw, int bw, int lmax, const double2 *in, const FftAxis &ax,
+                                 int T, double *pv, int64_t *pi, int *nblk, hipStream_t s);
+int fft_reg_argmax_partials(int N);
 // the LDS Stockham pass for any N (mixed radices), complex64 or complex128 storage
 hipError_t launch_fft_stockham(int N, const double2 *tw, int bw, int lmax, const void *in, void *out,
                                const FftAxis &ax, int T, int in_c64, int out_c64, int col0, int ncols, hipStream_t s);

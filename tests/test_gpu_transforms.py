@@ -438,6 +438,29 @@ def test_correlation_peaks_find_grid_rotations():
     np.testing.assert_allclose(val, np.abs(cc).reshape(n, -1).max(1), rtol=1e-14)
 
 
+@pytest.mark.parametrize("bw", [5, 6, 24, 64, 96])
+def test_correlation_peaks_equal_full_grid_argmax(bw):
+    """The peak search (the last FFT pass fused with the search where that pass has a
+    one-tile shape: 2bw = 12, 48, 128, 192 here; the stored grid and a separate search at
+    2bw = 10) returns, per pair, the first C-order index of max |C| over the full
+    correlation grid and that maximum, for random (not rotated) pairs."""
+    rng = np.random.default_rng(900 + bw)
+    s = _soft(bw)
+    n = 3 if bw < 96 else 2
+    f = rng.standard_normal((n, bw * bw)) + 1j * rng.standard_normal((n, bw * bw))
+    g = rng.standard_normal((n, bw * bw)) + 1j * rng.standard_normal((n, bw * bw))
+    ft, gt = torch.from_numpy(f).cuda(), torch.from_numpy(g).cuda()
+    peak, val = s.correlation_peaks(ft, gt)
+    cc = s.cross_correlation_ylm_cmplx_many(ft, gt)
+    m = cc.abs().reshape(n, -1)
+    flat = m.argmax(1)
+    N = 2 * bw
+    ig, rem = flat // (N * N), flat % (N * N)
+    ref = torch.stack([rem // N, rem % N, ig], 1)
+    assert torch.equal(peak, ref)
+    torch.testing.assert_close(val, m.max(1).values, rtol=1e-14, atol=0)
+
+
 def test_workload_bw256_real_round_trip():
     """Real FP64 transform at bw=256: Hermitian-symmetric coefficients (iid N(0,1) re/im
     scaled by (2l+1)^-1/2), irsoft onto the 512^3 float64 grid, rsoft back: relative L2
