@@ -61,14 +61,14 @@ __device__ inline void refine_window(const double *ein, int64_t s, int Bg, int (
 //   quarter 1 (X-): v3 (-p, -q) at beta, v5 (-p, q) at pi - beta; rows m = -p
 //   quarter 2 (Y+): v2 (q, p) at beta,  v7 (-q, p) at pi - beta;  rows n = +p
 //   quarter 3 (Y-): v1 (-q, -p) at beta, v6 (q, -p) at pi - beta; rows n = -p
-// X quarters fold F(beta, m, n) over n with e^{-i n gamma_c} into R[beta][p][c], Y
-// quarters over m with e^{-i m alpha_a} into R[beta][p][a]; the remaining sum over the
+// X quarters fold F(beta, m, n) over n with e^{-i n gamma_c} into R[beta][c][p], Y
+// quarters over m with e^{-i m alpha_a} into R[beta][a][p]; the remaining sum over the
 // row index follows at the end.  Work items are the order pairs in q-major order: the
 // lanes of a wave run recurrences of equal length (Lc - q) and write distinct rows p.
 // Each item runs the recurrence at the 5 window betas and at pi - beta (10 chains) and
 // accumulates F over l in registers.  The quarter stages in LDS (FP64) the coefficient
 // set indexed by +-p on its side only (f for X, g for Y) and the other set in full.
-// dynamic LDS: R [RW b][Lc p][RW k] | ep [RW k][2Lc-1] fold phases | hs [Lc(Lc+1)/2]
+// dynamic LDS: R [RW b][RW k][Lc p] | ep [RW k][2Lc-1] fold phases | hs [Lc(Lc+1)/2]
 // (the +-p side set) | fs [Lc^2] (the other set); hs / fs hold at least RW Lc entries:
 // the row phases of the last phase reuse them
 __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g,
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
     constexpr int W = RW;
     const int NM = 2 * Lc - 1;
     extern __shared__ double sm[];
-    double2 *R = reinterpret_cast<double2 *>(sm);     // [b][p][k]
+    double2 *R = reinterpret_cast<double2 *>(sm);     // [b][k][p]: p fastest, so the lanes' rows p hit distinct banks
     double2 *ep = R + W * Lc * W;                      // [k][j + Lc - 1]: e^{-i j gamma_k} (X) / e^{-i j alpha_k} (Y)
     double2 *hs = ep + W * NM;                         // side set at +-p: [l(l+1)/2 + p]
     double2 *fs = hs + Lc * (Lc + 1) / 2;              // other set: [l(l+1) + j]
@@ -96,7 +96,9 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
     __shared__ double lcb[2 * W], lsb[2 * W], cbv[2 * W];  // log cos(b/2), log sin(b/2), cos b; b, pi - b
     __shared__ double bet[W];
     __shared__ int win[3][W];
+    __shared__ int qnext;                              // next unclaimed work item (wave queue)
     const int tid = threadIdx.x;
+    if (tid == 0) qnext = 0;
     const double pi = 3.14159265358979323846;
     const double h = pi / Bg;
     if (tid == 0) {
@@ -146,13 +148,16 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
     const int vb = ytype ? (neg ? 6 : 7) : (neg ? 5 : 4);
     const double cb[2 * W] = {cbv[0], cbv[1], cbv[2], cbv[3], cbv[4], cbv[5], cbv[6], cbv[7], cbv[8], cbv[9]};
     const int npq = Lc * (Lc + 1) / 2;
-    for (int P = tid; P < npq; P += blockDim.x) {
+    // items in q-major order, i.e. by decreasing recurrence length; each wave claims the next
+    // 64 from a queue in LDS, so the waves end together (longest items first) instead of the
+    // waves holding the long items setting the block's time
+    auto item = [&](int P) {
         int q = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
         while (q * (q + 1) / 2 > P) --q;
         while ((q + 1) * (q + 2) / 2 <= P) ++q;
         const int p = P - q * (q + 1) / 2;
         const Variant v0 = variant(va, p, q), v1 = variant(vb, p, q);
-        if (!v0.valid && !v1.valid) continue;
+        if (!v0.valid && !v1.valid) return;
         // seeds d~^q_{p,q} at beta and pi - beta (closed form, log domain)
         const double lc = 0.5 * (lfac[2 * q] - lfac[q + p] - lfac[q - p]) + lq[q];
         double d[2 * W], dp[2 * W];
@@ -219,7 +224,7 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
             step(B, l + 1);
         }
         if (l < Lc) step(A, l);
-        // fold the other index's phase into R[b][p][k] (distinct p across the lanes)
+        // fold the other index's phase into R[b][k][p] (distinct p across the lanes of one q)
         auto fold = [&](int jj, const double2 (&Fk)[W]) {
 #pragma unroll 1
             for (int k = 0; k < W; ++k) {
@@ -227,7 +232,7 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
 #pragma unroll
                 for (int bb = 0; bb < W; ++bb) {
                     const double2 z = Fk[bb];
-                    double2 *dst = &R[(bb * Lc + p) * W + k];
+                    double2 *dst = &R[(bb * W + k) * Lc + p];
                     atomicAdd(&dst->x, z.x * e.x - z.y * e.y);
                     atomicAdd(&dst->y, z.x * e.y + z.y * e.x);
                 }
@@ -235,10 +240,18 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
         };
         if (v0.valid) fold(j0, F0);
         if (v1.valid) fold(j1, F1);
+    };
+    const int lane = tid & 63;
+    for (;;) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&qnext, 64);
+        base = __shfl(base, 0, 64);
+        if (base >= npq) break;
+        if (base + lane < npq) item(base + lane);
     }
     __syncthreads();
-    // the row sums: X quarters sum_p R[b][p][c] e^{-i m alpha_a} (m = +-p), Y quarters
-    // sum_p R[b][p][a] e^{-i n gamma_c} (n = +-p); the row phases reuse the hs space
+    // the row sums: X quarters sum_p R[b][c][p] e^{-i m alpha_a} (m = +-p), Y quarters
+    // sum_p R[b][a][p] e^{-i n gamma_c} (n = +-p); the row phases reuse the hs space
     double2 *er = hs;   // [k'][p]
     const double sgn = neg ? 1.0 : -1.0;
     for (int i = tid; i < W * Lc; i += blockDim.x) {
@@ -253,7 +266,7 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
         const int kf = ytype ? a : c, kr = ytype ? c : a;   // fold index in R, row-phase index
         double re = 0.0, im = 0.0;
         for (int pr = 0; pr < Lc; ++pr) {
-            const double2 z = R[(b * Lc + pr) * W + kf], e = er[kr * Lc + pr];
+            const double2 z = R[(b * W + kf) * Lc + pr], e = er[kr * Lc + pr];
             re += z.x * e.x - z.y * e.y;
             im += z.x * e.y + z.y * e.x;
         }
