@@ -460,19 +460,52 @@ __device__ __forceinline__ double2 sh_coeff(const double2 *__restrict__ x, int l
 // branches are this formula with the negative orders completed).  Transform t = blockIdx.y
 // reads its sets at t * t_ts; nr shells r at stride r_ts are summed with weights w (w ==
 // nullptr: one shell of weight 1).  Degrees l > lcap are zero (the Risbo loops of
-// soft.f08:2564-2580, 3046-3074 stop at l = lmax).  One block per (m, n).
-__global__ void corr_coeffs_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g, int64_t t_ts,
-                                   int layout, int nr, int64_t r_ts, const double *__restrict__ w,
-                                   double2 *__restrict__ out, int64_t c_ts, const int64_t *__restrict__ slice,
-                                   int bw, int lcap)
+// soft.f08:2564-2580, 3046-3074 stop at l = lmax).  One block per order pair and transform.
+__global__ __launch_bounds__(64) void corr_coeffs_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g,
+                                                         int64_t t_ts, int layout, int nr, int64_t r_ts,
+                                                         const double *__restrict__ w, double2 *__restrict__ out,
+                                                         int64_t c_ts, const int64_t *__restrict__ slice, int bw,
+                                                         int lcap)
 {
-    const int s = blockIdx.x;
-    const int m = s / (2 * bw - 1) - (bw - 1), n = s % (2 * bw - 1) - (bw - 1);
-    const int lmin = max(m < 0 ? -m : m, n < 0 ? -n : n);
-    const int64_t base = slice[s];
+    // block = order pair 0 <= m1 <= m2 < bw (triangular index blockIdx.x, m1 outer) of
+    // transform blockIdx.y: its variant slices are contiguous in the mnl layout, so the
+    // block's lanes write consecutive entries
+    const int P = blockIdx.x;
     const int64_t t = blockIdx.y;
-    const double sg = ((m + n) & 1) ? -1.0 : 1.0;
-    for (int l = lmin + (int)threadIdx.x; l < bw; l += blockDim.x) {
+    const double B2 = 2.0 * bw + 1.0;
+    int m1 = (int)((B2 - sqrt(B2 * B2 - 8.0 * P)) * 0.5);
+    m1 = max(0, min(m1, bw - 1));
+    while (m1 > 0 && m1 * (2 * bw - m1 + 1) / 2 > P) --m1;
+    while (m1 + 1 < bw && (m1 + 1) * (2 * bw - m1) / 2 <= P) ++m1;
+    const int m2 = m1 + (P - m1 * (2 * bw - m1 + 1) / 2);
+    const int L = bw - m2;
+    // the pair's variants in the order of the mnl layout (pysofft/utils.f90:87-201):
+    // k = 0..7: (m1,m2), (-m2,-m1), (m2,m1), (-m1,-m2), (m1,-m2), (-m1,m2), (m2,-m1),
+    // (-m2,m1); m1 = m2 keeps k = 0, 1, 4, 5, m1 = 0 keeps k = 0..3, (0, 0) only k = 0
+    const int nv = (m1 == 0 && m2 == 0) ? 1 : ((m1 == 0 || m1 == m2) ? 4 : 8);
+    const int64_t base = slice[(int64_t)(m1 + bw - 1) * (2 * bw - 1) + (m2 + bw - 1)];
+    // 2 pi sqrt(2/(2l+1)) of the pair's degrees, once per block
+    __shared__ double wl[512];
+    for (int i = threadIdx.x; i < L; i += blockDim.x)
+        wl[i] = 2.0 * 3.14159265358979323846 * sqrt(2.0 / (double)(2 * (m2 + i) + 1));
+    __syncthreads();
+    const float rL = 1.0f / (float)L;
+    for (int o = threadIdx.x; o < nv * L; o += blockDim.x) {
+        int v = (int)(((float)o + 0.5f) * rL), li = o - v * L;   // o = v L + li, 0 <= li < L
+        if (li < 0) { --v; li += L; } else if (li >= L) { ++v; li -= L; }
+        const int l = m2 + li;
+        const int k = (m1 == m2 && v >= 2) ? v + 2 : v;
+        int m, n;
+        switch (k) {
+        case 0: m = m1; n = m2; break;
+        case 1: m = -m2; n = -m1; break;
+        case 2: m = m2; n = m1; break;
+        case 3: m = -m1; n = -m2; break;
+        case 4: m = m1; n = -m2; break;
+        case 5: m = -m1; n = m2; break;
+        case 6: m = m2; n = -m1; break;
+        default: m = -m2; n = m1; break;
+        }
         double re = 0.0, im = 0.0;
         const int nrl = l <= lcap ? nr : 0;
         for (int r = 0; r < nrl; ++r) {
@@ -482,8 +515,8 @@ __global__ void corr_coeffs_kernel(const double2 *__restrict__ f, const double2 
             re += wr * (a.x * b.x + a.y * b.y);
             im += wr * (a.y * b.x - a.x * b.y);
         }
-        const double wn = sg * 2.0 * 3.14159265358979323846 * sqrt(2.0 / (double)(2 * l + 1));
-        out[t * c_ts + base + (l - lmin)] = make_double2(wn * re, wn * im);
+        const double wn = ((m + n) & 1) ? -wl[li] : wl[li];
+        out[t * c_ts + base + o] = make_double2(wn * re, wn * im);
     }
 }
 
@@ -491,8 +524,8 @@ hipError_t launch_corr_coeffs(const DeviceTables &tb, int bw, const double2 *f, 
                               int layout, int nr, int64_t r_ts, const double *w, double2 *out, int64_t c_ts,
                               int lcap, int T, hipStream_t s)
 {
-    const int nsl = (2 * bw - 1) * (2 * bw - 1);
-    hipLaunchKernelGGL(corr_coeffs_kernel, dim3(nsl, T), dim3(64), 0, s, f, g, t_ts, layout, nr, r_ts, w, out, c_ts,
+    const int npq = bw * (bw + 1) / 2;
+    hipLaunchKernelGGL(corr_coeffs_kernel, dim3(npq, T), dim3(64), 0, s, f, g, t_ts, layout, nr, r_ts, w, out, c_ts,
                        tb.slice, bw, lcap);
     return hipGetLastError();
 }
