@@ -247,7 +247,7 @@ class Aligner:
         self.mlp.mean = x.mean(0).contiguous()
         self.mlp.std = (x.std(0) + 1e-6).contiguous()
 
-    def train(self, pairs=8_192_000, batch=4096, lr=1e-3, seed=2, log_every=0):
+    def train(self, pairs=24_576_000, batch=4096, lr=3e-3, seed=2, log_every=0):
         """Train on `pairs` freshly generated pairs (one step per batch), cosine-decayed lr."""
         torch = _torch()
         if self.mlp.mean is None:
