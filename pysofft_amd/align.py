@@ -307,7 +307,7 @@ class Aligner:
         bw = int(round(math.sqrt(nb)))
         e0 = self.propose(f, g)
         e, score = self.refine(f, g, e0)
-        norm = torch.sqrt((f.abs() ** 2).sum(1) * (g.abs() ** 2).sum(1))
+        norm = torch.linalg.vector_norm(f, dim=1) * torch.linalg.vector_norm(g, dim=1)
         bad = score / norm < self.threshold
         if bool(bad.any()):
             idx = torch.nonzero(bad).flatten()

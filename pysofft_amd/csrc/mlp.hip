@@ -241,52 +241,58 @@ __global__ void adam_kernel(float *__restrict__ w, const float *__restrict__ g, 
 //    l < 64 (2 x 64)], coefficients divided by s = sqrt(sum_{l<=8} |f_lm|^2 / 81), powers by
 //   s^2, then standardised with (mean, std) when given.  P(l) = sum_m |y_lm|^2; degrees
 //   >= bw enter as log of a tiny floor.
-__global__ void align_features_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g, int64_t nb,
-                                      int bw, int n, const float *__restrict__ mean, const float *__restrict__ stdv,
-                                      float *__restrict__ x)
+__global__ __launch_bounds__(256) void align_features_kernel(const double2 *__restrict__ f,
+                                                             const double2 *__restrict__ g, int64_t nb, int bw, int n,
+                                                             const float *__restrict__ mean,
+                                                             const float *__restrict__ stdv, float *__restrict__ x)
 {
     const int s = blockIdx.x;
     if (s >= n) return;
     const double2 *fs = f + (int64_t)s * nb, *gs = g + (int64_t)s * nb;
-    __shared__ double red[64];
     __shared__ double pw[2][64];
-    const int t = threadIdx.x;  // 64 threads
-    double e = 0.0;
-    for (int i = t; i < 81; i += 64) e += fs[i].x * fs[i].x + fs[i].y * fs[i].y;
-    red[t] = e;
-    // per-degree powers: thread t handles degree t
-    double pf = 0.0, pg = 0.0;
-    if (t < bw) {
-        for (int m = -t; m <= t; ++m) {
-            const double2 a = fs[t * (t + 1) + m], b = gs[t * (t + 1) + m];
-            pf += a.x * a.x + a.y * a.y;
-            pg += b.x * b.x + b.y * b.y;
+    const int t = threadIdx.x, lane = t & 63, nw = blockDim.x >> 6;
+    // per-degree powers: wave w takes degrees w, w + nw, ...; its lanes read the degree's
+    // 2l+1 consecutive coefficients and a shuffle reduction sums them
+    for (int l = t >> 6; l < 64; l += nw) {
+        double pf = 0.0, pg = 0.0;
+        if (l < bw) {
+            for (int m = lane - l; m <= l; m += 64) {
+                const double2 a = fs[l * (l + 1) + m], b = gs[l * (l + 1) + m];
+                pf += a.x * a.x + a.y * a.y;
+                pg += b.x * b.x + b.y * b.y;
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            pf += __shfl_down(pf, o, 64);
+            pg += __shfl_down(pg, o, 64);
+        }
+        if (lane == 0) {
+            pw[0][l] = pf;
+            pw[1][l] = pg;
         }
     }
-    pw[0][t] = pf;
-    pw[1][t] = pg;
     __syncthreads();
-    if (t == 0) {
-        double sum = 0.0;
-        for (int i = 0; i < 64; ++i) sum += red[i];
-        red[0] = sum;
-    }
-    __syncthreads();
-    const double sc = sqrt(red[0] / 81.0) + 1e-300;
+    double e = 0.0;   // sum_{l <= 8} |f_lm|^2
+#pragma unroll
+    for (int l = 0; l <= 8; ++l) e += pw[0][l];
+    const double sc = sqrt(e / 81.0) + 1e-300;
     float *xs = x + (int64_t)s * 452;
     auto put = [&](int k, double v) {
         float y = (float)v;
         if (mean) y = (y - mean[k]) / stdv[k];
         xs[k] = y;
     };
-    for (int i = t; i < 81; i += 64) {
+    for (int i = t; i < 81; i += blockDim.x) {
         put(i, fs[i].x / sc);
         put(81 + i, fs[i].y / sc);
         put(162 + i, gs[i].x / sc);
         put(243 + i, gs[i].y / sc);
     }
-    put(324 + t, log(pw[0][t] / (sc * sc) + 1e-30));
-    put(388 + t, log(pw[1][t] / (sc * sc) + 1e-30));
+    for (int l = t; l < 64; l += blockDim.x) {
+        put(324 + l, log(pw[0][l] / (sc * sc) + 1e-30));
+        put(388 + l, log(pw[1][l] / (sc * sc) + 1e-30));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -589,7 +595,7 @@ int so3ft_align_features(int64_t bw, const void *f_lms, const void *g_lms, int64
     if (bw < 9 || bw > 64 || !f_lms || !g_lms || !x || n < 0) return mfail(SO3FT_EINVAL, "bad argument (9 <= bw <= 64)");
     if ((mean == nullptr) != (stdv == nullptr)) return mfail(SO3FT_EINVAL, "mean and std go together");
     if (n == 0) return SO3FT_OK;
-    hipLaunchKernelGGL(align_features_kernel, dim3(n), dim3(64), 0, reinterpret_cast<hipStream_t>(stream),
+    hipLaunchKernelGGL(align_features_kernel, dim3(n), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
                        reinterpret_cast<const double2 *>(f_lms), reinterpret_cast<const double2 *>(g_lms), bw * bw,
                        (int)bw, (int)n, mean, stdv, x);
     MHIP(hipGetLastError());
