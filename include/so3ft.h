@@ -130,7 +130,9 @@ int so3ft_soft_many_c64_noalloc(so3ft_handle h, const void *so3funcs, void *coef
  * TB = transforms per wave, inverse {1, 2}, forward {1, 2, 4}; schedule 0 = one grouped DWT
  * launch then the bandwidths' FFT passes as parallel graph branches, 1 = one graph branch
  * per bandwidth holding its DWT and FFT passes (its forward DWT with 4 transforms per wave
- * from bw = 96, else 2), -1 = both timed on first use (in a graph), the faster kept. */
+ * from bw = 64, else 2), 2 = the bandwidths one after another on one stream through the
+ * single-bandwidth complex64 kernels (what each bandwidth's *_many_c64 call runs), -1 = all
+ * three timed on first use (in a graph), the fastest kept. */
 int so3ft_group_layout(const int64_t *bws, int64_t n, int64_t *coeff_off, int64_t *grid_off,
                        int64_t *coeff_total, int64_t *grid_total);
 int so3ft_group_create(const int64_t *bws, int64_t n, int64_t *group);

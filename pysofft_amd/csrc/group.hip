@@ -60,9 +60,10 @@ struct Group {
     std::vector<int> fwd_seg_tb;           // forward transforms per wave of each segment's list
     // schedule per direction ([0] forward, [1] inverse): 0 = one grouped DWT launch and the
     // segments' FFT passes as parallel branches; 1 = one branch per segment holding its DWT
-    // and FFT passes (a segment's FFT overlaps another's DWT); -1 = choose by timing
+    // and FFT passes (a segment's FFT overlaps another's DWT); 2 = the segments one after
+    // another on one stream through the single-bandwidth kernels; -1 = choose by timing
     int sched[2] = {-1, -1};
-    float sched_ms[2][2] = {{0.f, 0.f}, {0.f, 0.f}};   // measured graph time per schedule
+    float sched_ms[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};   // measured graph time per schedule
     hipStream_t cap = nullptr;
     std::vector<hipStream_t> side;
     hipEvent_t ev_fork = nullptr;
@@ -342,7 +343,32 @@ int enqueue(Group &G, bool inverse, const void *in, void *out, hipStream_t s, bo
 {
     const float2 *fin = reinterpret_cast<const float2 *>(in);
     float2 *fout = reinterpret_cast<float2 *>(out);
-    if (G.sched[inverse ? 1 : 0] == 1) {
+    const int sc = G.sched[inverse ? 1 : 0];
+    if (sc == 2) {
+        // the segments in sequence on s through the single-bandwidth path's kernels
+        // (launch_dwt_*_mixed on each segment's plan tables), as the bandwidths' own
+        // isoft_many_c64 / soft_many_c64 calls would run them
+        for (const Seg &sg : G.segs) {
+            const int64_t N3 = sg.N * sg.N * sg.N;
+            const int npair = (int)(sg.bw * (sg.bw + 1) / 2);
+            if (!inverse)
+                if (int rc = seg_fft(G, sg, false, fin, nullptr, s)) return rc;
+            for (int64_t i0 = 0; i0 < sg.T; i0 += 4096) {
+                const int T = (int)std::min<int64_t>(4096, sg.T - i0);
+                float2 *Fs = G.F + sg.goff + i0 * N3;
+                if (inverse)
+                    GCHK(launch_dwt_inverse_mixed(sg.tb, (int)sg.bw, (int)sg.N, npair, DWT_COMPLEX,
+                                                  fin + sg.coff + i0 * sg.nc, sg.nc, nullptr, Fs, N3, T, s));
+                else
+                    GCHK(launch_dwt_forward_mixed(sg.tb, (int)sg.bw, (int)sg.N, npair, DWT_COMPLEX, Fs, N3,
+                                                  fout + sg.coff + i0 * sg.nc, sg.nc, T, s));
+            }
+            if (inverse)
+                if (int rc = seg_fft(G, sg, true, nullptr, fout, s)) return rc;
+        }
+        return SO3FT_OK;
+    }
+    if (sc == 1) {
         // one branch per segment: DWT then FFT (inverse) or FFT then DWT (forward)
         if (fork) {
             GCHK(hipEventRecord(G.ev_fork, s));
@@ -558,14 +584,14 @@ int run(int64_t h, bool inverse, const void *in, void *out, void *stream)
             evicted = nullptr;
         }
         if (sc < 0) {
-            // time both schedules' graphs on the caller's buffers (the output is rewritten by
-            // the launch below) and keep the faster
+            // time every schedule's graph on the caller's buffers (the output is rewritten by
+            // the launch below) and keep the fastest
             hipEvent_t e0, e1;
             GCHK(hipEventCreate(&e0));
             GCHK(hipEventCreate(&e1));
             float best = 1e30f;
             int rc = SO3FT_OK;
-            for (int cand = 0; cand < 2 && !rc; ++cand) {
+            for (int cand = 0; cand < 3 && !rc; ++cand) {
                 sc = cand;
                 hipGraphExec_t cx = nullptr;
                 if ((rc = capture(*G, inverse, in, out, &cx))) break;
@@ -599,7 +625,10 @@ int run(int64_t h, bool inverse, const void *in, void *out, void *stream)
                 sc = -1;
                 return rc;
             }
-            sc = G->sched_ms[inverse ? 1 : 0][1] < G->sched_ms[inverse ? 1 : 0][0] ? 1 : 0;
+            const float *ms3 = G->sched_ms[inverse ? 1 : 0];
+            sc = 0;
+            for (int c = 1; c < 3; ++c)
+                if (ms3[c] < ms3[sc]) sc = c;
         } else if (int rc = capture(*G, inverse, in, out, &ex, evicted)) {
             hipEventDestroy(done);
             return rc;
@@ -740,8 +769,9 @@ int so3ft_group_set_config(int64_t h, const int32_t *cfg)
     if (!inv_ok || !fwd_ok || c.inv_grid < 0 || c.fwd_grid < 0 || (c.inv_grid % 8) || (c.fwd_grid % 8))
         return set_error(SO3FT_EINVAL, "config: inverse TB in {1,2}; forward TB in {1,2,4} with NW = 1; "
                                        "grids 0 (one block per item) or a positive multiple of 8");
-    if (cfg[6] < -1 || cfg[6] > 1 || cfg[7] < -1 || cfg[7] > 1)
-        return set_error(SO3FT_EINVAL, "config: schedules -1 (by timing), 0 (grouped launch) or 1 (segment branches)");
+    if (cfg[6] < -1 || cfg[6] > 2 || cfg[7] < -1 || cfg[7] > 2)
+        return set_error(SO3FT_EINVAL, "config: schedules -1 (by timing), 0 (grouped launch), 1 (segment branches) "
+                                       "or 2 (segments in sequence)");
     G->cfg = c;
     if (int rc = set_items(*G)) return rc;
     drop_graphs(*G);

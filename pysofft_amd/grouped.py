@@ -4,8 +4,8 @@
 whole batch with complex64 storage (FP64 recurrence and FFT arithmetic, MFMA
 contraction with d split into FP32 hi + bfloat16 lo parts) as grouped DWT launches over work lists and
 per-bandwidth FFT passes on parallel branches of a HIP graph captured on first use; the
-launch shapes and the schedule (one grouped DWT launch, or one branch per bandwidth so that
-FFT passes overlap other bandwidths' DWTs) are chosen by timing (so3ft_group_* in
+launch shapes and the schedule (one grouped DWT launch, one branch per bandwidth so that
+FFT passes overlap other bandwidths' DWTs, or the bandwidths in sequence) are chosen by timing (so3ft_group_* in
 include/so3ft.h).  The reference has no counterpart: its ``*_many`` calls
 (pysofft/soft.py:255-330) take one bandwidth per ``Soft`` instance.
 

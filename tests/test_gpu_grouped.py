@@ -70,7 +70,7 @@ def test_grouped_every_launch_shape_same_result():
     for inv_tb in (1, 2):
         for tb in (4, 2, 1):
             for grid in (0, 64):
-                for sched in (0, 1):
+                for sched in (0, 1, 2):
                     g.config = dict(inv_tb=inv_tb, inv_grid=grid, fwd_tb=tb, fwd_nw=1, fwd_grid=grid,
                                     inv_sched=sched, fwd_sched=sched)
                     assert torch.equal(g.isoft(c), f0), (inv_tb, grid, sched)
@@ -80,7 +80,7 @@ def test_grouped_every_launch_shape_same_result():
     cfg = g.autotune()
     assert cfg["tuned"] == 1 and cfg["inv_tb"] in (1, 2) and cfg["fwd_tb"] in (1, 2, 4) and cfg["fwd_nw"] == 1
     assert torch.equal(g.isoft(c), f0) and torch.equal(g.soft(f0), c0)
-    assert g.config["inv_sched"] in (0, 1) and g.config["fwd_sched"] in (0, 1)
+    assert g.config["inv_sched"] in (0, 1, 2) and g.config["fwd_sched"] in (0, 1, 2)
 
 
 def test_grouped_graph_replays_match_eager():
