@@ -34,6 +34,19 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b)  // a * conj(b)
 
 constexpr int RW = 5;        // window points per axis
 
+// Step constants of the Kostelec recurrence (wigner.f90:243-256) for 0 <= p <= q <= l < RL_MAX - 1,
+// d^{l+1} = (a cos(b) - a x) d^l - b d^{l-1} with a = A_l / sqrt(P_{l+1}),
+// b = B_l sqrt(P_l) / sqrt(P_{l+1}), x = p q / (l (l+1)), P_l = (l^2 - p^2)(l^2 - q^2),
+// A_l = (l+1) sqrt((2l+1)(2l+3)), B_l = (l+1) sqrt(2l+3) / (l sqrt(2l-1)); stored as
+// (a, -a x, -b) in rows [q][l - q][p] so that the lanes of a wave (consecutive p, one q)
+// read consecutive entries.  Degrees l in [q, RL_MAX - 2] per q.
+__host__ __device__ inline int64_t rc_base(int q)
+{
+    // sum over q' < q of (q' + 1)(RL_MAX - 1 - q') = RL_MAX q(q+1)/2 - q(q+1)(2q+1)/6
+    const int64_t t = (int64_t)q * (q + 1);
+    return RL_MAX * (t / 2) - t * (2 * q + 1) / 6;
+}
+
 // window of pair s at grid bandwidth Bg: alpha / gamma grid indices and the beta samples
 // (2k+1) pi/(4Bg), centred on the nearest grid sample to the estimate (beta kept inside
 // (0, pi))
@@ -73,7 +86,8 @@ __device__ inline void refine_window(const double *ein, int64_t s, int Bg, int (
 // the row phases of the last phase reuse them
 __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g,
                                                      int64_t nb, int n, const double *__restrict__ ein, int Bg, int Lc,
-                                                     double2 *__restrict__ part)
+                                                     double2 *__restrict__ part, const double *__restrict__ rca,
+                                                     const double *__restrict__ rcx, const double *__restrict__ rcb)
 {
     const int s = blockIdx.x >> 2, quarter = blockIdx.x & 3;
     if (s >= n) return;
@@ -86,11 +100,6 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
     double2 *hs = ep + W * NM;                         // side set at +-p: [l(l+1)/2 + p]
     double2 *fs = hs + Lc * (Lc + 1) / 2;              // other set: [l(l+1) + j]
     __shared__ double wl[RL_MAX];                      // 2 pi sqrt(2/(2l+1))
-    // recurrence constants of the step l -> l+1 (wigner.f90:243-256) factored as
-    // a = A_l / sqrt(P_{l+1}), b = B_l sqrt(P_l) / sqrt(P_{l+1}), x = p q X_l with
-    // P_l = (l^2 - p^2)(l^2 - q^2): A_l = (l+1) sqrt((2l+1)(2l+3)),
-    // B_l = (l+1) sqrt(2l+3) / (l sqrt(2l-1)), X_l = 1 / (l (l+1))
-    __shared__ double TA[RL_MAX], TB[RL_MAX], TX[RL_MAX];
     __shared__ double lfac[2 * RL_MAX + 1];            // log k!
     __shared__ double lq[RL_MAX];                      // log sqrt((2q+1)/2)
     __shared__ double lcb[2 * W], lsb[2 * W], cbv[2 * W];  // log cos(b/2), log sin(b/2), cos b; b, pi - b
@@ -123,10 +132,6 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
     for (int l = tid; l < Lc; l += blockDim.x) {
         wl[l] = 2.0 * pi * sqrt(2.0 / (2.0 * l + 1.0));
         lq[l] = 0.5 * log((2.0 * l + 1.0) * 0.5);
-        const double dl = (double)l;
-        TA[l] = (dl + 1.0) * sqrt((2.0 * dl + 1.0) * (2.0 * dl + 3.0));
-        TB[l] = l == 0 ? 0.0 : (dl + 1.0) * sqrt(2.0 * dl + 3.0) / (dl * sqrt(2.0 * dl - 1.0));
-        TX[l] = l == 0 ? 0.0 : 1.0 / (dl * (dl + 1.0));
     }
     for (int k = tid; k <= 2 * Lc; k += blockDim.x) lfac[k] = lgamma((double)k + 1.0);
     __syncthreads();
@@ -173,16 +178,23 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
         const double s1 = v1.valid ? (((v1.m + v1.n) & 1) ? -1.0 : 1.0) * v1.sign : 0.0;
         // the other set's index of each variant (n for X, m for Y; +-q); the side index is +-p
         const int j0 = v0.valid ? (ytype ? v0.m : v0.n) : 0, j1 = v1.valid ? (ytype ? v1.m : v1.n) : 0;
-        const double pp = (double)p * p, qq = (double)q * q, pq = (double)p * q;
-        double sP = 0.0;   // sqrt(P_l); P_q = 0
+        // the step constants of (p, q): table row of degree l at rc_row(q, l) + p
+        const int64_t rq = rc_base(q);
         struct Ops {
             double2 hv, o0, o1;
+            double ca, cx, cb;   // step l -> l+1: a, -a x, -b (degree l < Lc - 1)
         };
         auto load = [&](Ops &o, int l) {
             const int rg = l * (l + 1);
             o.hv = hs[rg / 2 + p];
             o.o0 = fs[rg + j0];
             o.o1 = fs[rg + j1];
+            const bool st = l < Lc - 1;   // degree Lc - 1 takes no step (and has no row)
+            const int64_t i = st ? rq + (int64_t)(l - q) * (q + 1) + p : 0;
+            const double a = rca[i], ax = rcx[i], b = rcb[i];
+            o.ca = st ? a : 0.0;
+            o.cx = st ? ax : 0.0;
+            o.cb = st ? b : 0.0;
         };
         // one degree: F += cc^l d^l, then d^{l+1}
         auto step = [&](const Ops &cu, int l) {
@@ -199,15 +211,10 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
                 F1[bb].x = fma(c1.x, d[W + bb], F1[bb].x); F1[bb].y = fma(c1.y, d[W + bb], F1[bb].y);
             }
             if (l < Lc - 1) {
-                const double L1 = (double)(l + 1) * (l + 1);
-                const double P1 = (L1 - pp) * (L1 - qq);
-                const double r = rsqrt(P1);
-                const double ca = TA[l] * r, cbb = TB[l] * sP * r, cacx = ca * (pq * TX[l]);
-                sP = P1 * r;
 #pragma unroll
                 for (int bb = 0; bb < 2 * W; ++bb) {
-                    const double k = fma(ca, cb[bb], -cacx);
-                    const double dn = fma(k, d[bb], -cbb * dp[bb]);
+                    const double k = fma(cu.ca, cb[bb], cu.cx);
+                    const double dn = fma(k, d[bb], cu.cb * dp[bb]);
                     dp[bb] = d[bb];
                     d[bb] = dn;
                 }
@@ -359,12 +366,55 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
         }
         part = e.first;
     }
+    // the step-constant table of this device, built once
+    const double *rca = nullptr, *rcx = nullptr, *rcb = nullptr;
+    {
+        static std::mutex mu;
+        static std::map<int, double *> tabs;
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return SO3FT_EHIP;
+        std::lock_guard<std::mutex> lk(mu);
+        double *&t = tabs[dev];
+        const int64_t ne = rc_base(RL_MAX);
+        if (!t) {
+            std::vector<double> h(3 * ne);
+            for (int q = 0; q < RL_MAX; ++q)
+                for (int l = q; l < RL_MAX - 1; ++l)
+                    for (int p = 0; p <= q; ++p) {
+                        const double dl = l, L1 = (double)(l + 1) * (l + 1), L0 = (double)l * l;
+                        const double pp = (double)p * p, qq = (double)q * q;
+                        const double A = (dl + 1.0) * std::sqrt((2.0 * dl + 1.0) * (2.0 * dl + 3.0));
+                        const double B = l == 0 ? 0.0 : (dl + 1.0) * std::sqrt(2.0 * dl + 3.0) / (dl * std::sqrt(2.0 * dl - 1.0));
+                        const double X = l == 0 ? 0.0 : (double)p * q / (dl * (dl + 1.0));
+                        const double r = 1.0 / std::sqrt((L1 - pp) * (L1 - qq));
+                        const double sP = l == q ? 0.0 : std::sqrt((L0 - pp) * (L0 - qq));
+                        const int64_t i = rc_base(q) + (int64_t)(l - q) * (q + 1) + p;
+                        h[i] = A * r;
+                        h[ne + i] = -A * r * X;
+                        h[2 * ne + i] = -B * sP * r;
+                    }
+            if (hipMalloc((void **)&t, sizeof(double) * 3 * ne) != hipSuccess) {
+                hipGetLastError();
+                t = nullptr;
+                return SO3FT_ENOMEM;
+            }
+            if (hipMemcpy(t, h.data(), sizeof(double) * 3 * ne, hipMemcpyHostToDevice) != hipSuccess) {
+                hipFree(t);
+                t = nullptr;
+                return SO3FT_EHIP;
+            }
+        }
+        rca = t;
+        rcx = t + ne;
+        rcb = t + 2 * ne;
+    }
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     // one thread per order pair up to 512 (small degree limits leave room for more blocks per CU)
     const int npq = Lc * (Lc + 1) / 2;
     const int threads = std::min(512, std::max(64, (npq + 63) / 64 * 64));
     hipLaunchKernelGGL(refine_kernel, dim3(4 * n), dim3(threads), bytes, st, reinterpret_cast<const double2 *>(f_lms),
-                       reinterpret_cast<const double2 *>(g_lms), bw * bw, (int)n, euler_in, (int)grid_bw, Lc, part);
+                       reinterpret_cast<const double2 *>(g_lms), bw * bw, (int)n, euler_in, (int)grid_bw, Lc, part,
+                       rca, rcx, rcb);
     if (hipGetLastError() != hipSuccess) return SO3FT_EHIP;
     hipLaunchKernelGGL(refine_finalize, dim3(n), dim3(128), 0, st, part, (int)n, euler_in, (int)grid_bw, euler_out,
                        score);
