@@ -423,8 +423,12 @@ __device__ __forceinline__ void fwd_mix_item32(const float2 *__restrict__ F, int
 #pragma unroll
     for (int h = 0; h < 8; ++h) load_b(0, h);
 
+    // a pair's last tile skips the recurrence steps of its second M-tile when at most 16
+    // degrees remain (that M-tile's products then read stale LDS rows; they would only give
+    // degrees past the last one and are not stored)
 #pragma unroll 1
     for (int l0 = 0; l0 < L; l0 += FT32_TD) {
+        const bool two = L - l0 > 16;
         double accr[TG][2][4], acci[TG][2][4];
 #pragma unroll
         for (int g = 0; g < TG; ++g)
@@ -441,9 +445,16 @@ __device__ __forceinline__ void fwd_mix_item32(const float2 *__restrict__ F, int
             double2 st = rs[c][lane];
             double dcur = st.x, dprev = st.y;
 #pragma unroll
-            for (int k = 0; k < FT32_TD; ++k) {   // the chain is zero past the last degree
+            for (int k = 0; k < 16; ++k) {   // the chain is zero past the last degree
                 Dt[k][lane] = (float)(wt * dcur);
                 rec_step(rcp, l0 + k, cb, dcur, dprev);
+            }
+            if (two) {
+#pragma unroll
+                for (int k = 16; k < FT32_TD; ++k) {
+                    Dt[k][lane] = (float)(wt * dcur);
+                    rec_step(rcp, l0 + k, cb, dcur, dprev);
+                }
             }
             rs[c][lane] = make_double2(dcur, dprev);
             asm volatile("" ::: "memory");
