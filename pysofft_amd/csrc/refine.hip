@@ -38,7 +38,7 @@ constexpr int RW = 5;        // window points per axis
 // d^{l+1} = (a cos(b) - a x) d^l - b d^{l-1} with a = A_l / sqrt(P_{l+1}),
 // b = B_l sqrt(P_l) / sqrt(P_{l+1}), x = p q / (l (l+1)), P_l = (l^2 - p^2)(l^2 - q^2),
 // A_l = (l+1) sqrt((2l+1)(2l+3)), B_l = (l+1) sqrt(2l+3) / (l sqrt(2l-1)); stored as
-// (a, -a x, -b) in rows [q][l - q][p] so that the lanes of a wave (consecutive p, one q)
+// (a, -a x, -b) in entries [q][l - q][p] so that the lanes of a wave (consecutive p, one q)
 // read consecutive entries.  Degrees l in [q, RL_MAX - 2] per q.
 __host__ __device__ inline int64_t rc_base(int q)
 {
@@ -86,8 +86,8 @@ __device__ inline void refine_window(const double *ein, int64_t s, int Bg, int (
 // the row phases of the last phase reuse them
 __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g,
                                                      int64_t nb, int n, const double *__restrict__ ein, int Bg, int Lc,
-                                                     double2 *__restrict__ part, const double *__restrict__ rca,
-                                                     const double *__restrict__ rcx, const double *__restrict__ rcb)
+                                                     double2 *__restrict__ part, const double *__restrict__ rct,
+                                                     const double *__restrict__ kt)
 {
     const int s = blockIdx.x >> 2, quarter = blockIdx.x & 3;
     if (s >= n) return;
@@ -95,11 +95,12 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
     constexpr int W = RW;
     const int NM = 2 * Lc - 1;
     extern __shared__ double sm[];
-    double2 *R = reinterpret_cast<double2 *>(sm);     // [b][k][p]: p fastest, so the lanes' rows p hit distinct banks
-    double2 *ep = R + W * Lc * W;                      // [k][j + Lc - 1]: e^{-i j gamma_k} (X) / e^{-i j alpha_k} (Y)
-    double2 *hs = ep + W * NM;                         // side set at +-p: [l(l+1)/2 + p]
-    double2 *fs = hs + Lc * (Lc + 1) / 2;              // other set: [l(l+1) + j]
-    __shared__ double wl[RL_MAX];                      // 2 pi sqrt(2/(2l+1))
+    // R [b][k][p] as real and imaginary arrays, p fastest: the lanes' 8-byte atomics on
+    // consecutive p fill the banks without conflicts
+    double *Rr = sm, *Ri = sm + W * Lc * W;
+    double2 *ep = reinterpret_cast<double2 *>(sm + 2 * W * Lc * W);   // [k][j + Lc - 1]: e^{-i j gamma_k} (X) / e^{-i j alpha_k} (Y)
+    double2 *hs = ep + W * NM;                         // w_l x side set at +-p: [l(l+1)/2 + p], l <= Lc
+    double2 *fs = hs + (Lc + 1) * (Lc + 2) / 2;        // other set: [l(l+1) + j]
     __shared__ double lfac[2 * RL_MAX + 1];            // log k!
     __shared__ double lq[RL_MAX];                      // log sqrt((2q+1)/2)
     __shared__ double lcb[2 * W], lsb[2 * W], cbv[2 * W];  // log cos(b/2), log sin(b/2), cos b; b, pi - b
@@ -120,20 +121,49 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
         }
     }
     const double2 *side = (ytype ? g : f) + (int64_t)s * nb, *other = (ytype ? f : g) + (int64_t)s * nb;
-    for (int i = tid; i < W * Lc * W; i += blockDim.x) R[i] = make_double2(0.0, 0.0);
-    for (int i = tid; i < Lc * Lc; i += blockDim.x) fs[i] = other[i];
-    for (int i = tid; i < Lc * (Lc + 1) / 2; i += blockDim.x) {
-        int l = (int)((sqrt(8.0 * i + 1.0) - 1.0) * 0.5);
-        while (l * (l + 1) / 2 > i) --l;
-        while ((l + 1) * (l + 2) / 2 <= i) ++l;
-        const int mp = i - l * (l + 1) / 2;
-        hs[i] = side[l * (l + 1) + (neg ? -mp : mp)];
+    for (int i = tid; i < W * Lc * W; i += blockDim.x) Rr[i] = Ri[i] = 0.0;
+    // the coefficient sets: every thread issues its loads before its first LDS write
+    constexpr int U = 8;
+    const int nf = Lc * Lc;
+    for (int i0 = tid; i0 < nf; i0 += U * blockDim.x) {
+        double2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * (int)blockDim.x;
+            v[u] = i < nf ? other[i] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * (int)blockDim.x;
+            if (i < nf) fs[i] = v[u];
+        }
     }
-    for (int l = tid; l < Lc; l += blockDim.x) {
-        wl[l] = 2.0 * pi * sqrt(2.0 / (2.0 * l + 1.0));
-        lq[l] = 0.5 * log((2.0 * l + 1.0) * 0.5);
+    // the side set times w_l = 2 pi sqrt(2/(2l+1)) (table kt), and a zero row at l = Lc for
+    // the lanes whose chain has ended
+    const double *wl = kt + 3 * RL_MAX + 1;
+    const int nh = (Lc + 1) * (Lc + 2) / 2;
+    for (int i0 = tid; i0 < nh; i0 += 4 * blockDim.x) {
+        double2 v[4];
+        double wv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * (int)blockDim.x;
+            int l = (int)((sqrtf(8.0f * (float)i + 1.0f) - 1.0f) * 0.5f);
+            while (l * (l + 1) / 2 > i) --l;
+            while ((l + 1) * (l + 2) / 2 <= i) ++l;
+            const int mp = i - l * (l + 1) / 2;
+            const bool ok = i < nh && l < Lc;
+            v[u] = ok ? side[l * (l + 1) + (neg ? -mp : mp)] : make_double2(0.0, 0.0);
+            wv[u] = ok ? wl[l] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * (int)blockDim.x;
+            if (i < nh) hs[i] = make_double2(wv[u] * v[u].x, wv[u] * v[u].y);
+        }
     }
-    for (int k = tid; k <= 2 * Lc; k += blockDim.x) lfac[k] = lgamma((double)k + 1.0);
+    for (int k = tid; k <= 2 * Lc; k += blockDim.x) lfac[k] = kt[k];
+    for (int l = tid; l < Lc; l += blockDim.x) lq[l] = kt[2 * RL_MAX + 1 + l];
     __syncthreads();
     if (tid < 2 * W) {
         const double b = tid < W ? bet[tid] : pi - bet[tid - W];
@@ -155,14 +185,18 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
     const int npq = Lc * (Lc + 1) / 2;
     // items in q-major order, i.e. by decreasing recurrence length; each wave claims the next
     // 64 from a queue in LDS, so the waves end together (longest items first) instead of the
-    // waves holding the long items setting the block's time
-    auto item = [&](int P) {
+    // waves holding the long items setting the block's time.  The lanes of a wave run the same
+    // number of degree steps, that of its longest chain (lane 0, the smallest q): a lane whose
+    // chain has ended reads the zero row of the side set and zero step constants, so its sums
+    // stay put with no branch in the loop.
+    const int lane = tid & 63;
+    const uint32_t zi = (uint32_t)rc_base(RL_MAX);   // the zero entry of the step-constant table
+    auto item = [&](int P, bool live, int qmin) {
         int q = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
         while (q * (q + 1) / 2 > P) --q;
         while ((q + 1) * (q + 2) / 2 <= P) ++q;
         const int p = P - q * (q + 1) / 2;
         const Variant v0 = variant(va, p, q), v1 = variant(vb, p, q);
-        if (!v0.valid && !v1.valid) return;
         // seeds d~^q_{p,q} at beta and pi - beta (closed form, log domain)
         const double lc = 0.5 * (lfac[2 * q] - lfac[q + p] - lfac[q - p]) + lq[q];
         double d[2 * W], dp[2 * W];
@@ -174,87 +208,98 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
         double2 F0[W], F1[W];
 #pragma unroll
         for (int bb = 0; bb < W; ++bb) { F0[bb] = make_double2(0.0, 0.0); F1[bb] = make_double2(0.0, 0.0); }
-        const double s0 = v0.valid ? (((v0.m + v0.n) & 1) ? -1.0 : 1.0) * v0.sign : 0.0;
-        const double s1 = v1.valid ? (((v1.m + v1.n) & 1) ? -1.0 : 1.0) * v1.sign : 0.0;
+        // signs (-1)^(m+n) x variant sign, with the variant's (-1)^l for the even (A) and the
+        // odd (B) steps; zero for an invalid variant or a padding lane
+        const double s0 = (live && v0.valid) ? (((v0.m + v0.n) & 1) ? -1.0 : 1.0) * v0.sign : 0.0;
+        const double s1 = (live && v1.valid) ? (((v1.m + v1.n) & 1) ? -1.0 : 1.0) * v1.sign : 0.0;
+        const double s0a = (v0.lsign && (q & 1)) ? -s0 : s0, s0b = v0.lsign ? -s0a : s0a;
+        const double s1a = (v1.lsign && (q & 1)) ? -s1 : s1, s1b = v1.lsign ? -s1a : s1a;
         // the other set's index of each variant (n for X, m for Y; +-q); the side index is +-p
         const int j0 = v0.valid ? (ytype ? v0.m : v0.n) : 0, j1 = v1.valid ? (ytype ? v1.m : v1.n) : 0;
-        // the step constants of (p, q): table row of degree l at rc_row(q, l) + p
-        const int64_t rq = rc_base(q);
+        // the step constants of (p, q): degree l at entry rc_base(q) + (l - q)(q + 1) + p
+        const uint32_t rq = (uint32_t)(rc_base(q) + p), rs = (uint32_t)(q + 1);
         struct Ops {
             double2 hv, o0, o1;
-            double ca, cx, cb;   // step l -> l+1: a, -a x, -b (degree l < Lc - 1)
+            double ca, cx, cb;   // step l -> l+1: a, -a x, -b (zero from degree Lc - 1 on)
         };
-        auto load = [&](Ops &o, int l) {
-            const int rg = l * (l + 1);
-            o.hv = hs[rg / 2 + p];
+        // operands of step t (degree q + t), addresses clamped into the staged sets
+        auto load = [&](Ops &o, int t) {
+            const uint32_t l = (uint32_t)(q + t);
+            const uint32_t la = min(l, (uint32_t)Lc), lb = min(l, (uint32_t)(Lc - 1));
+            o.hv = hs[((la * (la + 1)) >> 1) + p];
+            const uint32_t rg = lb * (lb + 1);
             o.o0 = fs[rg + j0];
             o.o1 = fs[rg + j1];
-            const bool st = l < Lc - 1;   // degree Lc - 1 takes no step (and has no row)
-            const int64_t i = st ? rq + (int64_t)(l - q) * (q + 1) + p : 0;
-            const double a = rca[i], ax = rcx[i], b = rcb[i];
-            o.ca = st ? a : 0.0;
-            o.cx = st ? ax : 0.0;
-            o.cb = st ? b : 0.0;
+            const uint32_t i = l < (uint32_t)(Lc - 1) ? rq + (uint32_t)t * rs : zi;
+            const double *e = rct + 3 * (size_t)i;
+            o.ca = e[0];
+            o.cx = e[1];
+            o.cb = e[2];
         };
-        // one degree: F += cc^l d^l, then d^{l+1}
-        auto step = [&](const Ops &cu, int l) {
-            const double wn = wl[l];
-            const double sl0 = wn * s0 * ((v0.lsign && (l & 1)) ? -1.0 : 1.0);
-            const double sl1 = wn * s1 * ((v1.lsign && (l & 1)) ? -1.0 : 1.0);
-            // f^l_m conj(g^l_n): X (side f) hv conj(o), Y (side g) o conj(hv)
-            const double2 z0 = ytype ? cmulc(cu.o0, cu.hv) : cmulc(cu.hv, cu.o0);
-            const double2 z1 = ytype ? cmulc(cu.o1, cu.hv) : cmulc(cu.hv, cu.o1);
-            const double2 c0 = make_double2(sl0 * z0.x, sl0 * z0.y), c1 = make_double2(sl1 * z1.x, sl1 * z1.y);
+        // one degree: F += cc^l d^l (side x conj(other); the Y quarters' conjugate is taken
+        // in the fold), then d^{l+1}
+        auto step = [&](const Ops &cu, double sa, double sb) {
+            const double2 z0 = cmulc(cu.hv, cu.o0), z1 = cmulc(cu.hv, cu.o1);
+            const double2 c0 = make_double2(sa * z0.x, sa * z0.y), c1 = make_double2(sb * z1.x, sb * z1.y);
 #pragma unroll
             for (int bb = 0; bb < W; ++bb) {
                 F0[bb].x = fma(c0.x, d[bb], F0[bb].x); F0[bb].y = fma(c0.y, d[bb], F0[bb].y);
                 F1[bb].x = fma(c1.x, d[W + bb], F1[bb].x); F1[bb].y = fma(c1.y, d[W + bb], F1[bb].y);
             }
-            if (l < Lc - 1) {
 #pragma unroll
-                for (int bb = 0; bb < 2 * W; ++bb) {
-                    const double k = fma(cu.ca, cb[bb], cu.cx);
-                    const double dn = fma(k, d[bb], cu.cb * dp[bb]);
-                    dp[bb] = d[bb];
-                    d[bb] = dn;
-                }
+            for (int bb = 0; bb < 2 * W; ++bb) {
+                const double k = fma(cu.ca, cb[bb], cu.cx);
+                const double dn = fma(k, d[bb], cu.cb * dp[bb]);
+                dp[bb] = d[bb];
+                d[bb] = dn;
             }
         };
-        // two operand sets in turn: the loads of degree l+1 are in flight during degree l
+        // the wave's step count, even; two operand sets in turn: the loads of step t+1 are in
+        // flight during step t (the barriers keep them there)
+        const int nst = (Lc - qmin + 1) & ~1;
         Ops A, B;
-        load(A, q);
-        int l = q;
-        for (; l + 1 < Lc; l += 2) {
-            load(B, l + 1);
-            step(A, l);
-            if (l + 2 < Lc) load(A, l + 2);
-            step(B, l + 1);
+        load(A, 0);
+#pragma unroll 1
+        for (int t = 0; t < nst; t += 2) {
+            load(B, t + 1);
+            __builtin_amdgcn_sched_barrier(0);
+            step(A, s0a, s1a);
+            __builtin_amdgcn_sched_barrier(0);
+            load(A, t + 2);
+            __builtin_amdgcn_sched_barrier(0);
+            step(B, s0b, s1b);
+            __builtin_amdgcn_sched_barrier(0);
         }
-        if (l < Lc) step(A, l);
-        // fold the other index's phase into R[b][k][p] (distinct p across the lanes of one q)
-        auto fold = [&](int jj, const double2 (&Fk)[W]) {
+        // fold the other index's phase into R[b][k][p] (distinct p across the lanes of one q):
+        // both variants of the quarter share the row p, so one sum per entry
+        if (live) {
+            const double cy = ytype ? -1.0 : 1.0;   // the Y quarters' conjugate
 #pragma unroll 1
             for (int k = 0; k < W; ++k) {
-                const double2 e = ep[k * NM + jj + Lc - 1];
+                const double2 e0 = ep[k * NM + j0 + Lc - 1], e1 = ep[k * NM + j1 + Lc - 1];
 #pragma unroll
                 for (int bb = 0; bb < W; ++bb) {
-                    const double2 z = Fk[bb];
-                    double2 *dst = &R[(bb * W + k) * Lc + p];
-                    atomicAdd(&dst->x, z.x * e.x - z.y * e.y);
-                    atomicAdd(&dst->y, z.x * e.y + z.y * e.x);
+                    const double2 z0 = make_double2(F0[bb].x, cy * F0[bb].y);
+                    const double2 z1 = make_double2(F1[bb].x, cy * F1[bb].y);
+                    const int o = (bb * W + k) * Lc + p;
+                    atomicAdd(&Rr[o], (z0.x * e0.x - z0.y * e0.y) + (z1.x * e1.x - z1.y * e1.y));
+                    atomicAdd(&Ri[o], (z0.x * e0.y + z0.y * e0.x) + (z1.x * e1.y + z1.y * e1.x));
                 }
             }
-        };
-        if (v0.valid) fold(j0, F0);
-        if (v1.valid) fold(j1, F1);
+        }
     };
-    const int lane = tid & 63;
     for (;;) {
         int base = 0;
         if (lane == 0) base = atomicAdd(&qnext, 64);
-        base = __shfl(base, 0, 64);
+        base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));   // the same in every lane
         if (base >= npq) break;
-        if (base + lane < npq) item(base + lane);
+        // padding lanes past the last item repeat it with zero signs
+        const bool live = base + lane < npq;
+        const int P = live ? base + lane : npq - 1;
+        int q0 = (int)((sqrt(8.0 * base + 1.0) - 1.0) * 0.5);
+        while (q0 * (q0 + 1) / 2 > base) --q0;
+        while ((q0 + 1) * (q0 + 2) / 2 <= base) ++q0;
+        item(P, live, q0);
     }
     __syncthreads();
     // the row sums: X quarters sum_p R[b][c][p] e^{-i m alpha_a} (m = +-p), Y quarters
@@ -273,7 +318,8 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
         const int kf = ytype ? a : c, kr = ytype ? c : a;   // fold index in R, row-phase index
         double re = 0.0, im = 0.0;
         for (int pr = 0; pr < Lc; ++pr) {
-            const double2 z = R[(b * W + kf) * Lc + pr], e = er[kr * Lc + pr];
+            const int ro = (b * W + kf) * Lc + pr;
+            const double2 z = make_double2(Rr[ro], Ri[ro]), e = er[kr * Lc + pr];
             re += z.x * e.x - z.y * e.y;
             im += z.x * e.y + z.y * e.x;
         }
@@ -332,7 +378,7 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
     if (n > (1 << 29)) return SO3FT_EINVAL;
     const int Lc = (int)lmax, NM = 2 * Lc - 1;
     // the alpha table [RW][Lc] reuses the f / g space after the main phase
-    const size_t fg = std::max<size_t>((size_t)Lc * (Lc + 1) / 2 + (size_t)Lc * Lc, (size_t)RW * Lc);
+    const size_t fg = std::max<size_t>((size_t)(Lc + 1) * (Lc + 2) / 2 + (size_t)Lc * Lc, (size_t)RW * Lc);
     const size_t bytes = sizeof(double2) * ((size_t)RW * Lc * RW + RW * NM + fg);
     static bool attr = false;
     if (!attr) {
@@ -366,8 +412,10 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
         }
         part = e.first;
     }
-    // the step-constant table of this device, built once
-    const double *rca = nullptr, *rcx = nullptr, *rcb = nullptr;
+    // the constant tables of this device, built once: step constants (a, -a x, -b) of every
+    // step, then one zero entry (index rc_base(RL_MAX)); then log k! (k <= 2 RL_MAX),
+    // log sqrt((2l+1)/2) and w_l = 2 pi sqrt(2/(2l+1)) (l < RL_MAX)
+    const double *rct = nullptr, *kt = nullptr;
     {
         static std::mutex mu;
         static std::map<int, double *> tabs;
@@ -375,9 +423,15 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
         if (hipGetDevice(&dev) != hipSuccess) return SO3FT_EHIP;
         std::lock_guard<std::mutex> lk(mu);
         double *&t = tabs[dev];
-        const int64_t ne = rc_base(RL_MAX);
+        const int64_t ne = rc_base(RL_MAX) + 1;
+        const int64_t nt = 3 * ne + 4 * RL_MAX + 1;
         if (!t) {
-            std::vector<double> h(3 * ne);
+            std::vector<double> h(nt, 0.0);
+            for (int k = 0; k <= 2 * RL_MAX; ++k) h[3 * ne + k] = std::lgamma((double)k + 1.0);
+            for (int l = 0; l < RL_MAX; ++l) {
+                h[3 * ne + 2 * RL_MAX + 1 + l] = 0.5 * std::log((2.0 * l + 1.0) * 0.5);
+                h[3 * ne + 3 * RL_MAX + 1 + l] = 2.0 * M_PI * std::sqrt(2.0 / (2.0 * l + 1.0));
+            }
             for (int q = 0; q < RL_MAX; ++q)
                 for (int l = q; l < RL_MAX - 1; ++l)
                     for (int p = 0; p <= q; ++p) {
@@ -389,24 +443,23 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
                         const double r = 1.0 / std::sqrt((L1 - pp) * (L1 - qq));
                         const double sP = l == q ? 0.0 : std::sqrt((L0 - pp) * (L0 - qq));
                         const int64_t i = rc_base(q) + (int64_t)(l - q) * (q + 1) + p;
-                        h[i] = A * r;
-                        h[ne + i] = -A * r * X;
-                        h[2 * ne + i] = -B * sP * r;
+                        h[3 * i] = A * r;
+                        h[3 * i + 1] = -A * r * X;
+                        h[3 * i + 2] = -B * sP * r;
                     }
-            if (hipMalloc((void **)&t, sizeof(double) * 3 * ne) != hipSuccess) {
+            if (hipMalloc((void **)&t, sizeof(double) * nt) != hipSuccess) {
                 hipGetLastError();
                 t = nullptr;
                 return SO3FT_ENOMEM;
             }
-            if (hipMemcpy(t, h.data(), sizeof(double) * 3 * ne, hipMemcpyHostToDevice) != hipSuccess) {
+            if (hipMemcpy(t, h.data(), sizeof(double) * nt, hipMemcpyHostToDevice) != hipSuccess) {
                 hipFree(t);
                 t = nullptr;
                 return SO3FT_EHIP;
             }
         }
-        rca = t;
-        rcx = t + ne;
-        rcb = t + 2 * ne;
+        rct = t;
+        kt = t + 3 * ne;
     }
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     // one thread per order pair up to 512 (small degree limits leave room for more blocks per CU)
@@ -414,7 +467,7 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
     const int threads = std::min(512, std::max(64, (npq + 63) / 64 * 64));
     hipLaunchKernelGGL(refine_kernel, dim3(4 * n), dim3(threads), bytes, st, reinterpret_cast<const double2 *>(f_lms),
                        reinterpret_cast<const double2 *>(g_lms), bw * bw, (int)n, euler_in, (int)grid_bw, Lc, part,
-                       rca, rcx, rcb);
+                       rct, kt);
     if (hipGetLastError() != hipSuccess) return SO3FT_EHIP;
     hipLaunchKernelGGL(refine_finalize, dim3(n), dim3(128), 0, st, part, (int)n, euler_in, (int)grid_bw, euler_out,
                        score);
