@@ -77,17 +77,16 @@ __device__ inline void refine_window(const double *ein, int64_t s, int Bg, int (
 // X quarters fold F(beta, m, n) over n with e^{-i n gamma_c} into R[beta][c][p], Y
 // quarters over m with e^{-i m alpha_a} into R[beta][a][p]; the remaining sum over the
 // row index follows at the end.  Work items are the order pairs in q-major order: the
-// lanes of a wave run recurrences of equal length (Lc - q) and write distinct rows p.
+// lanes of a wave run recurrences of about equal length (Lc - q) and write distinct rows p.
 // Each item runs the recurrence at the 5 window betas and at pi - beta (10 chains) and
-// accumulates F over l in registers.  The quarter stages in LDS (FP64) the coefficient
-// set indexed by +-p on its side only (f for X, g for Y) and the other set in full.
-// dynamic LDS: R [RW b][RW k][Lc p] | ep [RW k][2Lc-1] fold phases | hs [Lc(Lc+1)/2]
-// (the +-p side set) | fs [Lc^2] (the other set); hs / fs hold at least RW Lc entries:
-// the row phases of the last phase reuse them
-__global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g,
-                                                     int64_t nb, int n, const double *__restrict__ ein, int Bg, int Lc,
-                                                     double2 *__restrict__ part, const double *__restrict__ rct,
-                                                     const double *__restrict__ kt)
+// accumulates F over l in registers, reading the coefficient sets from global memory
+// (L2; the pair's four blocks share them).
+// dynamic LDS: R [RW b][RW k][Lc p] (real, imaginary) | ep [RW k][2Lc-1] fold phases | er
+// [RW k'][Lc p] row phases: 40 KB at Lc = 64, so three 256-thread blocks share a CU and one
+// block's set-up and row sums overlap the others' degree loops
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void refine_kernel(
+    const double2 *__restrict__ f, const double2 *__restrict__ g, int64_t nb, int n, const double *__restrict__ ein,
+    int Bg, int Lc, double2 *__restrict__ part, const double *__restrict__ rct, const double *__restrict__ kt)
 {
     const int s = blockIdx.x >> 2, quarter = blockIdx.x & 3;
     if (s >= n) return;
@@ -99,8 +98,7 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
     // consecutive p fill the banks without conflicts
     double *Rr = sm, *Ri = sm + W * Lc * W;
     double2 *ep = reinterpret_cast<double2 *>(sm + 2 * W * Lc * W);   // [k][j + Lc - 1]: e^{-i j gamma_k} (X) / e^{-i j alpha_k} (Y)
-    double2 *hs = ep + W * NM;                         // w_l x side set at +-p: [l(l+1)/2 + p], l <= Lc
-    double2 *fs = hs + (Lc + 1) * (Lc + 2) / 2;        // other set: [l(l+1) + j]
+    double2 *er = ep + W * NM;                         // [k'][p]: row phases of the last phase
     __shared__ double lfac[2 * RL_MAX + 1];            // log k!
     __shared__ double lq[RL_MAX];                      // log sqrt((2q+1)/2)
     __shared__ double lcb[2 * W], lsb[2 * W], cbv[2 * W];  // log cos(b/2), log sin(b/2), cos b; b, pi - b
@@ -122,46 +120,7 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
     }
     const double2 *side = (ytype ? g : f) + (int64_t)s * nb, *other = (ytype ? f : g) + (int64_t)s * nb;
     for (int i = tid; i < W * Lc * W; i += blockDim.x) Rr[i] = Ri[i] = 0.0;
-    // the coefficient sets: every thread issues its loads before its first LDS write
-    constexpr int U = 8;
-    const int nf = Lc * Lc;
-    for (int i0 = tid; i0 < nf; i0 += U * blockDim.x) {
-        double2 v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int i = i0 + u * (int)blockDim.x;
-            v[u] = i < nf ? other[i] : make_double2(0.0, 0.0);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int i = i0 + u * (int)blockDim.x;
-            if (i < nf) fs[i] = v[u];
-        }
-    }
-    // the side set times w_l = 2 pi sqrt(2/(2l+1)) (table kt), and a zero row at l = Lc for
-    // the lanes whose chain has ended
-    const double *wl = kt + 3 * RL_MAX + 1;
-    const int nh = (Lc + 1) * (Lc + 2) / 2;
-    for (int i0 = tid; i0 < nh; i0 += 4 * blockDim.x) {
-        double2 v[4];
-        double wv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int i = i0 + u * (int)blockDim.x;
-            int l = (int)((sqrtf(8.0f * (float)i + 1.0f) - 1.0f) * 0.5f);
-            while (l * (l + 1) / 2 > i) --l;
-            while ((l + 1) * (l + 2) / 2 <= i) ++l;
-            const int mp = i - l * (l + 1) / 2;
-            const bool ok = i < nh && l < Lc;
-            v[u] = ok ? side[l * (l + 1) + (neg ? -mp : mp)] : make_double2(0.0, 0.0);
-            wv[u] = ok ? wl[l] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int i = i0 + u * (int)blockDim.x;
-            if (i < nh) hs[i] = make_double2(wv[u] * v[u].x, wv[u] * v[u].y);
-        }
-    }
+    const double *wlg = kt + 3 * RL_MAX + 1;   // w_l = 2 pi sqrt(2/(2l+1))
     for (int k = tid; k <= 2 * Lc; k += blockDim.x) lfac[k] = kt[k];
     for (int l = tid; l < Lc; l += blockDim.x) lq[l] = kt[2 * RL_MAX + 1 + l];
     __syncthreads();
@@ -187,8 +146,8 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
     // 64 from a queue in LDS, so the waves end together (longest items first) instead of the
     // waves holding the long items setting the block's time.  The lanes of a wave run the same
     // number of degree steps, that of its longest chain (lane 0, the smallest q): a lane whose
-    // chain has ended reads the zero row of the side set and zero step constants, so its sums
-    // stay put with no branch in the loop.
+    // chain has ended takes w_l = 0 and zero step constants, so its sums stay put with no
+    // branch in the loop.
     const int lane = tid & 63;
     const uint32_t zi = (uint32_t)rc_base(RL_MAX);   // the zero entry of the step-constant table
     auto item = [&](int P, bool live, int qmin) {
@@ -216,20 +175,24 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
         const double s1a = (v1.lsign && (q & 1)) ? -s1 : s1, s1b = v1.lsign ? -s1a : s1a;
         // the other set's index of each variant (n for X, m for Y; +-q); the side index is +-p
         const int j0 = v0.valid ? (ytype ? v0.m : v0.n) : 0, j1 = v1.valid ? (ytype ? v1.m : v1.n) : 0;
+        const int sp = neg ? -p : p;
         // the step constants of (p, q): degree l at entry rc_base(q) + (l - q)(q + 1) + p
         const uint32_t rq = (uint32_t)(rc_base(q) + p), rs = (uint32_t)(q + 1);
         struct Ops {
             double2 hv, o0, o1;
             double ca, cx, cb;   // step l -> l+1: a, -a x, -b (zero from degree Lc - 1 on)
         };
-        // operands of step t (degree q + t), addresses clamped into the staged sets
+        // operands of step t (degree q + t): the side term times w_l, both sets read at degree
+        // min(l, Lc - 1)
         auto load = [&](Ops &o, int t) {
             const uint32_t l = (uint32_t)(q + t);
-            const uint32_t la = min(l, (uint32_t)Lc), lb = min(l, (uint32_t)(Lc - 1));
-            o.hv = hs[((la * (la + 1)) >> 1) + p];
+            const uint32_t lb = min(l, (uint32_t)(Lc - 1));
             const uint32_t rg = lb * (lb + 1);
-            o.o0 = fs[rg + j0];
-            o.o1 = fs[rg + j1];
+            const double2 hz = side[rg + sp];
+            const double wn = l < (uint32_t)Lc ? wlg[lb] : 0.0;   // zero once the chain has ended
+            o.hv = make_double2(wn * hz.x, wn * hz.y);
+            o.o0 = other[rg + j0];
+            o.o1 = other[rg + j1];
             const uint32_t i = l < (uint32_t)(Lc - 1) ? rq + (uint32_t)t * rs : zi;
             const double *e = rct + 3 * (size_t)i;
             o.ca = e[0];
@@ -303,8 +266,7 @@ __global__ __launch_bounds__(512) void refine_kernel(const double2 *__restrict__
     }
     __syncthreads();
     // the row sums: X quarters sum_p R[b][c][p] e^{-i m alpha_a} (m = +-p), Y quarters
-    // sum_p R[b][a][p] e^{-i n gamma_c} (n = +-p); the row phases reuse the hs space
-    double2 *er = hs;   // [k'][p]
+    // sum_p R[b][a][p] e^{-i n gamma_c} (n = +-p)
     const double sgn = neg ? 1.0 : -1.0;
     for (int i = tid; i < W * Lc; i += blockDim.x) {
         const int k = i / Lc, pr = i % Lc;
@@ -378,7 +340,7 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
     if (n > (1 << 29)) return SO3FT_EINVAL;
     const int Lc = (int)lmax, NM = 2 * Lc - 1;
     // the alpha table [RW][Lc] reuses the f / g space after the main phase
-    const size_t fg = std::max<size_t>((size_t)(Lc + 1) * (Lc + 2) / 2 + (size_t)Lc * Lc, (size_t)RW * Lc);
+    const size_t fg = (size_t)RW * Lc;
     const size_t bytes = sizeof(double2) * ((size_t)RW * Lc * RW + RW * NM + fg);
     static bool attr = false;
     if (!attr) {
@@ -464,7 +426,7 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     // one thread per order pair up to 512 (small degree limits leave room for more blocks per CU)
     const int npq = Lc * (Lc + 1) / 2;
-    const int threads = std::min(512, std::max(64, (npq + 63) / 64 * 64));
+    const int threads = std::min(256, std::max(64, (npq + 63) / 64 * 64));
     hipLaunchKernelGGL(refine_kernel, dim3(4 * n), dim3(threads), bytes, st, reinterpret_cast<const double2 *>(f_lms),
                        reinterpret_cast<const double2 *>(g_lms), bw * bw, (int)n, euler_in, (int)grid_bw, Lc, part,
                        rct, kt);
