@@ -369,6 +369,114 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
     }
 }
 
+// The pass fused with the peak search, complex128 in, nothing stored: a block walks AW
+// consecutive outer indices of one column tile, loading the rows of the next while
+// transforming the current (a read-only pass has no stores to overlap its loads with), and
+// reduces its best (|out|^2, C-order index; the smallest index on ties) once, into
+// k.pv / k.pi [transform][blockIdx.y * gridDim.x + blockIdx.x].
+constexpr int AW = 4;          // outer indices per block
+constexpr int AW_MAXN = 128;   // longer passes keep one index per block (fft_reg_lean): the walk's
+                               // prefetched rows would spill
+
+template <int R1, int R2>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void fft_amax_walk(const double2 *__restrict__ in, const double2 *__restrict__ twg,
+                                                     FftRegK k)
+{
+    constexpr int N = R1 * R2;
+    constexpr int RM = R1 > R2 ? R1 : R2;
+    extern __shared__ double2 sm[];
+    double2 *tw = sm;
+    double *Yd = reinterpret_cast<double *>(sm + N);   // [k1][n2][c]
+    const int C = k.C, tid = threadIdx.x;
+    const int c = tid % C, grp = tid / C;
+    const int64_t t = blockIdx.z;
+    const int cb = k.col0 + (int)blockIdx.x * C;
+    const bool colok = (cb + c) < (k.col0 + k.ncols);
+    const int ob = (int)blockIdx.y * AW;
+    for (int i = tid; i < N; i += blockDim.x) {
+        double2 w = twg[i];
+        if (k.sign > 0) w.y = -w.y;
+        tw[i] = w;
+    }
+    // step-1 rows of outer index o for thread (n2 = grp, c): rows R2 n1 + n2, masked rows zero
+    uint32_t rowmask = 0;
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) {
+        const int r = R2 * n1 + grp;
+        if (colok && grp < R2 && (!k.mask_rows || order_ok2(r, N, k.bw, k.lmax))) rowmask |= 1u << n1;
+    }
+    const uint32_t rstep = (uint32_t)(R2 * k.in_r);
+    auto load = [&](double2 (&dst)[R1], int o) {
+        const bool live = o < N;
+        const double2 *pb = in + (t * k.in_t + (int64_t)min(o, N - 1) * k.in_o + cb + c + (int64_t)grp * k.in_r);
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) {
+            double2 x = make_double2(0.0, 0.0);
+            if (live && ((rowmask >> n1) & 1u)) x = ld_nt(pb + (uint32_t)n1 * rstep);
+            dst[n1] = x;
+        }
+    };
+    double2 raw[R1];
+    load(raw, ob);
+    double bv = -1.0;
+    int64_t bi = INT64_MAX;
+#pragma unroll 1
+    for (int w = 0; w < AW; ++w) {
+        const int o = ob + w;
+        double2 v[RM];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) v[n1] = raw[n1];
+        if (w + 1 < AW) load(raw, o + 1);
+        __syncthreads();   // twiddles visible / the previous index's exchange consumed
+        if (grp < R2) {
+            const int n2 = grp;
+            Dft<R1>::run(v, tw, N / R1, k.sign);
+            // the twiddles are read from LDS here, on every index (held in registers across
+            // the walk they would cost 4 registers each)
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int k1 = 1; k1 < R1; ++k1)
+                if (n2 != 0) v[k1] = cmul2(v[k1], tw[n2 * k1]);
+#pragma unroll
+            for (int k1 = 0; k1 < R1; ++k1) Yd[(k1 * R2 + n2) * C + c] = v[k1].x;
+        }
+        __syncthreads();
+        double re[R2];
+        if (grp < R1) {
+#pragma unroll
+            for (int n2 = 0; n2 < R2; ++n2) re[n2] = Yd[(grp * R2 + n2) * C + c];
+        }
+        __syncthreads();
+        if (grp < R2) {
+#pragma unroll
+            for (int k1 = 0; k1 < R1; ++k1) Yd[(k1 * R2 + grp) * C + c] = v[k1].y;
+        }
+        __syncthreads();
+        if (grp < R1 && o < N) {
+            const int k1 = grp;
+#pragma unroll
+            for (int n2 = 0; n2 < R2; ++n2) v[n2] = make_double2(re[n2], Yd[(k1 * R2 + n2) * C + c]);
+            Dft<R2>::run(v, tw, N / R2, k.sign);
+            if (colok) {
+                const int64_t ib = (int64_t)o * k.out_o + cb + c + (int64_t)k1 * k.out_r;
+#pragma unroll
+                for (int k2 = 0; k2 < R2; ++k2) {
+                    const double2 x = make_double2(v[k2].x * k.scale, v[k2].y * k.scale);
+                    amax_better(bv, bi, x.x * x.x + x.y * x.y, ib + (int64_t)(R1 * k2) * k.out_r);
+                }
+            }
+        }
+    }
+    __shared__ double sv[16];
+    __shared__ int64_t si[16];
+    amax_block_reduce(bv, bi, sv, si);
+    if (threadIdx.x == 0) {
+        const int64_t nb = (int64_t)gridDim.x * gridDim.y;
+        k.pv[t * nb + (int64_t)blockIdx.y * gridDim.x + blockIdx.x] = bv;
+        k.pi[t * nb + (int64_t)blockIdx.y * gridDim.x + blockIdx.x] = bi;
+    }
+}
+
 static bool split_pow2(int N, int &R1, int &R2)
 {
     switch (N) {
@@ -496,7 +604,7 @@ int fft_reg_argmax_partials(int N)
     int R1 = 0, R2 = 0;
     if (!split_pow2(N, R1, R2) || N > 256) return 0;
     const int C = std::min((R1 >= 24) ? 8 : 16, N);
-    return (N + C - 1) / C * N;
+    return (N + C - 1) / C * (N <= AW_MAXN ? (N + AW - 1) / AW : N);
 }
 
 hipError_t launch_fft_reg_argmax(int N, const double2 *tw, int bw, int lmax, const double2 *in, const FftAxis &ax,
@@ -509,28 +617,39 @@ hipError_t launch_fft_reg_argmax(int N, const double2 *tw, int bw, int lmax, con
     k.pv = pv;
     k.pi = pi;
     const int ntile = (N + k.C - 1) / k.C;
-    *nblk = ntile * N;
-    const dim3 grid(ntile, N, T), block((R1 > R2 ? R1 : R2) * k.C);
     const int bytes = (int)(sizeof(double2) * N + sizeof(double) * N * k.C);
+    const dim3 block((R1 > R2 ? R1 : R2) * k.C);
+    if (N > AW_MAXN) {
+        *nblk = ntile * N;
+        const dim3 grid(ntile, N, T);
+        if (N == 192)
+            hipLaunchKernelGGL((fft_reg_lean<16, 12, double2, double2, true>), grid, block, bytes, s, in, nullptr, tw, k);
+        else
+            hipLaunchKernelGGL((fft_reg_lean<16, 16, double2, double2, true>), grid, block, bytes, s, in, nullptr, tw, k);
+        return hipGetLastError();
+    }
+    const int nouter = (N + AW - 1) / AW;
+    *nblk = ntile * nouter;
+    const dim3 grid(ntile, nouter, T);
 #define SO3_AMAX(A, B)                                                                                       \
     do {                                                                                                    \
-        if constexpr (A * B <= 256)                                                                         \
-            hipLaunchKernelGGL((fft_reg_lean<A, B, double2, double2, true>), grid, block, bytes, s, in, nullptr, tw, k); \
+        if constexpr (A * B <= AW_MAXN)                                                                     \
+            hipLaunchKernelGGL((fft_amax_walk<A, B>), grid, block, bytes, s, in, tw, k);                     \
     } while (0)
+    // the smaller radix first: every thread of the block loads rows (R2 >= R1 groups), and
+    // the prefetched rows take fewer registers
     switch (N) {
     case 4: SO3_AMAX(2, 2); break;
-    case 6: SO3_AMAX(3, 2); break;
-    case 8: SO3_AMAX(4, 2); break;
-    case 12: SO3_AMAX(4, 3); break;
+    case 6: SO3_AMAX(2, 3); break;
+    case 8: SO3_AMAX(2, 4); break;
+    case 12: SO3_AMAX(3, 4); break;
     case 16: SO3_AMAX(4, 4); break;
-    case 24: SO3_AMAX(6, 4); break;
-    case 32: SO3_AMAX(8, 4); break;
-    case 48: SO3_AMAX(8, 6); break;
+    case 24: SO3_AMAX(4, 6); break;
+    case 32: SO3_AMAX(4, 8); break;
+    case 48: SO3_AMAX(6, 8); break;
     case 64: SO3_AMAX(8, 8); break;
-    case 96: SO3_AMAX(12, 8); break;
-    case 128: SO3_AMAX(16, 8); break;
-    case 192: SO3_AMAX(16, 12); break;
-    default: SO3_AMAX(16, 16); break;
+    case 96: SO3_AMAX(8, 12); break;
+    default: SO3_AMAX(8, 16); break;
     }
 #undef SO3_AMAX
     return hipGetLastError();
