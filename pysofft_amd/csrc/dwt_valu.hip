@@ -461,16 +461,18 @@ __device__ __forceinline__ double2 sh_coeff(const double2 *__restrict__ x, int l
 // reads its sets at t * t_ts; nr shells r at stride r_ts are summed with weights w (w ==
 // nullptr: one shell of weight 1).  Degrees l > lcap are zero (the Risbo loops of
 // soft.f08:2564-2580, 3046-3074 stop at l = lmax).  One block per order pair and transform.
-__global__ __launch_bounds__(64) void corr_coeffs_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g,
-                                                         int64_t t_ts, int layout, int nr, int64_t r_ts,
-                                                         const double *__restrict__ w, double2 *__restrict__ out,
-                                                         int64_t c_ts, const int64_t *__restrict__ slice, int bw,
-                                                         int lcap)
+__global__ __launch_bounds__(256) void corr_coeffs_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g,
+                                                          int64_t t_ts, int layout, int nr, int64_t r_ts,
+                                                          const double *__restrict__ w, double2 *__restrict__ out,
+                                                          int64_t c_ts, const int64_t *__restrict__ slice, int bw,
+                                                          int lcap)
 {
-    // block = order pair 0 <= m1 <= m2 < bw (triangular index blockIdx.x, m1 outer) of
+    // wave = order pair 0 <= m1 <= m2 < bw (triangular index, m1 outer; four per block) of
     // transform blockIdx.y: its variant slices are contiguous in the mnl layout, so the
-    // block's lanes write consecutive entries
-    const int P = blockIdx.x;
+    // wave's lanes write consecutive entries
+    const int P = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (P >= bw * (bw + 1) / 2) return;
     const int64_t t = blockIdx.y;
     const double B2 = 2.0 * bw + 1.0;
     int m1 = (int)((B2 - sqrt(B2 * B2 - 8.0 * P)) * 0.5);
@@ -484,13 +486,8 @@ __global__ __launch_bounds__(64) void corr_coeffs_kernel(const double2 *__restri
     // (-m2,m1); m1 = m2 keeps k = 0, 1, 4, 5, m1 = 0 keeps k = 0..3, (0, 0) only k = 0
     const int nv = (m1 == 0 && m2 == 0) ? 1 : ((m1 == 0 || m1 == m2) ? 4 : 8);
     const int64_t base = slice[(int64_t)(m1 + bw - 1) * (2 * bw - 1) + (m2 + bw - 1)];
-    // 2 pi sqrt(2/(2l+1)) of the pair's degrees, once per block
-    __shared__ double wl[512];
-    for (int i = threadIdx.x; i < L; i += blockDim.x)
-        wl[i] = 2.0 * 3.14159265358979323846 * sqrt(2.0 / (double)(2 * (m2 + i) + 1));
-    __syncthreads();
     const float rL = 1.0f / (float)L;
-    for (int o = threadIdx.x; o < nv * L; o += blockDim.x) {
+    for (int o = lane; o < nv * L; o += 64) {
         int v = (int)(((float)o + 0.5f) * rL), li = o - v * L;   // o = v L + li, 0 <= li < L
         if (li < 0) { --v; li += L; } else if (li >= L) { ++v; li -= L; }
         const int l = m2 + li;
@@ -515,7 +512,8 @@ __global__ __launch_bounds__(64) void corr_coeffs_kernel(const double2 *__restri
             re += wr * (a.x * b.x + a.y * b.y);
             im += wr * (a.y * b.x - a.x * b.y);
         }
-        const double wn = ((m + n) & 1) ? -wl[li] : wl[li];
+        const double wl = 2.0 * 3.14159265358979323846 * sqrt(2.0 / (double)(2 * l + 1));   // 2 pi sqrt(2/(2l+1))
+        const double wn = ((m + n) & 1) ? -wl : wl;
         out[t * c_ts + base + o] = make_double2(wn * re, wn * im);
     }
 }
@@ -525,8 +523,8 @@ hipError_t launch_corr_coeffs(const DeviceTables &tb, int bw, const double2 *f, 
                               int lcap, int T, hipStream_t s)
 {
     const int npq = bw * (bw + 1) / 2;
-    hipLaunchKernelGGL(corr_coeffs_kernel, dim3(npq, T), dim3(64), 0, s, f, g, t_ts, layout, nr, r_ts, w, out, c_ts,
-                       tb.slice, bw, lcap);
+    hipLaunchKernelGGL(corr_coeffs_kernel, dim3((npq + 3) / 4, T), dim3(256), 0, s, f, g, t_ts, layout, nr, r_ts, w,
+                       out, c_ts, tb.slice, bw, lcap);
     return hipGetLastError();
 }
 
