@@ -374,7 +374,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
 // transforming the current (a read-only pass has no stores to overlap its loads with), and
 // reduces its best (|out|^2, C-order index; the smallest index on ties) once, into
 // k.pv / k.pi [transform][blockIdx.y * gridDim.x + blockIdx.x].
-constexpr int AW = 4;          // outer indices per block
+constexpr int AW = 8;          // outer indices per block
 constexpr int AW_MAXN = 128;   // longer passes keep one index per block (fft_reg_lean): the walk's
                                // prefetched rows would spill
 
