@@ -10,7 +10,7 @@
 // d~^q_{p,q}(b) = sqrt((2q+1)/2) sqrt((2q)!/((q+p)!(q-p)!)) cos(b/2)^(q+p) sin(b/2)^(q-p)
 // (the seeds of wigner.f90:130-187 for 0 < b < pi), and the 8-variant symmetry table.
 //
-// Four blocks per pair (quarters of the symmetric variants; see refine_kernel), then a
+// Two blocks per pair (halves of the symmetric variants; see refine_kernel), then a
 // finalize kernel for |C| and the argmax.  No counterpart in the reference (its callers
 // evaluate the whole grid).
 #include <cmath>
@@ -67,38 +67,34 @@ __device__ inline void refine_window(const double *ein, int64_t s, int Bg, int (
     }
 }
 
-// Four blocks per pair, one per quarter of the 8 symmetric variants of every order pair
-// 0 <= p <= q < Lc, chosen so that every variant of a quarter has its "row" index +p or
-// -p:
-//   quarter 0 (X+): v0 (p, q) at beta,  v4 (p, -q) at pi - beta;  rows m = +p
-//   quarter 1 (X-): v3 (-p, -q) at beta, v5 (-p, q) at pi - beta; rows m = -p
-//   quarter 2 (Y+): v2 (q, p) at beta,  v7 (-q, p) at pi - beta;  rows n = +p
-//   quarter 3 (Y-): v1 (-q, -p) at beta, v6 (q, -p) at pi - beta; rows n = -p
-// X quarters fold F(beta, m, n) over n with e^{-i n gamma_c} into R[beta][c][p], Y
-// quarters over m with e^{-i m alpha_a} into R[beta][a][p]; the remaining sum over the
-// row index follows at the end.  Work items are the order pairs in q-major order: the
-// lanes of a wave run recurrences of about equal length (Lc - q) and write distinct rows p.
-// Each item runs the recurrence at the 5 window betas and at pi - beta (10 chains) and
-// accumulates F over l in registers, reading the coefficient sets from global memory
-// (L2; the pair's four blocks share them).
-// dynamic LDS: R [RW b][RW k][Lc p] (real, imaginary) | ep [RW k][2Lc-1] fold phases | er
-// [RW k'][Lc p] row phases: 40 KB at Lc = 64, so three 256-thread blocks share a CU and one
-// block's set-up and row sums overlap the others' degree loops
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void refine_kernel(
+// Two blocks per pair, one per half of the 8 symmetric variants of every order pair
+// 0 <= p <= q < Lc.  Each half holds the variants whose "row" index is the first (X) or the
+// second (Y) index, at +p or -p:
+//   X: v0 (p, q), v3 (-p, -q) at beta; v4 (p, -q), v5 (-p, q) at pi - beta; rows m = +-p
+//   Y: v2 (q, p), v1 (-q, -p) at beta; v7 (-q, p), v6 (q, -p) at pi - beta; rows n = +-p
+// so the four variants of an item share the recurrence chains (5 betas and pi - beta) and
+// read the other set at +-q only.  X halves fold F(beta, m, n) over n with e^{-i n gamma_c}
+// into R+-[beta][c][p] (rows m = +-p), Y halves over m with e^{-i m alpha_a} into
+// R+-[beta][a][p]; the remaining sum over the row index follows at the end.  Work items are
+// the order pairs in q-major order: the lanes of a wave run recurrences of about equal length
+// (Lc - q) and write distinct rows p.  Each item accumulates F over l in registers, reading
+// the coefficient sets from global memory (L2; the pair's two blocks share them).
+// dynamic LDS: R+, R- [RW b][RW k][Lc p] (real, imaginary arrays) | ep [RW k][2Lc-1] fold
+// phases | er [2][RW k'][Lc p] row phases: 72 KB at Lc = 64, two 256-thread blocks per CU
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void refine_kernel(
     const double2 *__restrict__ f, const double2 *__restrict__ g, int64_t nb, int n, const double *__restrict__ ein,
     int Bg, int Lc, double2 *__restrict__ part, const double *__restrict__ rct, const double *__restrict__ kt)
 {
-    const int s = blockIdx.x >> 2, quarter = blockIdx.x & 3;
+    const int s = blockIdx.x >> 1, ytype = blockIdx.x & 1;
     if (s >= n) return;
-    const int ytype = quarter >> 1, neg = quarter & 1;
     constexpr int W = RW;
-    const int NM = 2 * Lc - 1;
+    const int NM = 2 * Lc - 1, NR = W * Lc * W;
     extern __shared__ double sm[];
-    // R [b][k][p] as real and imaginary arrays, p fastest: the lanes' 8-byte atomics on
+    // R+- [b][k][p] as real and imaginary arrays, p fastest: the lanes' 8-byte atomics on
     // consecutive p fill the banks without conflicts
-    double *Rr = sm, *Ri = sm + W * Lc * W;
-    double2 *ep = reinterpret_cast<double2 *>(sm + 2 * W * Lc * W);   // [k][j + Lc - 1]: e^{-i j gamma_k} (X) / e^{-i j alpha_k} (Y)
-    double2 *er = ep + W * NM;                         // [k'][p]: row phases of the last phase
+    double *Rr = sm, *Ri = sm + NR;                    // rows +p; rows -p at + 2 NR
+    double2 *ep = reinterpret_cast<double2 *>(sm + 4 * NR);   // [k][j + Lc - 1]: e^{-i j gamma_k} (X) / e^{-i j alpha_k} (Y)
+    double2 *er = ep + W * NM;                         // [sign][k'][p]: row phases of the last phase
     __shared__ double lfac[2 * RL_MAX + 1];            // log k!
     __shared__ double lq[RL_MAX];                      // log sqrt((2q+1)/2)
     __shared__ double lcb[2 * W], lsb[2 * W], cbv[2 * W];  // log cos(b/2), log sin(b/2), cos b; b, pi - b
@@ -119,7 +115,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
         }
     }
     const double2 *side = (ytype ? g : f) + (int64_t)s * nb, *other = (ytype ? f : g) + (int64_t)s * nb;
-    for (int i = tid; i < W * Lc * W; i += blockDim.x) Rr[i] = Ri[i] = 0.0;
+    for (int i = tid; i < 4 * NR; i += blockDim.x) sm[i] = 0.0;
     const double *wlg = kt + 3 * RL_MAX + 1;   // w_l = 2 pi sqrt(2/(2l+1))
     for (int k = tid; k <= 2 * Lc; k += blockDim.x) lfac[k] = kt[k];
     for (int l = tid; l < Lc; l += blockDim.x) lq[l] = kt[2 * RL_MAX + 1 + l];
@@ -137,9 +133,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
     }
     __syncthreads();
 
-    // variant at beta (va) and at pi - beta (vb) of this quarter
-    const int va = ytype ? (neg ? 1 : 2) : (neg ? 3 : 0);
-    const int vb = ytype ? (neg ? 6 : 7) : (neg ? 5 : 4);
+    // the half's variants at beta (a: rows +p, c: rows -p) and at pi - beta (b: +p, d: -p)
+    const int va = ytype ? 2 : 0, vc = ytype ? 1 : 3, vb = ytype ? 7 : 4, vd = ytype ? 6 : 5;
     const double cb[2 * W] = {cbv[0], cbv[1], cbv[2], cbv[3], cbv[4], cbv[5], cbv[6], cbv[7], cbv[8], cbv[9]};
     const int npq = Lc * (Lc + 1) / 2;
     // items in q-major order, i.e. by decreasing recurrence length; each wave claims the next
@@ -155,7 +150,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
         while (q * (q + 1) / 2 > P) --q;
         while ((q + 1) * (q + 2) / 2 <= P) ++q;
         const int p = P - q * (q + 1) / 2;
-        const Variant v0 = variant(va, p, q), v1 = variant(vb, p, q);
         // seeds d~^q_{p,q} at beta and pi - beta (closed form, log domain)
         const double lc = 0.5 * (lfac[2 * q] - lfac[q + p] - lfac[q - p]) + lq[q];
         double d[2 * W], dp[2 * W];
@@ -164,50 +158,64 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
             d[bb] = exp(lc + (double)(q + p) * lcb[bb] + (double)(q - p) * lsb[bb]);
             dp[bb] = 0.0;
         }
-        double2 F0[W], F1[W];
+        // F of the variants a, c (at beta) and b, d (at pi - beta)
+        double2 Fa[W], Fb[W], Fc[W], Fd[W];
 #pragma unroll
-        for (int bb = 0; bb < W; ++bb) { F0[bb] = make_double2(0.0, 0.0); F1[bb] = make_double2(0.0, 0.0); }
-        // signs (-1)^(m+n) x variant sign, with the variant's (-1)^l for the even (A) and the
-        // odd (B) steps; zero for an invalid variant or a padding lane
-        const double s0 = (live && v0.valid) ? (((v0.m + v0.n) & 1) ? -1.0 : 1.0) * v0.sign : 0.0;
-        const double s1 = (live && v1.valid) ? (((v1.m + v1.n) & 1) ? -1.0 : 1.0) * v1.sign : 0.0;
-        const double s0a = (v0.lsign && (q & 1)) ? -s0 : s0, s0b = v0.lsign ? -s0a : s0a;
-        const double s1a = (v1.lsign && (q & 1)) ? -s1 : s1, s1b = v1.lsign ? -s1a : s1a;
-        // the other set's index of each variant (n for X, m for Y; +-q); the side index is +-p
-        const int j0 = v0.valid ? (ytype ? v0.m : v0.n) : 0, j1 = v1.valid ? (ytype ? v1.m : v1.n) : 0;
-        const int sp = neg ? -p : p;
-        // the step constants of (p, q): degree l at entry rc_base(q) + (l - q)(q + 1) + p
-        const uint32_t rq = (uint32_t)(rc_base(q) + p), rs = (uint32_t)(q + 1);
+        for (int bb = 0; bb < W; ++bb)
+            Fa[bb] = Fb[bb] = Fc[bb] = Fd[bb] = make_double2(0.0, 0.0);
+        // signs (-1)^(m+n) x variant sign, with the variant's (-1)^l for the even (.0) and the
+        // odd (.1) steps; zero for an invalid variant or a padding lane
+        auto sgn2 = [&](int v, double (&o)[2]) {
+            const Variant vr = variant(v, p, q);
+            const double s0 = (live && vr.valid) ? (((vr.m + vr.n) & 1) ? -1.0 : 1.0) * vr.sign : 0.0;
+            o[0] = (vr.lsign && (q & 1)) ? -s0 : s0;
+            o[1] = vr.lsign ? -o[0] : o[0];
+        };
+        double sa[2], sb[2], sc[2], sd[2];
+        sgn2(va, sa);
+        sgn2(vb, sb);
+        sgn2(vc, sc);
+        sgn2(vd, sd);
+        // the other set at +q (variants a, d) and -q (b, c); the side set at +p (a, b), -p (c, d)
         struct Ops {
-            double2 hv, o0, o1;
+            double2 hp, hm, oq, om;
             double ca, cx, cb;   // step l -> l+1: a, -a x, -b (zero from degree Lc - 1 on)
         };
-        // operands of step t (degree q + t): the side term times w_l, both sets read at degree
+        // the step constants of (p, q): degree l at entry rc_base(q) + (l - q)(q + 1) + p
+        const uint32_t rq = (uint32_t)(rc_base(q) + p), rs = (uint32_t)(q + 1);
+        // operands of step t (degree q + t): the side terms times w_l, both sets read at degree
         // min(l, Lc - 1)
         auto load = [&](Ops &o, int t) {
             const uint32_t l = (uint32_t)(q + t);
             const uint32_t lb = min(l, (uint32_t)(Lc - 1));
             const uint32_t rg = lb * (lb + 1);
-            const double2 hz = side[rg + sp];
+            const double2 zp = side[rg + p], zm = side[rg - p];
             const double wn = l < (uint32_t)Lc ? wlg[lb] : 0.0;   // zero once the chain has ended
-            o.hv = make_double2(wn * hz.x, wn * hz.y);
-            o.o0 = other[rg + j0];
-            o.o1 = other[rg + j1];
+            o.hp = make_double2(wn * zp.x, wn * zp.y);
+            o.hm = make_double2(wn * zm.x, wn * zm.y);
+            o.oq = other[rg + q];
+            o.om = other[rg - q];
             const uint32_t i = l < (uint32_t)(Lc - 1) ? rq + (uint32_t)t * rs : zi;
             const double *e = rct + 3 * (size_t)i;
             o.ca = e[0];
             o.cx = e[1];
             o.cb = e[2];
         };
-        // one degree: F += cc^l d^l (side x conj(other); the Y quarters' conjugate is taken
-        // in the fold), then d^{l+1}
-        auto step = [&](const Ops &cu, double sa, double sb) {
-            const double2 z0 = cmulc(cu.hv, cu.o0), z1 = cmulc(cu.hv, cu.o1);
-            const double2 c0 = make_double2(sa * z0.x, sa * z0.y), c1 = make_double2(sb * z1.x, sb * z1.y);
+        // one degree: F += cc^l d^l (side x conj(other); the Y halves' conjugate is taken in
+        // the fold), then d^{l+1}
+        auto step = [&](const Ops &cu, int par) {
+            const double2 za = cmulc(cu.hp, cu.oq), zb = cmulc(cu.hp, cu.om);
+            const double2 zc = cmulc(cu.hm, cu.om), zd = cmulc(cu.hm, cu.oq);
+            const double2 ka = make_double2(sa[par] * za.x, sa[par] * za.y);
+            const double2 kb = make_double2(sb[par] * zb.x, sb[par] * zb.y);
+            const double2 kc = make_double2(sc[par] * zc.x, sc[par] * zc.y);
+            const double2 kd = make_double2(sd[par] * zd.x, sd[par] * zd.y);
 #pragma unroll
             for (int bb = 0; bb < W; ++bb) {
-                F0[bb].x = fma(c0.x, d[bb], F0[bb].x); F0[bb].y = fma(c0.y, d[bb], F0[bb].y);
-                F1[bb].x = fma(c1.x, d[W + bb], F1[bb].x); F1[bb].y = fma(c1.y, d[W + bb], F1[bb].y);
+                Fa[bb].x = fma(ka.x, d[bb], Fa[bb].x); Fa[bb].y = fma(ka.y, d[bb], Fa[bb].y);
+                Fc[bb].x = fma(kc.x, d[bb], Fc[bb].x); Fc[bb].y = fma(kc.y, d[bb], Fc[bb].y);
+                Fb[bb].x = fma(kb.x, d[W + bb], Fb[bb].x); Fb[bb].y = fma(kb.y, d[W + bb], Fb[bb].y);
+                Fd[bb].x = fma(kd.x, d[W + bb], Fd[bb].x); Fd[bb].y = fma(kd.y, d[W + bb], Fd[bb].y);
             }
 #pragma unroll
             for (int bb = 0; bb < 2 * W; ++bb) {
@@ -226,27 +234,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
         for (int t = 0; t < nst; t += 2) {
             load(B, t + 1);
             __builtin_amdgcn_sched_barrier(0);
-            step(A, s0a, s1a);
+            step(A, 0);
             __builtin_amdgcn_sched_barrier(0);
             load(A, t + 2);
             __builtin_amdgcn_sched_barrier(0);
-            step(B, s0b, s1b);
+            step(B, 1);
             __builtin_amdgcn_sched_barrier(0);
         }
-        // fold the other index's phase into R[b][k][p] (distinct p across the lanes of one q):
-        // both variants of the quarter share the row p, so one sum per entry
+        // fold the other index's phase into R+-[b][k][p] (distinct p across the lanes of one
+        // q): a and b into the rows +p, c and d into the rows -p
         if (live) {
-            const double cy = ytype ? -1.0 : 1.0;   // the Y quarters' conjugate
+            const double cy = ytype ? -1.0 : 1.0;   // the Y halves' conjugate
 #pragma unroll 1
             for (int k = 0; k < W; ++k) {
-                const double2 e0 = ep[k * NM + j0 + Lc - 1], e1 = ep[k * NM + j1 + Lc - 1];
+                const double2 eq = ep[k * NM + q + Lc - 1], em = ep[k * NM - q + Lc - 1];
 #pragma unroll
                 for (int bb = 0; bb < W; ++bb) {
-                    const double2 z0 = make_double2(F0[bb].x, cy * F0[bb].y);
-                    const double2 z1 = make_double2(F1[bb].x, cy * F1[bb].y);
+                    const double2 za = make_double2(Fa[bb].x, cy * Fa[bb].y);
+                    const double2 zb = make_double2(Fb[bb].x, cy * Fb[bb].y);
+                    const double2 zc = make_double2(Fc[bb].x, cy * Fc[bb].y);
+                    const double2 zd = make_double2(Fd[bb].x, cy * Fd[bb].y);
                     const int o = (bb * W + k) * Lc + p;
-                    atomicAdd(&Rr[o], (z0.x * e0.x - z0.y * e0.y) + (z1.x * e1.x - z1.y * e1.y));
-                    atomicAdd(&Ri[o], (z0.x * e0.y + z0.y * e0.x) + (z1.x * e1.y + z1.y * e1.x));
+                    atomicAdd(&Rr[o], (za.x * eq.x - za.y * eq.y) + (zb.x * em.x - zb.y * em.y));
+                    atomicAdd(&Ri[o], (za.x * eq.y + za.y * eq.x) + (zb.x * em.y + zb.y * em.x));
+                    atomicAdd(&Rr[2 * NR + o], (zc.x * em.x - zc.y * em.y) + (zd.x * eq.x - zd.y * eq.y));
+                    atomicAdd(&Ri[2 * NR + o], (zc.x * em.y + zc.y * em.x) + (zd.x * eq.y + zd.y * eq.x));
                 }
             }
         }
@@ -265,31 +277,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
         item(P, live, q0);
     }
     __syncthreads();
-    // the row sums: X quarters sum_p R[b][c][p] e^{-i m alpha_a} (m = +-p), Y quarters
-    // sum_p R[b][a][p] e^{-i n gamma_c} (n = +-p)
-    const double sgn = neg ? 1.0 : -1.0;
-    for (int i = tid; i < W * Lc; i += blockDim.x) {
-        const int k = i / Lc, pr = i % Lc;
+    // the row sums: X halves sum_p R+-[b][c][p] e^{-i m alpha_a} (m = +-p), Y halves
+    // sum_p R+-[b][a][p] e^{-i n gamma_c} (n = +-p)
+    for (int i = tid; i < 2 * W * Lc; i += blockDim.x) {
+        const int sg = i / (W * Lc), k = (i / Lc) % W, pr = i % Lc;
         double sn, cs;
         sincos((double)pr * (ytype ? win[2][k] : win[0][k]) * h, &sn, &cs);
-        er[i] = make_double2(cs, sgn * sn);
+        er[i] = make_double2(cs, sg ? sn : -sn);
     }
     __syncthreads();
     for (int o = tid; o < W * W * W; o += blockDim.x) {
         const int a = o / (W * W), b = (o / W) % W, c = o % W;
         const int kf = ytype ? a : c, kr = ytype ? c : a;   // fold index in R, row-phase index
         double re = 0.0, im = 0.0;
-        for (int pr = 0; pr < Lc; ++pr) {
-            const int ro = (b * W + kf) * Lc + pr;
-            const double2 z = make_double2(Rr[ro], Ri[ro]), e = er[kr * Lc + pr];
-            re += z.x * e.x - z.y * e.y;
-            im += z.x * e.y + z.y * e.x;
-        }
-        part[((int64_t)s * 4 + quarter) * (W * W * W) + o] = make_double2(re, im);
+        for (int sg = 0; sg < 2; ++sg)
+            for (int pr = 0; pr < Lc; ++pr) {
+                const int ro = 2 * sg * NR + (b * W + kf) * Lc + pr;
+                const double2 z = make_double2(Rr[ro], Ri[ro]), e = er[(sg * W + kr) * Lc + pr];
+                re += z.x * e.x - z.y * e.y;
+                im += z.x * e.y + z.y * e.x;
+            }
+        part[((int64_t)s * 2 + ytype) * (W * W * W) + o] = make_double2(re, im);
     }
 }
 
-// |C| on the window from the four quarters, argmax (ties: the window centre, then the first
+// |C| on the window from the two halves, argmax (ties: the window centre, then the first
 // C-order index), euler_out / score
 __global__ void refine_finalize(const double2 *__restrict__ part, int n, const double *__restrict__ ein, int Bg,
                                 double *__restrict__ eout, double *__restrict__ score)
@@ -301,8 +313,8 @@ __global__ void refine_finalize(const double2 *__restrict__ part, int n, const d
     const double pi = 3.14159265358979323846;
     for (int o = threadIdx.x; o < W * W * W; o += blockDim.x) {
         double re = 0.0, im = 0.0;
-        for (int k = 0; k < 4; ++k) {
-            const double2 u = part[((int64_t)s * 4 + k) * (W * W * W) + o];
+        for (int k = 0; k < 2; ++k) {
+            const double2 u = part[((int64_t)s * 2 + k) * (W * W * W) + o];
             re += u.x;
             im += u.y;
         }
@@ -340,8 +352,8 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
     if (n > (1 << 29)) return SO3FT_EINVAL;
     const int Lc = (int)lmax, NM = 2 * Lc - 1;
     // the alpha table [RW][Lc] reuses the f / g space after the main phase
-    const size_t fg = (size_t)RW * Lc;
-    const size_t bytes = sizeof(double2) * ((size_t)RW * Lc * RW + RW * NM + fg);
+    // R+- (2 x RW Lc RW complex), fold phases RW (2Lc - 1), row phases 2 x RW Lc
+    const size_t bytes = sizeof(double2) * (2 * (size_t)RW * Lc * RW + RW * NM + 2 * (size_t)RW * Lc);
     static bool attr = false;
     if (!attr) {
         if (hipFuncSetAttribute((const void *)refine_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024) !=
@@ -350,7 +362,7 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
         attr = true;
     }
     if (bytes > 150 * 1024) return SO3FT_EINVAL;
-    // the four quarters' window sums, per device (grown on demand; calls on one device share it)
+    // the two halves' window sums, per device (grown on demand; calls on one device share it)
     double2 *part = nullptr;
     {
         static std::mutex mu;
@@ -366,7 +378,7 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
             }
             e.first = nullptr;
             e.second = 0;
-            if (hipMalloc((void **)&e.first, sizeof(double2) * 4 * RW * RW * RW * (size_t)n) != hipSuccess) {
+            if (hipMalloc((void **)&e.first, sizeof(double2) * 2 * RW * RW * RW * (size_t)n) != hipSuccess) {
                 hipGetLastError();
                 return SO3FT_ENOMEM;
             }
@@ -427,7 +439,7 @@ int so3ft_refine_correlation_peaks(int64_t bw, const void *f_lms, const void *g_
     // one thread per order pair up to 512 (small degree limits leave room for more blocks per CU)
     const int npq = Lc * (Lc + 1) / 2;
     const int threads = std::min(256, std::max(64, (npq + 63) / 64 * 64));
-    hipLaunchKernelGGL(refine_kernel, dim3(4 * n), dim3(threads), bytes, st, reinterpret_cast<const double2 *>(f_lms),
+    hipLaunchKernelGGL(refine_kernel, dim3(2 * n), dim3(threads), bytes, st, reinterpret_cast<const double2 *>(f_lms),
                        reinterpret_cast<const double2 *>(g_lms), bw * bw, (int)n, euler_in, (int)grid_bw, Lc, part,
                        rct, kt);
     if (hipGetLastError() != hipSuccess) return SO3FT_EHIP;
