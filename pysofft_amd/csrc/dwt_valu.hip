@@ -460,7 +460,7 @@ __device__ __forceinline__ double2 sh_coeff(const double2 *__restrict__ x, int l
 // branches are this formula with the negative orders completed).  Transform t = blockIdx.y
 // reads its sets at t * t_ts; nr shells r at stride r_ts are summed with weights w (w ==
 // nullptr: one shell of weight 1).  Degrees l > lcap are zero (the Risbo loops of
-// soft.f08:2564-2580, 3046-3074 stop at l = lmax).  One block per order pair and transform.
+// soft.f08:2564-2580, 3046-3074 stop at l = lmax).  One wave per order pair and transform.
 __global__ __launch_bounds__(256) void corr_coeffs_kernel(const double2 *__restrict__ f, const double2 *__restrict__ g,
                                                           int64_t t_ts, int layout, int nr, int64_t r_ts,
                                                           const double *__restrict__ w, double2 *__restrict__ out,
