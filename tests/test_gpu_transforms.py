@@ -461,6 +461,28 @@ def test_correlation_peaks_equal_full_grid_argmax(bw):
     torch.testing.assert_close(val, m.max(1).values, rtol=1e-14, atol=0)
 
 
+@pytest.mark.parametrize("bw,lmax,rt", [(24, 15, 0), (64, 40, 0), (16, 9, 1)])
+def test_correlation_peaks_lmax_below_bw(bw, lmax, rt):
+    """The fused peak search on handles that drop degrees (and orders) above lmax -- the
+    last pass then reads only the rows of orders |m| <= lmax -- returns the argmax of the
+    same handle's full correlation grids and that maximum (Kostelec and Risbo handles)."""
+    rng = np.random.default_rng(700 + bw)
+    s = _soft(bw, lmax=lmax, recurrence_type=rt)
+    n = 3
+    f = rng.standard_normal((n, bw * bw)) + 1j * rng.standard_normal((n, bw * bw))
+    g = rng.standard_normal((n, bw * bw)) + 1j * rng.standard_normal((n, bw * bw))
+    ft, gt = torch.from_numpy(f).cuda(), torch.from_numpy(g).cuda()
+    peak, val = s.correlation_peaks(ft, gt)
+    cc = s.cross_correlation_ylm_cmplx_many(ft, gt)
+    m = cc.abs().reshape(n, -1)
+    flat = m.argmax(1)
+    N = 2 * bw
+    ig, rem = flat // (N * N), flat % (N * N)
+    ref = torch.stack([rem // N, rem % N, ig], 1)
+    assert torch.equal(peak, ref)
+    torch.testing.assert_close(val, m.max(1).values, rtol=1e-14, atol=0)
+
+
 def test_workload_bw256_real_round_trip():
     """Real FP64 transform at bw=256: Hermitian-symmetric coefficients (iid N(0,1) re/im
     scaled by (2l+1)^-1/2), irsoft onto the 512^3 float64 grid, rsoft back: relative L2
