@@ -134,6 +134,41 @@ def test_refine_window_matches_full_correlation_grid():
         np.testing.assert_allclose(e[i], [a[ia[ja]], b[ib[jb]], a[ig[jg]]], atol=1e-12)
 
 
+def test_refine_window_on_a_finer_grid_matches_padded_correlation():
+    """A refinement level whose grid bandwidth exceeds the sets' (grid 32, degrees < 16, the
+    shape of the last ladder level) returns the max |C| over the 5 x 5 x 5 window of the
+    bandwidth-32 correlation grid of the zero-padded sets, at its location."""
+    from pysofft_amd import Soft
+    from pysofft_amd.align import Aligner, AlignmentMLP
+    bw, gb, n = 16, 32, 5
+    rng = np.random.default_rng(31)
+    f = rng.standard_normal((n, bw * bw)) + 1j * rng.standard_normal((n, bw * bw))
+    g = rng.standard_normal((n, bw * bw)) + 1j * rng.standard_normal((n, bw * bw))
+    fp = np.zeros((n, gb * gb), complex)
+    gp = np.zeros((n, gb * gb), complex)
+    fp[:, :bw * bw] = f   # LMc order: degrees < bw come first
+    gp[:, :bw * bw] = g
+    cc = np.asarray(Soft(gb).cross_correlation_ylm_cmplx_many(fp, gp))
+    N = 2 * gb
+    a = np.arange(N) * np.pi / gb
+    b = (np.arange(N) + 0.5) * np.pi / N
+    start = rng.integers(0, N, (n, 3))
+    start[:, 1] = np.clip(start[:, 1], 2, N - 3)
+    e0 = np.stack([a[start[:, 0]], b[start[:, 1]], a[start[:, 2]]], 1)
+    al = Aligner(mlp=AlignmentMLP((452, 16, 6)))
+    e, score = al.refine(torch.from_numpy(f).cuda(), torch.from_numpy(g).cuda(), torch.from_numpy(e0).cuda(),
+                         levels=[(gb, bw)])
+    e, score = e.cpu().numpy(), score.cpu().numpy()
+    for i in range(n):
+        ia = (start[i, 0] + np.arange(-2, 3)) % N
+        ib = start[i, 1] + np.arange(-2, 3)
+        ig = (start[i, 2] + np.arange(-2, 3)) % N
+        win = np.abs(cc[i][np.ix_(ig, ia, ib)])
+        assert abs(score[i] - win.max()) < 1e-10 * win.max(), i
+        jg, ja, jb = np.unravel_index(np.argmax(win), win.shape)
+        np.testing.assert_allclose(e[i], [a[ia[ja]], b[ib[jb]], a[ig[jg]]], atol=1e-12)
+
+
 def test_training_reduces_loss_and_beats_untrained_proposals():
     """A short training run on generated pairs lowers the loss and the proposals' angular
     error against the untrained network (what the learned part gains over none)."""
