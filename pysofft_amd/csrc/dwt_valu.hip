@@ -134,7 +134,10 @@ __device__ __forceinline__ bool decode_unit(int nch, int nunit, int &unit, int &
 // Inverse.  Block = one wave, lanes = 64 betas of one chunk; TB transforms per wave.
 // Work unit = (pair, transform group), pairs fastest.  TAB: d~ is read from the plan's
 // d table (rows of the pair's degrees, the lane's beta) instead of the recurrence.
-template <int TB, bool TAB>
+// CORR: the coefficients are the correlation products cc^l_{mn} of SH pairs (coeff = f,
+// aux = g, LMc sets at stride c_ts; soft.f08:2242-2341), formed as each is loaded instead
+// of being staged in mnl order first (degrees l > lcap are zero).
+template <int TB, bool TAB, bool CORR>
 __global__ __launch_bounds__(64) void dwt_inverse_valu(const double2 *__restrict__ coeff, int64_t c_ts,
                                                        double2 *__restrict__ F, int64_t f_ts, int T,
                                                        const double *__restrict__ cosb,
@@ -142,7 +145,8 @@ __global__ __launch_bounds__(64) void dwt_inverse_valu(const double2 *__restrict
                                                        const int64_t *__restrict__ slice,
                                                        const int32_t *__restrict__ pairs, int bw, int N, int mode,
                                                        int npair, int nch, int nunit,
-                                                       const double *__restrict__ dtab)
+                                                       const double *__restrict__ dtab,
+                                                       const double2 *__restrict__ aux, int lcap)
 {
     // recurrence constants (dynamic LDS, 3 (bw + 2) doubles); the last step of a tail
     // reads up to entry 3 L + 2
@@ -178,12 +182,32 @@ __global__ __launch_bounds__(64) void dwt_inverse_valu(const double2 *__restrict
     const int64_t offk = variant(kv, p, q).valid ? inv_src(mode, kv, p, q, bw, slice).off : off0;
     const double2 *src[TB];
 #pragma unroll
-    for (int t = 0; t < TB; ++t) src[t] = coeff + (int64_t)(t0 + min(t, nt - 1)) * c_ts + offk + kh;
+    for (int t = 0; t < TB; ++t)
+        src[t] = coeff + (int64_t)(t0 + min(t, nt - 1)) * c_ts + (CORR ? 0 : offk + kh);
+    // CORR: the lane's variant (variant 0 for an invalid one) and the g sets
+    const Variant vk = variant(variant(kv, p, q).valid ? kv : 0, p, q);
+    const double2 *gsr[TB];
+#pragma unroll
+    for (int t = 0; t < TB; ++t) gsr[t] = CORR ? aux + (int64_t)(t0 + min(t, nt - 1)) * c_ts : nullptr;
+    // 2 pi sqrt(2/(2l+1)) of the pair's degrees (LDS after the recurrence constants)
+    double *cstw = cst + 3 * (bw + 2);
     // degrees past the slice are read clamped to its last entry and enter with d = 0
     auto load = [&](double2 (&dst)[TB], int j0) {
-        const int j = min(j0 + kh, L - 1) - kh;
+        if constexpr (CORR) {
+            const int lr = min(j0 + kh, L - 1), l = q + lr, ix = l * (l + 1);
+            const double wl = l <= lcap ? cstw[lr] : 0.0;
+            const double wn = ((vk.m + vk.n) & 1) ? -wl : wl;
 #pragma unroll
-        for (int t = 0; t < TB; ++t) dst[t] = src[t][j];
+            for (int t = 0; t < TB; ++t) {
+                const double2 a = src[t][ix + vk.m], b = gsr[t][ix + vk.n];
+                const double re = a.x * b.x + a.y * b.y, im = a.y * b.x - a.x * b.y;
+                dst[t] = make_double2(wn * re, wn * im);
+            }
+        } else {
+            const int j = min(j0 + kh, L - 1) - kh;
+#pragma unroll
+            for (int t = 0; t < TB; ++t) dst[t] = src[t][j];
+        }
     };
 
     if (!TAB)
@@ -192,6 +216,11 @@ __global__ __launch_bounds__(64) void dwt_inverse_valu(const double2 *__restrict
             rec_consts_v(q + i, p, q, a, x, b);
             cst[3 * i] = a; cst[3 * i + 1] = x; cst[3 * i + 2] = b;
         }
+    if (CORR) {
+        for (int i = lane; i < L; i += 64)
+            cstw[i] = 2.0 * 3.14159265358979323846 * sqrt(2.0 / (double)(2 * (q + i) + 1));
+        __syncthreads();   // the first loads read them
+    }
     const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
     double dcur = (!TAB && beta < N) ? seeds[tri * N + beta] : 0.0;
     const double cb = (!TAB && beta < N) ? cosb[beta] : 0.0;
@@ -540,31 +569,38 @@ static int valu_tb_env()
 static int grid_of(int nunit, int nch) { return (nunit + 7) / 8 * 8 * nch; }
 
 hipError_t launch_dwt_inverse_valu(const DeviceTables &tb, int bw, int N, int npair, int mode, const double2 *coeff,
-                                   int64_t c_ts, double2 *F, int64_t f_ts, int T, hipStream_t s)
+                                   int64_t c_ts, const double2 *aux, int lcap, double2 *F, int64_t f_ts, int T,
+                                   hipStream_t s)
 {
-    if (mode == DWT_CORR) return hipErrorInvalidValue;  // staged by launch_corr_coeffs
     int TBv = T >= 4 ? 4 : (T >= 2 ? 2 : 1);
     if (valu_tb_env()) TBv = valu_tb_env();
     const int nch = (N + 63) / 64;
-    const size_t cst_bytes = sizeof(double) * 3 * (size_t)(bw + 2);
-#define SO3_INV(TBV)                                                                                              \
+    // recurrence constants, then (correlation) the degree weights
+    const bool corr = mode == DWT_CORR;
+    const size_t cst_bytes = sizeof(double) * (3 * (size_t)(bw + 2) + (corr ? (size_t)bw : 0));
+    const size_t tab_bytes = corr ? cst_bytes : 0;
+    const int kmode = corr ? (int)DWT_COMPLEX : mode;   // the products land in the complex slices
+#define SO3_INV(TBV, CR)                                                                                          \
     do {                                                                                                          \
         const int ngr = (T + TBV - 1) / TBV;                                                                      \
         const int nunit = npair * ngr;                                                                            \
         const int nunit_t = (npair + 7) / 8 * 8 * ngr;   /* table order: 8 pairs x groups per slot run */       \
         if (tb.dtab)                                                                                              \
-            hipLaunchKernelGGL((dwt_inverse_valu<TBV, true>), dim3(grid_of(nunit_t, nch)), dim3(64), 0, s, coeff, \
-                               c_ts, F, f_ts, T, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, mode, npair, nch,  \
-                               nunit_t, tb.dtab);                                                                 \
+            hipLaunchKernelGGL((dwt_inverse_valu<TBV, true, CR>), dim3(grid_of(nunit_t, nch)), dim3(64),          \
+                               tab_bytes, s, coeff, c_ts, F, f_ts, T, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw,  \
+                               N, kmode, npair, nch, nunit_t, tb.dtab, aux, lcap);                                \
         else                                                                                                      \
-            hipLaunchKernelGGL((dwt_inverse_valu<TBV, false>), dim3(grid_of(nunit, nch)), dim3(64), cst_bytes, s, \
-                               coeff, c_ts, F, f_ts, T, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw, N, mode, npair, \
-                               nch, nunit, nullptr);                                                              \
+            hipLaunchKernelGGL((dwt_inverse_valu<TBV, false, CR>), dim3(grid_of(nunit, nch)), dim3(64),           \
+                               cst_bytes, s, coeff, c_ts, F, f_ts, T, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw,  \
+                               N, kmode, npair, nch, nunit, nullptr, aux, lcap);                                  \
     } while (0)
-    switch (TBv) {
-    case 1: SO3_INV(1); break;
-    case 2: SO3_INV(2); break;
-    default: SO3_INV(4); break;
+    switch (TBv * 2 + corr) {
+    case 2: SO3_INV(1, false); break;
+    case 3: SO3_INV(1, true); break;
+    case 4: SO3_INV(2, false); break;
+    case 5: SO3_INV(2, true); break;
+    case 9: SO3_INV(4, true); break;
+    default: SO3_INV(4, false); break;
     }
 #undef SO3_INV
     return hipGetLastError();

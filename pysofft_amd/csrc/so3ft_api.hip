@@ -422,17 +422,17 @@ int ensure_pstage(Plan *P, int64_t T)
 }
 
 // inverse DWT of T transforms into F on the plan's engine; DWT_CORR takes SH pairs
-// (c, aux) with stride c_ts and stages their mnl correlation coefficients first
+// (c, aux) with stride c_ts, whose correlation coefficients the kernels form as they load
 int dwt_inverse(Plan *P, int mode, const double2 *c, int64_t c_ts, const double2 *aux, double2 *F, int64_t f_ts,
                 int64_t T, hipStream_t s)
 {
     const int bw = (int)P->bw, N = (int)P->N, np = (int)P->npair;
-    // the MFMA engine forms correlation coefficients in its kernel; the other paths (and
-    // handles that drop degrees above lmax) stage them first
+    // both engines form correlation coefficients as their kernels load them; the MFMA
+    // engine stages them first for handles that drop degrees above lmax
     // table plans read d on the default engine's kernels; the MFMA engine's LDS holds the
     // recurrence constants of bw <= 256
     const bool mfma = P->dwt_engine == SO3FT_DWT_MFMA && !P->tb.dtab && P->bw <= 256;
-    if (mode == DWT_CORR && (!mfma || P->lcap < P->bw - 1)) {
+    if (mode == DWT_CORR && mfma && P->lcap < P->bw - 1) {
         if (int rc = ensure_cstage(P, T)) return rc;
         HIPCHK(launch_corr_coeffs(P->tb, bw, c, aux, c_ts, 0, 1, 0, nullptr, P->cstage, P->nc, (int)P->lcap, (int)T,
                                   s));
@@ -442,7 +442,7 @@ int dwt_inverse(Plan *P, int mode, const double2 *c, int64_t c_ts, const double2
         HIPCHK(launch_dwt_inverse(P->tb, bw, N, np, mode, c, c_ts, aux, F, f_ts, (int)T, s));
         return SO3FT_OK;
     }
-    HIPCHK(launch_dwt_inverse_valu(P->tb, bw, N, np, mode, c, c_ts, F, f_ts, (int)T, s));
+    HIPCHK(launch_dwt_inverse_valu(P->tb, bw, N, np, mode, c, c_ts, aux, (int)P->lcap, F, f_ts, (int)T, s));
     return SO3FT_OK;
 }
 
@@ -1163,7 +1163,7 @@ int so3ft_bench_kernel(so3ft_handle h, int which, int reps, void *a, void *b, in
                                       P->nc, (int)T, s);
         case 4:
             return launch_dwt_inverse_valu(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX,
-                                           reinterpret_cast<const double2 *>(a), P->nc,
+                                           reinterpret_cast<const double2 *>(a), P->nc, nullptr, 0,
                                            reinterpret_cast<double2 *>(b), N3, (int)T, s);
         case 5:   // the default FP64 forward (complex mode)
             return launch_dwt_forward_m64(P->tb, (int)P->bw, (int)N, (int)P->npair, DWT_COMPLEX,

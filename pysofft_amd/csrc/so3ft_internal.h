@@ -102,10 +102,11 @@ hipError_t launch_dwt_forward(const DeviceTables &t, int bw, int N, int npair, i
                               hipStream_t s);
 
 // FP64 on the vector ALUs (lane = beta for the inverse, lane = degree for the forward)
-// (modes DWT_COMPLEX and DWT_REAL; correlation coefficients are staged by launch_corr_coeffs)
+// (modes DWT_COMPLEX and DWT_REAL; DWT_CORR forms the correlation coefficients of the SH
+// pairs coeff (f) and aux (g), LMc sets at stride c_ts, as it loads them, degrees > lcap zero)
 hipError_t launch_dwt_inverse_valu(const DeviceTables &t, int bw, int N, int npair, int mode,
-                                   const double2 *coeff, int64_t c_ts, double2 *F, int64_t f_ts, int T,
-                                   hipStream_t s);
+                                   const double2 *coeff, int64_t c_ts, const double2 *aux, int lcap,
+                                   double2 *F, int64_t f_ts, int T, hipStream_t s);
 hipError_t launch_dwt_forward_valu(const DeviceTables &t, int bw, int N, int npair, int mode,
                                    const double2 *F, int64_t f_ts, double2 *coeff, int64_t c_ts, int T,
                                    hipStream_t s);
