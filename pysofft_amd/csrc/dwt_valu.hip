@@ -47,6 +47,11 @@ __device__ inline void rec_consts_v(int l, int p, int q, double &a, double &x, d
     }
 }
 
+// the plan's recurrence-constant table, read-only while a kernel runs: in the constant
+// address space a load whose index is the same in every lane is a scalar load
+typedef const __attribute__((address_space(4))) double *cdptr;
+typedef const __attribute__((address_space(4))) int64_t *ci64ptr;
+
 __device__ __forceinline__ int64_t slice_v(const int64_t *slice, int m, int n, int bw)
 {
     return slice[(int64_t)(m + bw - 1) * (2 * bw - 1) + (n + bw - 1)];
@@ -146,11 +151,12 @@ __global__ __launch_bounds__(64) void dwt_inverse_valu(const double2 *__restrict
                                                        const int32_t *__restrict__ pairs, int bw, int N, int mode,
                                                        int npair, int nch, int nunit,
                                                        const double *__restrict__ dtab,
-                                                       const double2 *__restrict__ aux, int lcap)
+                                                       const double2 *__restrict__ aux, int lcap,
+                                                       const double *__restrict__ rc,
+                                                       const int64_t *__restrict__ rc_off)
 {
-    // recurrence constants (dynamic LDS, 3 (bw + 2) doubles); the last step of a tail
-    // reads up to entry 3 L + 2
-    extern __shared__ double cst[];
+    // CORR: the degree weights (dynamic LDS, bw doubles)
+    extern __shared__ double cstw[];
     int unit, chunk;
     if (!decode_unit(nch, nunit, unit, chunk)) return;
     const int lane = threadIdx.x;
@@ -189,8 +195,7 @@ __global__ __launch_bounds__(64) void dwt_inverse_valu(const double2 *__restrict
     const double2 *gsr[TB];
 #pragma unroll
     for (int t = 0; t < TB; ++t) gsr[t] = CORR ? aux + (int64_t)(t0 + min(t, nt - 1)) * c_ts : nullptr;
-    // 2 pi sqrt(2/(2l+1)) of the pair's degrees (LDS after the recurrence constants)
-    double *cstw = cst + 3 * (bw + 2);
+    // 2 pi sqrt(2/(2l+1)) of the pair's degrees in cstw
     // degrees past the slice are read clamped to its last entry and enter with d = 0
     auto load = [&](double2 (&dst)[TB], int j0) {
         if constexpr (CORR) {
@@ -210,18 +215,15 @@ __global__ __launch_bounds__(64) void dwt_inverse_valu(const double2 *__restrict
         }
     };
 
-    if (!TAB)
-        for (int i = lane; i < L - 1; i += 64) {
-            double a, x, b;
-            rec_consts_v(q + i, p, q, a, x, b);
-            cst[3 * i] = a; cst[3 * i + 1] = x; cst[3 * i + 2] = b;
-        }
     if (CORR) {
         for (int i = lane; i < L; i += 64)
             cstw[i] = 2.0 * 3.14159265358979323846 * sqrt(2.0 / (double)(2 * (q + i) + 1));
         __syncthreads();   // the first loads read them
     }
     const int64_t tri = (int64_t)p * (2 * bw - p + 1) / 2 + (q - p);
+    // the pair's recurrence constants (a, x, b) from the plan table, zero past the last
+    // degree: the same entry in every lane, so scalar loads
+    const cdptr rcp = TAB ? nullptr : (cdptr)rc + 3 * ((ci64ptr)rc_off)[tri];
     double dcur = (!TAB && beta < N) ? seeds[tri * N + beta] : 0.0;
     const double cb = (!TAB && beta < N) ? cosb[beta] : 0.0;
     double dprev = 0.0;
@@ -252,8 +254,8 @@ __global__ __launch_bounds__(64) void dwt_inverse_valu(const double2 *__restrict
     __syncthreads();
 
     // one step = degrees j0, j0+1 from buf[u]; the recurrence always advances twice
-    // (past the last degree it reads in-bounds LDS beyond the constants and the
-    // result is never used)
+    // (past the last degree it reads the table's zero padding and the result is never
+    // used)
     auto step = [&](int u, int j0, bool tail) {
         if (TAB) {
             const double d0 = dring[u][0], d1 = dring[u][1];
@@ -264,14 +266,14 @@ __global__ __launch_bounds__(64) void dwt_inverse_valu(const double2 *__restrict
         }
         const double d0 = dcur;
         {
-            const double ca = cst[3 * j0], cx = cst[3 * j0 + 1], cbb = cst[3 * j0 + 2];
+            const double ca = rcp[3 * j0], cx = rcp[3 * j0 + 1], cbb = rcp[3 * j0 + 2];
             const double dn = (ca * (cb - cx)) * dcur - cbb * dprev;
             dprev = dcur;
             dcur = dn;
         }
         const double d1 = (!tail || j0 + 1 < L) ? dcur : 0.0;
         {
-            const double ca = cst[3 * j0 + 3], cx = cst[3 * j0 + 4], cbb = cst[3 * j0 + 5];
+            const double ca = rcp[3 * j0 + 3], cx = rcp[3 * j0 + 4], cbb = rcp[3 * j0 + 5];
             const double dn = (ca * (cb - cx)) * dcur - cbb * dprev;
             dprev = dcur;
             dcur = dn;
@@ -575,10 +577,10 @@ hipError_t launch_dwt_inverse_valu(const DeviceTables &tb, int bw, int N, int np
     int TBv = T >= 4 ? 4 : (T >= 2 ? 2 : 1);
     if (valu_tb_env()) TBv = valu_tb_env();
     const int nch = (N + 63) / 64;
-    // recurrence constants, then (correlation) the degree weights
+    // dynamic LDS: the degree weights of a correlation
     const bool corr = mode == DWT_CORR;
-    const size_t cst_bytes = sizeof(double) * (3 * (size_t)(bw + 2) + (corr ? (size_t)bw : 0));
-    const size_t tab_bytes = corr ? cst_bytes : 0;
+    const size_t cst_bytes = corr ? sizeof(double) * (size_t)bw : 0;
+    const size_t tab_bytes = cst_bytes;
     const int kmode = corr ? (int)DWT_COMPLEX : mode;   // the products land in the complex slices
 #define SO3_INV(TBV, CR)                                                                                          \
     do {                                                                                                          \
@@ -588,11 +590,11 @@ hipError_t launch_dwt_inverse_valu(const DeviceTables &tb, int bw, int N, int np
         if (tb.dtab)                                                                                              \
             hipLaunchKernelGGL((dwt_inverse_valu<TBV, true, CR>), dim3(grid_of(nunit_t, nch)), dim3(64),          \
                                tab_bytes, s, coeff, c_ts, F, f_ts, T, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw,  \
-                               N, kmode, npair, nch, nunit_t, tb.dtab, aux, lcap);                                \
+                               N, kmode, npair, nch, nunit_t, tb.dtab, aux, lcap, tb.rc, tb.rc_off);              \
         else                                                                                                      \
             hipLaunchKernelGGL((dwt_inverse_valu<TBV, false, CR>), dim3(grid_of(nunit, nch)), dim3(64),           \
                                cst_bytes, s, coeff, c_ts, F, f_ts, T, tb.cosb, tb.seeds, tb.slice, tb.pairs, bw,  \
-                               N, kmode, npair, nch, nunit, nullptr, aux, lcap);                                  \
+                               N, kmode, npair, nch, nunit, nullptr, aux, lcap, tb.rc, tb.rc_off);                \
     } while (0)
     switch (TBv * 2 + corr) {
     case 2: SO3_INV(1, false); break;
