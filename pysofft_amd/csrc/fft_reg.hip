@@ -386,7 +386,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
     constexpr int RM = R1 > R2 ? R1 : R2;
     extern __shared__ double2 sm[];
     double2 *tw = sm;
-    double *Yd = reinterpret_cast<double *>(sm + N);   // [k1][n2][c]
+    double2 *Y = sm + N;   // [k1][n2][c]: the whole complex exchange (N C 16 bytes, 32 KB at N = 128)
     const int C = k.C, tid = threadIdx.x;
     const int c = tid % C, grp = tid / C;
     const int64_t t = blockIdx.z;
@@ -438,24 +438,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
             for (int k1 = 1; k1 < R1; ++k1)
                 if (n2 != 0) v[k1] = cmul2(v[k1], tw[n2 * k1]);
 #pragma unroll
-            for (int k1 = 0; k1 < R1; ++k1) Yd[(k1 * R2 + n2) * C + c] = v[k1].x;
-        }
-        __syncthreads();
-        double re[R2];
-        if (grp < R1) {
-#pragma unroll
-            for (int n2 = 0; n2 < R2; ++n2) re[n2] = Yd[(grp * R2 + n2) * C + c];
-        }
-        __syncthreads();
-        if (grp < R2) {
-#pragma unroll
-            for (int k1 = 0; k1 < R1; ++k1) Yd[(k1 * R2 + grp) * C + c] = v[k1].y;
+            for (int k1 = 0; k1 < R1; ++k1) Y[(k1 * R2 + n2) * C + c] = v[k1];
         }
         __syncthreads();
         if (grp < R1 && o < N) {
             const int k1 = grp;
 #pragma unroll
-            for (int n2 = 0; n2 < R2; ++n2) v[n2] = make_double2(re[n2], Yd[(k1 * R2 + n2) * C + c]);
+            for (int n2 = 0; n2 < R2; ++n2) v[n2] = Y[(k1 * R2 + n2) * C + c];
             Dft<R2>::run(v, tw, N / R2, k.sign);
             if (colok) {
                 const int64_t ib = (int64_t)o * k.out_o + cb + c + (int64_t)k1 * k.out_r;
@@ -631,10 +620,11 @@ hipError_t launch_fft_reg_argmax(int N, const double2 *tw, int bw, int lmax, con
     const int nouter = (N + AW - 1) / AW;
     *nblk = ntile * nouter;
     const dim3 grid(ntile, nouter, T);
+    const int wbytes = (int)(sizeof(double2) * N * (1 + k.C));   // twiddles + complex exchange
 #define SO3_AMAX(A, B)                                                                                       \
     do {                                                                                                    \
         if constexpr (A * B <= AW_MAXN)                                                                     \
-            hipLaunchKernelGGL((fft_amax_walk<A, B>), grid, block, bytes, s, in, tw, k);                     \
+            hipLaunchKernelGGL((fft_amax_walk<A, B>), grid, block, wbytes, s, in, tw, k);                    \
     } while (0)
     // the smaller radix first: every thread of the block loads rows (R2 >= R1 groups), and
     // the prefetched rows take fewer registers
