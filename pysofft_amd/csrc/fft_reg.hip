@@ -273,15 +273,26 @@ hipError_t launch_copy16(const void *a, void *b, int64_t nvec, hipStream_t s)
 // AMAX: the pass's output is not stored; each block leaves the largest |out|^2 of its tile
 // and that element's C-order index within the transform (the smallest on ties) in
 // k.pv / k.pi [transform][block], for a peak search that never writes the grid.
+//
+// Up to N = 128 the whole complex step fits (34 KB with 16 columns, still three blocks per
+// CU): one exchange barrier instead of three.
+__host__ __device__ constexpr bool lean_fullx(int N) { return N <= 128; }
+__host__ __device__ constexpr int lean_lds_bytes(int N, int C)
+{
+    return (int)(sizeof(double2) * N + (lean_fullx(N) ? sizeof(double2) : sizeof(double)) * N * C);
+}
+
 template <int R1, int R2, class TI, class TO, bool AMAX = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void fft_reg_lean(
     const void *__restrict__ in, void *__restrict__ out, const double2 *__restrict__ twg, FftRegK k)
 {
     constexpr int N = R1 * R2;
     constexpr int RM = R1 > R2 ? R1 : R2;
+    constexpr bool FX = lean_fullx(N);
     extern __shared__ double2 sm[];
     double2 *tw = sm;
     double *Yd = reinterpret_cast<double *>(sm + N);   // [k1][n2][c]
+    double2 *Y = sm + N;                               // FX: [k1][n2][c] complex
     const int C = k.C, tid = threadIdx.x;
     const int c = tid % C, grp = tid / C;
     const int64_t o = blockIdx.y, t = blockIdx.z;
@@ -314,27 +325,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
 #pragma unroll
         for (int k1 = 1; k1 < R1; ++k1)
             if (n2 != 0) v[k1] = cmul2(v[k1], tw[n2 * k1]);
+        if constexpr (FX) {
 #pragma unroll
-        for (int k1 = 0; k1 < R1; ++k1) Yd[(k1 * R2 + n2) * C + c] = v[k1].x;
+            for (int k1 = 0; k1 < R1; ++k1) Y[(k1 * R2 + n2) * C + c] = v[k1];
+        } else {
+#pragma unroll
+            for (int k1 = 0; k1 < R1; ++k1) Yd[(k1 * R2 + n2) * C + c] = v[k1].x;
+        }
     }
     __syncthreads();
     double re[R2];
-    if (grp < R1) {
+    if constexpr (!FX) {
+        if (grp < R1) {
 #pragma unroll
-        for (int n2 = 0; n2 < R2; ++n2) re[n2] = Yd[(grp * R2 + n2) * C + c];
-    }
-    __syncthreads();
-    if (grp < R2) {
+            for (int n2 = 0; n2 < R2; ++n2) re[n2] = Yd[(grp * R2 + n2) * C + c];
+        }
+        __syncthreads();
+        if (grp < R2) {
 #pragma unroll
-        for (int k1 = 0; k1 < R1; ++k1) Yd[(k1 * R2 + grp) * C + c] = v[k1].y;
+            for (int k1 = 0; k1 < R1; ++k1) Yd[(k1 * R2 + grp) * C + c] = v[k1].y;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     double bv = -1.0;
     int64_t bi = INT64_MAX;
     if (grp < R1) {
         const int k1 = grp;
 #pragma unroll
-        for (int n2 = 0; n2 < R2; ++n2) v[n2] = make_double2(re[n2], Yd[(k1 * R2 + n2) * C + c]);
+        for (int n2 = 0; n2 < R2; ++n2)
+            v[n2] = FX ? Y[(k1 * R2 + n2) * C + c] : make_double2(re[n2], Yd[(k1 * R2 + n2) * C + c]);
         Dft<R2>::run(v, tw, N / R2, k.sign);
         if (AMAX && colok) {
             const int64_t ib = o * k.out_o + cb + c + (int64_t)k1 * k.out_r;
@@ -539,7 +558,7 @@ static hipError_t launch_reg_typed(int N, const double2 *tw, int bw, int lmax, c
     // one tile per block with the exchange in halves: complex128 input up to N = 256,
     // complex64 input up to N = 192 (not for real input, read as scalars by the kernel above)
     const bool lean = !ax.in_real && (sizeof(TI) == 16 ? N <= 256 : N <= 192);
-    const int lean_bytes = (int)(sizeof(double2) * N + sizeof(double) * N * k.C);
+    const int lean_bytes = lean_lds_bytes(N, k.C);
 #define SO3_REG(A, B)                                                                                   \
     do {                                                                                                \
         static bool attr = false;                                                                       \
@@ -606,7 +625,7 @@ hipError_t launch_fft_reg_argmax(int N, const double2 *tw, int bw, int lmax, con
     k.pv = pv;
     k.pi = pi;
     const int ntile = (N + k.C - 1) / k.C;
-    const int bytes = (int)(sizeof(double2) * N + sizeof(double) * N * k.C);
+    const int bytes = lean_lds_bytes(N, k.C);
     const dim3 block((R1 > R2 ? R1 : R2) * k.C);
     if (N > AW_MAXN) {
         *nblk = ntile * N;
