@@ -419,6 +419,23 @@ def test_cross_correlation_many_vs_oracle(engine, n):
         assert rel_l2(got[i], P.cross_correlation_ylm_cmplx(f[i], g[i])) < 1e-12, i
 
 
+@pytest.mark.parametrize("bw", [1, 2, 3])
+@pytest.mark.parametrize("engine", ["valu", "mfma"])
+def test_cross_correlation_smallest_bandwidths_vs_oracle(bw, engine):
+    """Correlation at the smallest bandwidths (one to three degrees, chains of one degree) vs
+    the oracle, single and batched (three pairs: one wave's tail)."""
+    rng = np.random.default_rng(90 + bw)
+    s = _soft(bw)
+    s.dwt_engine = engine
+    P = O.OraclePlan(bw)
+    f = _rand_c(rng, (3, bw * bw))
+    g = _rand_c(rng, (3, bw * bw))
+    assert rel_l2(s.cross_correlation_ylm_cmplx(f[0], g[0]), P.cross_correlation_ylm_cmplx(f[0], g[0])) < 1e-12
+    got = np.asarray(s.cross_correlation_ylm_cmplx_many(f, g))
+    for i in range(3):
+        assert rel_l2(got[i], P.cross_correlation_ylm_cmplx(f[i], g[i])) < 1e-12, i
+
+
 def test_correlation_peaks_find_grid_rotations():
     """g = exact Wigner-D rotation of f by a grid rotation: the fused correlation + peak
     search returns that rotation's grid indices (the reference's own check,
