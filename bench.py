@@ -128,7 +128,8 @@ def ragged_batch(args, dev, timed):
     """Grouped mixed-precision isoft+soft of transforms with bandwidths drawn uniformly
     from {32, 48, 64, 96, 128}: one grouped launch per direction (tuned launch shapes,
     replayed HIP graph) against the same transforms run bandwidth by bandwidth through
-    the single-bandwidth complex64 path, and per-transform relative L2 vs FP64."""
+    the single-bandwidth complex64 path and the FP64 path, and per-transform relative L2 vs
+    FP64."""
     import torch
     from pysofft_amd import Soft
     from pysofft_amd.grouped import SoftGroup
@@ -167,6 +168,23 @@ def ragged_batch(args, dev, timed):
     step_single()
     ms1, _ = timed(step_single, k)
     ms1 /= k
+    # the same transforms on the FP64 path, bandwidth by bandwidth (isoft_many + soft_many)
+    segs64 = []
+    for s, c, f, r in segs:
+        c128 = c.to(torch.complex128)
+        segs64.append((s, c128, torch.empty(tuple(f.shape), dtype=torch.complex128, device=dev),
+                       torch.empty_like(c128)))
+
+    def step_fp64():
+        for s, c, f, r in segs64:
+            s.isoft_many(c, out=f)
+            s.soft_many(f, out=r)
+
+    step_fp64()
+    k64 = max(1, k // 2)
+    ms64, _ = timed(step_fp64, k64)
+    ms64 /= k64
+    del segs64
     step()
     torch.cuda.synchronize()
     plans = {s.bw: s for s, _, _, _ in segs}
@@ -199,12 +217,14 @@ def ragged_batch(args, dev, timed):
         del cb, fb, rb, s64
     ideal = sum(per_tf[str(b)] for b in bws)
     print(f"# ragged: grouped {ms:.3f} ms/step, bandwidth by bandwidth {ms1:.3f} ms/step, "
-          f"sum of full-batch per-transform costs {ideal:.3f} ms", file=sys.stderr, flush=True)
+          f"sum of full-batch per-transform costs {ideal:.3f} ms, FP64 bandwidth by bandwidth {ms64:.3f} ms/step",
+          file=sys.stderr, flush=True)
     return {"workload": f"{len(bws)} complex64 transforms, bw uniform over {{32,48,64,96,128}}, coefficients iid "
                         f"complex N(0,1); isoft+soft",
             "bw_counts": {str(b): bws.count(b) for b in sorted(set(bws))},
             "ms_per_step": ms, "pairs_per_s": len(bws) / (ms * 1e-3),
             "bandwidth_by_bandwidth_ms_per_step": ms1, "grouped_speedup": ms1 / ms,
+            "fp64_ms_per_step": ms64, "fp64_pairs_per_s": len(bws) / (ms64 * 1e-3), "mixed_over_fp64": ms64 / ms,
             "full_batch_ms_per_transform": per_tf, "ideal_ms_per_step": ideal,
             "speedup_bound_over_bandwidth_by_bandwidth": ms1 / ideal,
             "relL2_vs_fp64_max": {"isoft": e[0], "soft": e[1], "round_trip": e[2]},
