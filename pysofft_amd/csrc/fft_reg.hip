@@ -274,9 +274,9 @@ hipError_t launch_copy16(const void *a, void *b, int64_t nvec, hipStream_t s)
 // and that element's C-order index within the transform (the smallest on ties) in
 // k.pv / k.pi [transform][block], for a peak search that never writes the grid.
 //
-// Up to N = 128 the whole complex step fits (34 KB with 16 columns, still three blocks per
+// Up to N = 192 the whole complex step fits (52 KB with 16 columns, still three blocks per
 // CU): one exchange barrier instead of three.
-__host__ __device__ constexpr bool lean_fullx(int N) { return N <= 128; }
+__host__ __device__ constexpr bool lean_fullx(int N) { return N <= 192; }
 __host__ __device__ constexpr int lean_lds_bytes(int N, int C)
 {
     return (int)(sizeof(double2) * N + (lean_fullx(N) ? sizeof(double2) : sizeof(double)) * N * C);
